@@ -1,0 +1,205 @@
+/*
+ * paratc_gpu.h -- C-ABI of the MI355X (gfx950) implementation of ParaTC's
+ * per-time-step hot path: the m_Poisson pressure solve, the m_TScheme stencils
+ * and wall-normal Crank-Nicolson solves, and the m_BC_and_Halo boundary fills.
+ *
+ * Design of the boundary
+ *  - One context per case (ptc_create) holds what the reference keeps in module
+ *    globals (m_Parameters, m_MeshAndMetries, m_Variables): grid, physics,
+ *    y-metrics and the device-resident field arrays.
+ *  - Fields live in HBM and are named by PTC_F_* ids, so the caller can alias
+ *    buffers exactly as the reference does (RhsX => RealArr1, prphiHalo =>
+ *    RealHalo, src/m_ChannelSystem.f90:108,124).  ptc_upload / ptc_download move
+ *    a field between the host and the device in the reference's own layouts.
+ *  - Every routine returns 0 on success and a negative PTC_E_* code on failure;
+ *    ptc_last_error() returns the message.  The reference aborts instead
+ *    (MainLog%CheckForError, src/m_LogInfo.f90:164-170); a host maps a nonzero
+ *    code to that abort.
+ *  - All work is ordered on the context's stream.  Only the calls that return a
+ *    host scalar (monitors, divmax) synchronise.
+ *  - Plain C types only: ints, doubles, pointers, sizes.
+ */
+#ifndef PARATC_GPU_H
+#define PARATC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTC_ABI_VERSION 1
+
+/* error codes */
+#define PTC_OK 0
+#define PTC_E_ARG (-1)         /* bad argument (size, id, layout, option) */
+#define PTC_E_HIP (-2)         /* a HIP runtime call failed */
+#define PTC_E_ALLOC (-3)       /* device allocation failed */
+#define PTC_E_STATE (-4)       /* call out of order (e.g. no Poisson plan) */
+#define PTC_E_UNSUPPORTED (-5) /* grid or option outside the built kernels */
+#define PTC_E_LIMIT (-6)       /* div_limit / vel_limit guard tripped */
+
+/* field ids: the arrays of src/m_Variables.f90:10-34 */
+enum {
+  PTC_F_UX = 0,   /* ux        haloed */
+  PTC_F_UY,       /* uy        haloed */
+  PTC_F_UZ,       /* uz        haloed */
+  PTC_F_PR,       /* pressure  haloed */
+  PTC_F_SCALAR,   /* scalar    haloed (ScalarFlow) */
+  PTC_F_HISTX,    /* HistxOld  interior */
+  PTC_F_HISTY,    /* HistyOld  interior */
+  PTC_F_HISTZ,    /* HistzOld  interior */
+  PTC_F_HISTC,    /* HistCOld  interior (ScalarFlow) */
+  PTC_F_ARR1,     /* RealArr1  interior (RhsX, prsrc) */
+  PTC_F_ARR2,     /* RealArr2  interior (RhsY) */
+  PTC_F_HALO,     /* RealHalo  haloed   (RhsZ, prphiHalo) */
+  PTC_F_ARRC,     /* RealArrC  haloed   (RhsC, ScalarFlow) */
+  PTC_F_WORK1,    /* spare work arrays for callers */
+  PTC_F_WORK2,
+  PTC_F_UY_YM,    /* uy_ym   wall plane (nx, nz) */
+  PTC_F_DUY_YM,   /* duy_ym  wall plane (nx, nz) */
+  PTC_NFIELDS
+};
+
+/* host layouts for upload/download (Fortran column-major FP64) */
+enum {
+  PTC_L_HALO = 0,     /* (nx+4)*(ny+2)*(nz+4): x -1..nx+2, y 0..ny+1, z -1..nz+2 (mb1) */
+  PTC_L_INTERIOR = 1, /* nx*ny*nz: x 1..nx, y 1..ny, z 1..nz (y1start:y1end) */
+  PTC_L_PLANE = 2     /* nx*nz (uy_ym, duy_ym) */
+};
+
+/* Poisson solve precision / variant flags (ptc_poisson_config) */
+enum {
+  PTC_PREC_FP64 = 0,        /* plain FP64 */
+  PTC_PREC_FP32_REFINE = 1  /* FP32 bulk + FP64 defect correction */
+};
+enum {
+  PTC_TRI_AUTO = 0,
+  PTC_TRI_THOMAS = 1,       /* one thread per line, the reference's InverseTridiagonal */
+  PTC_TRI_PARTITIONED = 2   /* register-resident chunks + reduced system (Choice-3 idea) */
+};
+
+/* scalar ids for ptc_get_scalar */
+enum {
+  PTC_S_DIVMAX_PRSRC = 0, /* last clcPrSrc max|div| */
+  PTC_S_PRGRADAVER = 1,   /* PrGradAver (src/m_TScheme.f90:284,298) */
+  PTC_S_SIMTIME = 2,      /* SimTime */
+  PTC_S_DT = 3            /* current dt */
+};
+
+typedef struct ptc_ctx ptc_ctx;
+
+/* Grid: nxc, nyc, nzc and domain lengths (src/m_Parameters.f90:17-19). */
+typedef struct ptc_grid {
+  int nx, ny, nz, pad_;
+  double xlx, yly, zlz;
+} ptc_grid;
+
+/* Physics and options: the BasicParam / ScalarFlowOptions keys the hot path
+ * reads (src/m_Parameters.f90:118-129), after the derivations of
+ * src/m_Parameters.f90:153-187 (uCRF shift of the wall values, SDiffCoe,
+ * GravityEffVec, FallingVelVec). */
+typedef struct ptc_params {
+  int flow_type;    /* 1 channel, 2 half channel */
+  int scalar;       /* 1 = the -DScalarFlow variant */
+  int is_ux_const;  /* IsUxConst */
+  int ischeme;      /* 1 AB2, 2 RK2, 3 RK3 */
+  int icfl;         /* 1 adaptive dt, 2 fixed */
+  int sbc_opt[2];   /* ScalarBcOption (y-, y+) */
+  int pad_;
+  double ubulk, xnu, ucrf, dtmax, cflc;
+  double gravity[3];
+  double uxbc[6], uybc[6], uzbc[6];
+  double sdiff;
+  double gev[3];
+  double fvv[3];
+  double sbc_val[4];
+  double vel_limit, div_limit;
+} ptc_params;
+
+/* ---- context: replaces InitMeshAndMetries + AllocateVariables
+ *      (src/m_MeshAndMetries.f90:71-335, src/m_Variables.f90:41-105).
+ *      yp holds the nodes yp(1..ny+1) computed by the host's clcYcoord. */
+int ptc_create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ctx **out);
+int ptc_destroy(ptc_ctx *ctx);
+const char *ptc_last_error(void);
+int ptc_abi_version(void);
+
+/* ---- field transfer (device-resident between calls) */
+int ptc_upload(ptc_ctx *ctx, int field, const double *host, int layout);
+int ptc_download(ptc_ctx *ctx, int field, double *host, int layout);
+int ptc_fill(ptc_ctx *ctx, int field, double value);
+int ptc_copy(ptc_ctx *ctx, int dst, int src);
+/* raw device address of a field (nx*(ny+2)*nz doubles, x fastest, y rows 0..ny+1) */
+int ptc_field_device_ptr(ptc_ctx *ctx, int field, void **dptr, long long *count);
+int ptc_synchronize(ptc_ctx *ctx);
+/* y-metric table, ny+2 values (same numbering as the oracle header) */
+int ptc_get_metric(ptc_ctx *ctx, int which, double *out);
+
+/* ---- m_Parameters: PMcoeUpdate (src/m_Parameters.f90:215-230) */
+int ptc_pm_coe_update(ptc_ctx *ctx, int ns, double dt);
+/* coefficient vector [gamma theta alpha alphaC beta betaSc dt simtime prgradaver] */
+int ptc_get_coe(ptc_ctx *ctx, double *out9);
+int ptc_set_coe(ptc_ctx *ctx, const double *in9);
+
+/* ---- m_Poisson (src/m_Poisson.f90:22) */
+/* InitPoissonSolver (:28-175): builds the FFT plans and autotunes the variant
+ * (tile shapes, Thomas vs partitioned) by timing, like the reference's 4-way
+ * Choice search; leaves src and dst zeroed as the reference does (:174). */
+int ptc_init_poisson_solver(ptc_ctx *ctx, int src, int dst);
+/* clcPPE (:16, :352-452): dst interior = L^-1 src with the (kx,kz)=(0,0) line
+ * pinned at j=1.  src is preserved (the reference destroys it; callers may not
+ * rely on either). */
+int ptc_clc_ppe(ptc_ctx *ctx, int src, int dst);
+/* Destory_Poisson_FFT_Plan (:339-347) */
+int ptc_destroy_poisson_plan(ptc_ctx *ctx);
+/* explicit variant selection (overrides the autotuner); prec = PTC_PREC_*,
+ * tri = PTC_TRI_*, tile = z-FFT column tile (0 = auto), sweeps = refinement
+ * sweeps for PTC_PREC_FP32_REFINE */
+int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
+/* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
+int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
+/* per-stage kernel times in ms of the last profiled solve (5 stages) */
+int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5);
+
+/* ---- m_TScheme (src/m_TScheme.f90:14-19) */
+int ptc_init_time_scheme(ptc_ctx *ctx);
+/* clcRhsXYZ (:113-360): pass -1 for scalar/rhsc/histc when not ScalarFlow */
+int ptc_clc_rhs_xyz(ptc_ctx *ctx, int ux, int uy, int uz, int scalar, int rhsx, int rhsy,
+                    int rhsz, int rhsc, int histx, int histy, int histz, int histc,
+                    int pressure);
+int ptc_clc_u1hat(ptc_ctx *ctx, int ux, int rhsx);                /* :365-394 */
+int ptc_clc_u2hat(ptc_ctx *ctx, int uy, int rhsy, int duy_ym);    /* :399-437 */
+int ptc_clc_u3hat(ptc_ctx *ctx, int uz, int rhsz);                /* :442-471 */
+int ptc_clc_scalar(ptc_ctx *ctx, int scalar, int rhsc);           /* :477-506 */
+/* clcPrSrc (:48-77); divmax may be NULL (then no host synchronisation) */
+int ptc_clc_prsrc(ptc_ctx *ctx, int ux, int uy, int uz, int prsrc, double *divmax);
+int ptc_fluid_vel_update(ptc_ctx *ctx, int phi, int ux, int uy, int uz); /* :82-107 */
+int ptc_pressure_update(ptc_ctx *ctx, int pressure, int phi);           /* :512-532 */
+
+/* ---- m_BC_and_Halo (src/m_BC_and_Halo.f90:13-17) and Update_uy_ym
+ *      (src/m_FlowCase.f90:112-122).  x and z are periodic in the device
+ *      layout, so halo refreshes (myupdate_halo) are implicit; these set the
+ *      wall ghost rows. */
+int ptc_set_bc_and_update_halo(ptc_ctx *ctx, int ux, int uy, int uz, int uy_ym);
+int ptc_set_bc_and_update_halo_for_prsrc(ptc_ctx *ctx, int ux, int uy, int uz, int uy_ym);
+int ptc_set_bc_and_update_halo_pr(ptc_ctx *ctx, int p);
+int ptc_set_bc_and_update_halo_scalar(ptc_ctx *ctx, int scalar);
+int ptc_update_uy_ym(ptc_ctx *ctx, int uy_ym, int duy_ym);
+
+/* ---- m_Tools monitors (src/m_Tools.f90:24-110, 256-301) */
+int ptc_calc_max_cfl(ptc_ctx *ctx, int ux, int uy, int uz, double *uddxmax);
+int ptc_check_divergence(ptc_ctx *ctx, int ux, int uy, int uz, double *divmax);
+int ptc_calc_vmax(ptc_ctx *ctx, int ux, int uy, int uz, double *vmax3);
+int ptc_calc_ux_aver(ptc_ctx *ctx, int ux, double *mean);
+int ptc_clc_mean_value(ptc_ctx *ctx, int ux, int scalar, double *res3);
+
+/* ---- m_ChannelSystem: nsteps calls of ChannelIterate (src/m_ChannelSystem.f90:80-145)
+ *      on the standard buffers.  mon (may be NULL) receives, for the last step,
+ *      [divmax1, divmax2, vmax x3, cfl*dt, dt]; returns PTC_E_LIMIT when the
+ *      reference's div_limit / vel_limit guard (:142-145) would abort. */
+int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7);
+int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARATC_GPU_H */

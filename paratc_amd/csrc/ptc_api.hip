@@ -1,0 +1,563 @@
+// ptc_api.hip -- the extern "C" entry points declared in include/paratc_gpu.h.
+//
+// Each entry point validates its arguments, resolves field ids to device
+// buffers and launches on the context stream.  The sequence of
+// ptc_channel_iterate is the ChannelIterate substep loop of
+// src/m_ChannelSystem.f90:80-145, with the same buffer aliasing.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "ptc_internal.hpp"
+
+namespace ptc {
+int create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ctx **out);
+void destroy(ptc_ctx *c);
+const char *last_error();
+int upload(ptc_ctx *c, int id, const double *host, int layout);
+int download(ptc_ctx *c, int id, double *host, int layout);
+int fill_device(ptc_ctx *c, double *a, long long n, double v);
+size_t field_count(const ptc_ctx *c, int id);
+int launch_add_shift(ptc_ctx *c, double *rhsx);
+const double *flowrate_shift_ptr(ptc_ctx *c);
+int set_prgrad(ptc_ctx *c, double v);
+int get_device_scalar(ptc_ctx *c, int slot, double *out);
+int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
+                  const double *shift);
+
+__global__ void k_set_slot(double *p, double v) { p[0] = v; }
+
+// device scalar slots (ptc_stencil.hip)
+enum { S_DP1NS = 0, S_PRGRAD = 1, S_DIVMAX = 2 };
+
+static int opt_field(ptc_ctx *c, int id, double **out) {
+  if (id < 0) {
+    *out = nullptr;
+    return PTC_OK;
+  }
+  return field(c, id, out);
+}
+
+// PMcoeUpdate (src/m_Parameters.f90:215-230); the PrGradAver reset of ns == 1
+// is a device write ordered on the stream.
+static int pm_coe_update(ptc_ctx *c, int ns, double dt) {
+  PTC_CHECK(ns >= 1 && ns <= c->iadvance, PTC_E_ARG, "substep index out of range");
+  const double a = c->aconst[ns - 1];
+  c->dt = dt;
+  c->pmGamma = c->gconst[ns - 1] * dt;
+  c->pmTheta = c->tconst[ns - 1] * dt;
+  c->pmAlpha = a * dt;
+  c->pmBeta = 0.5 * a * c->prm.xnu * dt;
+  c->pmBetaSc = 0.5 * a * c->prm.sdiff * dt;
+  c->simtime = c->simtime + dt * a;
+  c->pmAlphaC = a;
+  if (ns == 1) {
+    hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, c->stream, c->d_scal + S_PRGRAD, 0.0);
+    PTC_HIP(hipGetLastError());
+  }
+  return PTC_OK;
+}
+
+// ---- InitPoissonSolver: time candidate variants, keep the fastest --------
+static int time_solve(ptc_ctx *c, const double *src, double *dst, int reps, double *ms) {
+  PTC_TRY(poisson_solve(c, src, dst));
+  hipEvent_t e0, e1;
+  PTC_HIP(hipEventCreate(&e0));
+  PTC_HIP(hipEventCreate(&e1));
+  PTC_HIP(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) PTC_TRY(poisson_solve(c, src, dst));
+  PTC_HIP(hipEventRecord(e1, c->stream));
+  PTC_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  PTC_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return PTC_OK;
+}
+
+static int init_poisson(ptc_ctx *c, int src, int dst) {
+  double *s = nullptr, *d = nullptr;
+  PTC_TRY(field(c, src, &s));
+  PTC_TRY(field(c, dst, &d));
+  struct Cand {
+    int tri, tile;
+  };
+  std::vector<Cand> cands;
+  for (int tri : {PTC_TRI_PARTITIONED, PTC_TRI_THOMAS})
+    for (int tile : {16, 8, 4}) cands.push_back({tri, tile});
+  double best = 1e300;
+  Cand pick{PTC_TRI_PARTITIONED, 0};
+  for (const Cand &cd : cands) {
+    poisson_free_plan(c);
+    c->pp.tri = cd.tri;
+    c->pp.ztile = cd.tile;
+    if (poisson_build_plan(c) != PTC_OK) continue;
+    if (c->pp.ztile != cd.tile) continue;  // tile clipped by LDS: already covered
+    double ms = 0;
+    if (time_solve(c, s, d, 3, &ms) != PTC_OK) continue;
+    if (ms < best) {
+      best = ms;
+      pick = cd;
+    }
+  }
+  poisson_free_plan(c);
+  c->pp.tri = pick.tri;
+  c->pp.ztile = pick.tile;
+  PTC_TRY(poisson_build_plan(c));
+  // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
+  PTC_TRY(fill_device(c, s, (long long)c->n3, 0.0));
+  PTC_TRY(fill_device(c, d, (long long)c->n3, 0.0));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  return PTC_OK;
+}
+
+// ---- one time step (ChannelIterate) ---------------------------------------
+static int channel_iterate(ptc_ctx *c, double *mon) {
+  double *ux, *uy, *uz, *p, *hx, *hy, *hz, *a1, *a2, *hl, *uym, *duym;
+  double *sc = nullptr, *hc = nullptr, *ac = nullptr;
+  PTC_TRY(field(c, PTC_F_UX, &ux));
+  PTC_TRY(field(c, PTC_F_UY, &uy));
+  PTC_TRY(field(c, PTC_F_UZ, &uz));
+  PTC_TRY(field(c, PTC_F_PR, &p));
+  PTC_TRY(field(c, PTC_F_HISTX, &hx));
+  PTC_TRY(field(c, PTC_F_HISTY, &hy));
+  PTC_TRY(field(c, PTC_F_HISTZ, &hz));
+  PTC_TRY(field(c, PTC_F_ARR1, &a1));
+  PTC_TRY(field(c, PTC_F_ARR2, &a2));
+  PTC_TRY(field(c, PTC_F_HALO, &hl));
+  PTC_TRY(field(c, PTC_F_UY_YM, &uym));
+  PTC_TRY(field(c, PTC_F_DUY_YM, &duym));
+  const bool scal = c->prm.scalar != 0;
+  if (scal) {
+    PTC_TRY(field(c, PTC_F_SCALAR, &sc));
+    PTC_TRY(field(c, PTC_F_HISTC, &hc));
+    PTC_TRY(field(c, PTC_F_ARRC, &ac));
+  }
+  // CalcMaxCFL and dt (:93-100)
+  double cflmp = 0.0, dt = c->prm.dtmax;
+  if (c->prm.icfl == 1 || mon) {
+    double uddxmax = 0.0;
+    PTC_TRY(launch_monitor(c, 0, ux, uy, uz, nullptr, &uddxmax));
+    uddxmax *= 0.5;
+    cflmp = uddxmax * c->dt;
+    if (c->prm.icfl == 1) dt = std::min(c->prm.cflc / uddxmax, c->prm.dtmax);
+  }
+  for (int ns = 1; ns <= c->iadvance; ++ns) {
+    PTC_TRY(pm_coe_update(c, ns, dt));
+    PTC_TRY(launch_uy_ym(c, uym, duym));
+    PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p));
+    if (scal) PTC_TRY(launch_tri_cn(c, 2, sc, ac, nullptr, c->pmBetaSc, nullptr));
+    PTC_TRY(launch_tri_cn(c, 0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr));
+    PTC_TRY(launch_tri_cn(c, 1, uy, a2, duym, c->pmBeta, nullptr));
+    PTC_TRY(launch_tri_cn(c, 0, uz, hl, nullptr, c->pmBeta, nullptr));
+    PTC_TRY(launch_bc_vel(c, ux, uy, uz, uym, true));
+    PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr));
+    PTC_TRY(poisson_solve(c, a1, hl));
+    PTC_TRY(launch_bc_pr(c, hl));
+    PTC_TRY(launch_vel_p_update(c, hl, ux, uy, uz, p));
+    PTC_TRY(launch_bc_vel(c, ux, uy, uz, uym, false));
+    PTC_TRY(launch_bc_pr(c, p));
+    if (scal) PTC_TRY(launch_bc_scalar(c, sc));
+  }
+  if (mon) {
+    PTC_TRY(get_device_scalar(c, S_DIVMAX, &mon[0]));
+    c->divmax_prsrc = mon[0];
+    PTC_TRY(launch_monitor(c, 1, ux, uy, uz, nullptr, &mon[1]));
+    PTC_TRY(launch_monitor(c, 2, ux, uy, uz, nullptr, &mon[2]));
+    PTC_TRY(launch_monitor(c, 3, ux, uy, uz, nullptr, &mon[3]));
+    PTC_TRY(launch_monitor(c, 4, ux, uy, uz, nullptr, &mon[4]));
+    mon[5] = cflmp;
+    mon[6] = dt;
+  }
+  return PTC_OK;
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+#define PTC_CTX(c)                                                                                   \
+  do {                                                                                               \
+    if (!(c)) {                                                                                      \
+      set_error("null context");                                                                     \
+      return PTC_E_ARG;                                                                              \
+    }                                                                                                \
+  } while (0)
+
+extern "C" {
+
+const char *ptc_last_error(void) { return ptc::last_error(); }
+int ptc_abi_version(void) { return PTC_ABI_VERSION; }
+
+int ptc_create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ctx **out) {
+  return ptc::create(grid, prm, yp, out);
+}
+
+int ptc_destroy(ptc_ctx *ctx) {
+  ptc::destroy(ctx);
+  return PTC_OK;
+}
+
+int ptc_upload(ptc_ctx *ctx, int f, const double *host, int layout) {
+  PTC_CTX(ctx);
+  return ptc::upload(ctx, f, host, layout);
+}
+
+int ptc_download(ptc_ctx *ctx, int f, double *host, int layout) {
+  PTC_CTX(ctx);
+  return ptc::download(ctx, f, host, layout);
+}
+
+int ptc_fill(ptc_ctx *ctx, int f, double value) {
+  PTC_CTX(ctx);
+  double *d;
+  PTC_TRY(field(ctx, f, &d));
+  return fill_device(ctx, d, (long long)field_count(ctx, f), value);
+}
+
+int ptc_copy(ptc_ctx *ctx, int dst, int src) {
+  PTC_CTX(ctx);
+  double *d, *s;
+  PTC_TRY(field(ctx, dst, &d));
+  PTC_TRY(field(ctx, src, &s));
+  PTC_CHECK(field_count(ctx, dst) == field_count(ctx, src), PTC_E_ARG, "ptc_copy: field shapes differ");
+  PTC_HIP(hipMemcpyAsync(d, s, sizeof(double) * field_count(ctx, src), hipMemcpyDeviceToDevice, ctx->stream));
+  return PTC_OK;
+}
+
+int ptc_field_device_ptr(ptc_ctx *ctx, int f, void **dptr, long long *count) {
+  PTC_CTX(ctx);
+  PTC_CHECK(dptr != nullptr, PTC_E_ARG, "null output pointer");
+  double *d;
+  PTC_TRY(field(ctx, f, &d));
+  *dptr = d;
+  if (count) *count = (long long)field_count(ctx, f);
+  return PTC_OK;
+}
+
+int ptc_synchronize(ptc_ctx *ctx) {
+  PTC_CTX(ctx);
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
+}
+
+int ptc_get_metric(ptc_ctx *ctx, int which, double *out) {
+  PTC_CTX(ctx);
+  PTC_CHECK(which >= 0 && which < M_NMET && out, PTC_E_ARG, "metric index out of range");
+  const size_t W = (size_t)ctx->ny + 2;
+  std::copy(ctx->h_met.begin() + which * W, ctx->h_met.begin() + (which + 1) * W, out);
+  return PTC_OK;
+}
+
+int ptc_pm_coe_update(ptc_ctx *ctx, int ns, double dt) {
+  PTC_CTX(ctx);
+  return pm_coe_update(ctx, ns, dt);
+}
+
+int ptc_get_coe(ptc_ctx *ctx, double *o) {
+  PTC_CTX(ctx);
+  PTC_CHECK(o != nullptr, PTC_E_ARG, "null output");
+  double pg = 0.0;
+  PTC_TRY(get_device_scalar(ctx, S_PRGRAD, &pg));
+  ctx->prgradaver = pg;
+  o[0] = ctx->pmGamma; o[1] = ctx->pmTheta; o[2] = ctx->pmAlpha; o[3] = ctx->pmAlphaC;
+  o[4] = ctx->pmBeta; o[5] = ctx->pmBetaSc; o[6] = ctx->dt; o[7] = ctx->simtime; o[8] = pg;
+  return PTC_OK;
+}
+
+int ptc_set_coe(ptc_ctx *ctx, const double *o) {
+  PTC_CTX(ctx);
+  PTC_CHECK(o != nullptr, PTC_E_ARG, "null input");
+  ctx->pmGamma = o[0]; ctx->pmTheta = o[1]; ctx->pmAlpha = o[2]; ctx->pmAlphaC = o[3];
+  ctx->pmBeta = o[4]; ctx->pmBetaSc = o[5]; ctx->dt = o[6]; ctx->simtime = o[7]; ctx->prgradaver = o[8];
+  return set_prgrad(ctx, o[8]);
+}
+
+int ptc_init_poisson_solver(ptc_ctx *ctx, int src, int dst) {
+  PTC_CTX(ctx);
+  return init_poisson(ctx, src, dst);
+}
+
+int ptc_clc_ppe(ptc_ctx *ctx, int src, int dst) {
+  PTC_CTX(ctx);
+  double *s, *d;
+  PTC_TRY(field(ctx, src, &s));
+  PTC_TRY(field(ctx, dst, &d));
+  PTC_CHECK(src != dst, PTC_E_ARG, "clcPPE: src and dst must differ");
+  PTC_CHECK(ctx->pp.prec == PTC_PREC_FP64, PTC_E_UNSUPPORTED, "FP32+refinement Poisson is not built yet");
+  return poisson_solve(ctx, s, d);
+}
+
+int ptc_destroy_poisson_plan(ptc_ctx *ctx) {
+  PTC_CTX(ctx);
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  return poisson_free_plan(ctx);
+}
+
+int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps) {
+  PTC_CTX(ctx);
+  PTC_CHECK(prec == PTC_PREC_FP64 || prec == PTC_PREC_FP32_REFINE, PTC_E_ARG, "unknown precision mode");
+  PTC_CHECK(tri >= PTC_TRI_AUTO && tri <= PTC_TRI_PARTITIONED, PTC_E_ARG, "unknown tridiagonal variant");
+  PTC_CHECK(tile == 0 || tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16, PTC_E_ARG,
+            "z tile must be 0, 1, 2, 4, 8 or 16");
+  PTC_CHECK(sweeps >= 0 && sweeps <= 8, PTC_E_ARG, "sweeps must be in 0..8");
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  poisson_free_plan(ctx);
+  ctx->pp.prec = prec;
+  ctx->pp.tri = tri == PTC_TRI_AUTO ? PTC_TRI_PARTITIONED : tri;
+  ctx->pp.ztile = tile;
+  ctx->pp.sweeps = sweeps;
+  return poisson_build_plan(ctx);
+}
+
+int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax) {
+  PTC_CTX(ctx);
+  PTC_CHECK(rmax != nullptr, PTC_E_ARG, "null output");
+  double *s, *d;
+  PTC_TRY(field(ctx, src, &s));
+  PTC_TRY(field(ctx, dst, &d));
+  return poisson_residual(ctx, s, d, rmax);
+}
+
+int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5) {
+  PTC_CTX(ctx);
+  if (ms5)
+    for (int s = 0; s < 5; ++s) ms5[s] = ctx->pp.stage_ms[s];
+  if (enable >= 0) {
+    ctx->pp.profile = enable != 0;
+    for (float &v : ctx->pp.stage_ms) v = 0.f;
+  }
+  return PTC_OK;
+}
+
+int ptc_init_time_scheme(ptc_ctx *ctx) {
+  PTC_CTX(ctx);
+  // The halo descriptors of InitTimeScheme (src/m_TScheme.f90:25-43) have no
+  // counterpart: x and z neighbours wrap inside the kernels.
+  return PTC_OK;
+}
+
+int ptc_clc_rhs_xyz(ptc_ctx *ctx, int ux, int uy, int uz, int scalar, int rhsx, int rhsy, int rhsz, int rhsc,
+                    int histx, int histy, int histz, int histc, int pressure) {
+  PTC_CTX(ctx);
+  double *u, *v, *w, *s, *rx, *ry, *rz, *rc, *hx, *hy, *hz, *hc, *p;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, uy, &v));
+  PTC_TRY(field(ctx, uz, &w));
+  PTC_TRY(field(ctx, rhsx, &rx));
+  PTC_TRY(field(ctx, rhsy, &ry));
+  PTC_TRY(field(ctx, rhsz, &rz));
+  PTC_TRY(field(ctx, histx, &hx));
+  PTC_TRY(field(ctx, histy, &hy));
+  PTC_TRY(field(ctx, histz, &hz));
+  PTC_TRY(field(ctx, pressure, &p));
+  if (ctx->prm.scalar) {
+    PTC_CHECK(scalar >= 0 && rhsc >= 0 && histc >= 0, PTC_E_ARG, "ScalarFlow needs scalar, rhsc and histc");
+    PTC_TRY(field(ctx, scalar, &s));
+    PTC_TRY(field(ctx, rhsc, &rc));
+    PTC_TRY(field(ctx, histc, &hc));
+  } else {
+    s = rc = hc = nullptr;
+  }
+  PTC_TRY(launch_rhs(ctx, u, v, w, s, rx, ry, rz, rc, hx, hy, hz, hc, p));
+  if (ctx->prm.is_ux_const) PTC_TRY(launch_add_shift(ctx, rx));
+  return PTC_OK;
+}
+
+int ptc_clc_u1hat(ptc_ctx *ctx, int ux, int rhsx) {
+  PTC_CTX(ctx);
+  double *u, *r;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, rhsx, &r));
+  return launch_tri_cn(ctx, 0, u, r, nullptr, ctx->pmBeta, nullptr);
+}
+
+int ptc_clc_u2hat(ptc_ctx *ctx, int uy, int rhsy, int duy_ym) {
+  PTC_CTX(ctx);
+  double *u, *r, *d;
+  PTC_TRY(field(ctx, uy, &u));
+  PTC_TRY(field(ctx, rhsy, &r));
+  PTC_TRY(field(ctx, duy_ym, &d));
+  PTC_CHECK(duy_ym == PTC_F_UY_YM || duy_ym == PTC_F_DUY_YM, PTC_E_ARG, "duy_ym must be a wall-plane field");
+  return launch_tri_cn(ctx, 1, u, r, d, ctx->pmBeta, nullptr);
+}
+
+int ptc_clc_u3hat(ptc_ctx *ctx, int uz, int rhsz) {
+  PTC_CTX(ctx);
+  double *u, *r;
+  PTC_TRY(field(ctx, uz, &u));
+  PTC_TRY(field(ctx, rhsz, &r));
+  return launch_tri_cn(ctx, 0, u, r, nullptr, ctx->pmBeta, nullptr);
+}
+
+int ptc_clc_scalar(ptc_ctx *ctx, int scalar, int rhsc) {
+  PTC_CTX(ctx);
+  PTC_CHECK(ctx->prm.scalar, PTC_E_STATE, "clcScalar needs the ScalarFlow variant");
+  double *u, *r;
+  PTC_TRY(field(ctx, scalar, &u));
+  PTC_TRY(field(ctx, rhsc, &r));
+  return launch_tri_cn(ctx, 2, u, r, nullptr, ctx->pmBetaSc, nullptr);
+}
+
+int ptc_clc_prsrc(ptc_ctx *ctx, int ux, int uy, int uz, int prsrc, double *divmax) {
+  PTC_CTX(ctx);
+  double *u, *v, *w, *f;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, uy, &v));
+  PTC_TRY(field(ctx, uz, &w));
+  PTC_TRY(field(ctx, prsrc, &f));
+  return launch_prsrc(ctx, u, v, w, f, divmax);
+}
+
+int ptc_fluid_vel_update(ptc_ctx *ctx, int phi, int ux, int uy, int uz) {
+  PTC_CTX(ctx);
+  double *f, *u, *v, *w;
+  PTC_TRY(field(ctx, phi, &f));
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, uy, &v));
+  PTC_TRY(field(ctx, uz, &w));
+  return launch_vel_update(ctx, f, u, v, w);
+}
+
+int ptc_pressure_update(ptc_ctx *ctx, int pressure, int phi) {
+  PTC_CTX(ctx);
+  double *p, *f;
+  PTC_TRY(field(ctx, pressure, &p));
+  PTC_TRY(field(ctx, phi, &f));
+  return launch_pressure_update(ctx, p, f);
+}
+
+int ptc_set_bc_and_update_halo(ptc_ctx *ctx, int ux, int uy, int uz, int uy_ym) {
+  PTC_CTX(ctx);
+  double *u, *v, *w, *b;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, uy, &v));
+  PTC_TRY(field(ctx, uz, &w));
+  PTC_TRY(field(ctx, uy_ym, &b));
+  return launch_bc_vel(ctx, u, v, w, b, false);
+}
+
+int ptc_set_bc_and_update_halo_for_prsrc(ptc_ctx *ctx, int ux, int uy, int uz, int uy_ym) {
+  PTC_CTX(ctx);
+  double *u, *v, *w, *b;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, uy, &v));
+  PTC_TRY(field(ctx, uz, &w));
+  PTC_TRY(field(ctx, uy_ym, &b));
+  return launch_bc_vel(ctx, u, v, w, b, true);
+}
+
+int ptc_set_bc_and_update_halo_pr(ptc_ctx *ctx, int p) {
+  PTC_CTX(ctx);
+  double *a;
+  PTC_TRY(field(ctx, p, &a));
+  return launch_bc_pr(ctx, a);
+}
+
+int ptc_set_bc_and_update_halo_scalar(ptc_ctx *ctx, int s) {
+  PTC_CTX(ctx);
+  PTC_CHECK(ctx->prm.scalar, PTC_E_STATE, "scalar BC needs the ScalarFlow variant");
+  double *a;
+  PTC_TRY(field(ctx, s, &a));
+  return launch_bc_scalar(ctx, a);
+}
+
+int ptc_update_uy_ym(ptc_ctx *ctx, int uy_ym, int duy_ym) {
+  PTC_CTX(ctx);
+  double *a, *b;
+  PTC_TRY(field(ctx, uy_ym, &a));
+  PTC_TRY(field(ctx, duy_ym, &b));
+  return launch_uy_ym(ctx, a, b);
+}
+
+static int three(ptc_ctx *ctx, int ux, int uy, int uz, double **u, double **v, double **w) {
+  PTC_TRY(field(ctx, ux, u));
+  PTC_TRY(field(ctx, uy, v));
+  PTC_TRY(field(ctx, uz, w));
+  return PTC_OK;
+}
+
+int ptc_calc_max_cfl(ptc_ctx *ctx, int ux, int uy, int uz, double *out) {
+  PTC_CTX(ctx);
+  PTC_CHECK(out, PTC_E_ARG, "null output");
+  double *u, *v, *w;
+  PTC_TRY(three(ctx, ux, uy, uz, &u, &v, &w));
+  PTC_TRY(launch_monitor(ctx, 0, u, v, w, nullptr, out));
+  *out *= 0.5;
+  return PTC_OK;
+}
+
+int ptc_check_divergence(ptc_ctx *ctx, int ux, int uy, int uz, double *out) {
+  PTC_CTX(ctx);
+  PTC_CHECK(out, PTC_E_ARG, "null output");
+  double *u, *v, *w;
+  PTC_TRY(three(ctx, ux, uy, uz, &u, &v, &w));
+  return launch_monitor(ctx, 1, u, v, w, nullptr, out);
+}
+
+int ptc_calc_vmax(ptc_ctx *ctx, int ux, int uy, int uz, double *v3) {
+  PTC_CTX(ctx);
+  PTC_CHECK(v3, PTC_E_ARG, "null output");
+  double *u, *v, *w;
+  PTC_TRY(three(ctx, ux, uy, uz, &u, &v, &w));
+  for (int q = 0; q < 3; ++q) PTC_TRY(launch_monitor(ctx, 2 + q, u, v, w, nullptr, &v3[q]));
+  return PTC_OK;
+}
+
+int ptc_calc_ux_aver(ptc_ctx *ctx, int ux, double *mean) {
+  PTC_CTX(ctx);
+  PTC_CHECK(mean, PTC_E_ARG, "null output");
+  double *u;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(launch_monitor(ctx, 5, u, u, u, nullptr, mean));
+  *mean = *mean / (double)(ctx->nx * ctx->nz) / ctx->yly;
+  return PTC_OK;
+}
+
+int ptc_clc_mean_value(ptc_ctx *ctx, int ux, int scalar, double *res3) {
+  PTC_CTX(ctx);
+  PTC_CHECK(res3, PTC_E_ARG, "null output");
+  double *u, *s;
+  PTC_TRY(field(ctx, ux, &u));
+  PTC_TRY(field(ctx, scalar, &s));
+  for (int q = 0; q < 3; ++q) {
+    PTC_TRY(launch_monitor(ctx, 6 + q, u, u, u, s, &res3[q]));
+    res3[q] = res3[q] / (double)(ctx->nx * ctx->nz) / ctx->yly;
+  }
+  return PTC_OK;
+}
+
+int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7) {
+  PTC_CTX(ctx);
+  PTC_CHECK(nsteps >= 1, PTC_E_ARG, "nsteps must be >= 1");
+  double mon[7];
+  for (int s = 0; s < nsteps; ++s) PTC_TRY(channel_iterate(ctx, (s == nsteps - 1) ? mon : nullptr));
+  if (mon7) std::copy(mon, mon + 7, mon7);
+  // src/m_ChannelSystem.f90:142-145
+  if (ctx->prm.div_limit > 0 && mon[1] > ctx->prm.div_limit) {
+    set_error("too big div: " + std::to_string(mon[1]));
+    return PTC_E_LIMIT;
+  }
+  if (ctx->prm.vel_limit > 0 && std::min({mon[2], mon[3], mon[4]}) > ctx->prm.vel_limit) {
+    set_error("too big velocity: " + std::to_string(mon[2]) + ", " + std::to_string(mon[3]) + ", " +
+              std::to_string(mon[4]));
+    return PTC_E_LIMIT;
+  }
+  return PTC_OK;
+}
+
+int ptc_get_scalar(ptc_ctx *ctx, int which, double *value) {
+  PTC_CTX(ctx);
+  PTC_CHECK(value, PTC_E_ARG, "null output");
+  switch (which) {
+    case PTC_S_DIVMAX_PRSRC: return get_device_scalar(ctx, S_DIVMAX, value);
+    case PTC_S_PRGRADAVER: return get_device_scalar(ctx, S_PRGRAD, value);
+    case PTC_S_SIMTIME: *value = ctx->simtime; return PTC_OK;
+    case PTC_S_DT: *value = ctx->dt; return PTC_OK;
+    default: set_error("unknown scalar id"); return PTC_E_ARG;
+  }
+}
+
+}  // extern "C"

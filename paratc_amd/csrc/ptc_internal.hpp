@@ -1,0 +1,178 @@
+// ptc_internal.hpp -- shared definitions of the gfx950 HIP library.
+//
+// Device layout of every 3-D field (one context, one case):
+//   index(i, j, k) = i + nx * (j + (ny + 2) * k)
+//   i = 0..nx-1   (reference ic = i + 1), periodic: no x halo is stored
+//   j = 0..ny+1   (reference jc = j), rows 0 and ny+1 are the wall ghost rows
+//   k = 0..nz-1   (reference kc = k + 1), periodic: no z halo is stored
+// Kernels wrap x and z neighbours with a conditional add, so the reference's
+// periodic halo refreshes (myupdate_halo, src/m_Decomp2d.f90:1554-1709) are
+// implicit and never read stale data.  Interior-only reference arrays use
+// the same layout and ignore their ghost rows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/paratc_gpu.h"
+
+namespace ptc {
+
+void set_error(const std::string &msg);
+
+#define PTC_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ::ptc::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+      return PTC_E_HIP;                                                                \
+    }                                                                                  \
+  } while (0)
+
+#define PTC_CHECK(cond, code, msg)                                                     \
+  do {                                                                                 \
+    if (!(cond)) {                                                                     \
+      ::ptc::set_error(msg);                                                           \
+      return (code);                                                                   \
+    }                                                                                  \
+  } while (0)
+
+#define PTC_TRY(expr)                                                                  \
+  do {                                                                                 \
+    int r_ = (expr);                                                                   \
+    if (r_ != PTC_OK) return r_;                                                       \
+  } while (0)
+
+// metric table rows in the device array d_met[(row) * (ny+2) + j]
+enum Met {
+  M_YC = 0, M_DYP, M_DYC, M_RDYP, M_RDYC, M_YINT,
+  M_AM2PH, M_AC2PH, M_AP2PH, M_AM2C, M_AC2C, M_AP2C,
+  M_AM2CCN, M_AC2CCN, M_AP2CCN, M_AM2P, M_AC2P, M_AP2P,
+  M_AM2CSC, M_AC2CSC, M_AP2CSC, M_AM2CSCCN, M_AC2CSCCN, M_AP2CSCCN,
+  M_NMET
+};
+
+// Geometry and stencil constants passed by value to every stencil kernel
+// (the scalars of m_MeshAndMetries, src/m_MeshAndMetries.f90:10-13,48-52).
+struct Geo {
+  int nx, ny, nz;
+  long long plane;  // nx * (ny + 2)
+  double rdx, rdz;
+  double dxc1, dxc2, dxc3, dxc4, dzc1, dzc2, dzc3, dzc4;
+  double dxxc1, dxxc2, dxxc3, dxxc4, dxxc5, dzzc1, dzzc2, dzzc3, dzzc4, dzzc5;
+  double ic1, ic2, ic3, ic4;
+  const double *met;  // d_met
+  int nyh;            // ny + 2
+};
+
+// One complex FFT length: radix plan and twiddle table w^m = exp(-2 pi i m / n).
+struct FftLen {
+  int n;
+  int nstage;
+  int radix[16];
+  const double2 *tw;
+};
+
+constexpr int kMaxRadix = 8;
+
+struct PoissonPlan {
+  bool ready = false;
+  int prec = PTC_PREC_FP64;
+  int tri = PTC_TRI_PARTITIONED;
+  int sweeps = 2;
+  int h = 0;          // nx / 2
+  int pitch = 0;      // complex row pitch of the spectral buffer (>= h + 1)
+  int ztile = 4;      // columns per z-FFT workgroup
+  int z_threads = 256;
+  int x_lines = 4;    // lines per x-FFT workgroup
+  int x_threads = 256;
+  int tri_chunks = 1; // chunks (waves) per line in the partitioned solver
+  int tri_mmax = 16;  // register rows per thread
+  FftLen fx{}, fz{};          // complex lengths h and nz
+  double2 *d_tw = nullptr;    // all twiddle tables
+  const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
+  double2 *d_spec = nullptr;  // spectral buffer pitch * ny * nz complex
+  double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
+  size_t spec_elems = 0;
+  bool profile = false;
+  float stage_ms[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[6] = {};
+};
+
+}  // namespace ptc
+
+struct ptc_ctx {
+  int nx = 0, ny = 0, nz = 0;
+  double xlx = 0, yly = 0, zlz = 0;
+  ptc_params prm{};
+  size_t plane = 0;   // nx * (ny + 2)
+  size_t n3 = 0;      // nx * (ny + 2) * nz
+  ptc::Geo geo{};
+  std::vector<double> h_met;  // M_NMET * (ny + 2)
+  std::vector<double> h_wnx, h_wnz;
+  double normfft = 0;
+  double *d_met = nullptr;
+  double *d_wnx = nullptr, *d_wnz = nullptr;
+  double *d_field[PTC_NFIELDS] = {};
+  double *d_red = nullptr;     // reduction partials
+  int red_cap = 0;
+  double *d_scal = nullptr;    // device scalars
+  double *h_pinned = nullptr;  // pinned host scalars
+  hipStream_t stream = nullptr;
+  int device = 0;
+  size_t lds_limit = 65536;
+  int n_cu = 256;
+  // time scheme (src/m_Parameters.f90:40-61)
+  double gconst[3] = {0, 0, 0}, tconst[3] = {0, 0, 0}, aconst[3] = {0, 0, 0};
+  int iadvance = 3;
+  double pmGamma = 0, pmTheta = 0, pmAlpha = 0, pmAlphaC = 0, pmBeta = 0, pmBetaSc = 0;
+  double dt = 0, simtime = 0, prgradaver = 0;
+  double divmax_prsrc = 0;
+  ptc::PoissonPlan pp;
+};
+
+namespace ptc {
+
+// field access (allocating on first use, zero-filled)
+int field(ptc_ctx *c, int id, double **out);
+int check_field(ptc_ctx *c, int id);
+
+// Poisson
+int poisson_build_plan(ptc_ctx *c);
+int poisson_free_plan(ptc_ctx *c);
+int poisson_solve(ptc_ctx *c, const double *src, double *dst);
+int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
+
+// FFT launchers (ptc_fft.hip)
+int launch_fft_x_fwd(ptc_ctx *c, const double *src, double2 *spec);
+int launch_fft_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale);
+int launch_fft_z(ptc_ctx *c, double2 *spec, bool inverse);
+int fft_make_len(int n, FftLen *L);
+
+// tridiagonal launchers (ptc_tridiag.hip)
+int launch_tri_poisson(ptc_ctx *c, double2 *spec);
+int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym,
+                  double beta, const double *shift);
+
+// stencils (ptc_stencil.hip)
+int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
+               const double *sc, double *rhsx, double *rhsy, double *rhsz, double *rhsc,
+               double *hx, double *hy, double *hz, double *hc, const double *p);
+int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
+                 double *divmax_host);
+int launch_vel_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz);
+int launch_pressure_update(ptc_ctx *c, double *p, const double *phi);
+int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p);
+int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, bool prsrc_only);
+int launch_bc_pr(ptc_ctx *c, double *p);
+int launch_bc_scalar(ptc_ctx *c, double *s);
+int launch_uy_ym(ptc_ctx *c, double *uy_ym, double *duy_ym);
+int launch_monitor(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz,
+                   const double *s, double *out);
+int reduce_sum_s3(ptc_ctx *c, int nparts, double *host_out);
+
+}  // namespace ptc

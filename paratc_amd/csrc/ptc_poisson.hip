@@ -1,0 +1,222 @@
+// ptc_poisson.hip -- the pressure Poisson solve (m_Poisson) on one GPU.
+//
+// The reference solves L phi = f with L = D_x G_x + D_y G_y + D_z G_z
+// (4th-order staggered in x and z, 2nd-order stretched in y) by real FFTs in
+// x and z, one tridiagonal solve in y per wavenumber pair, and the inverse
+// FFTs (clcPPE_x1, src/m_Poisson.f90:352-452).  On one device the six pencil
+// transposes of that routine are unnecessary: the whole field is resident and
+// every pass reads the dimension it transforms directly.  A solve is five
+// streaming passes over a complex half-spectrum buffer S[kz][j][kx]
+// (kx = 0..nx/2, row pitch rounded up to 16 complex):
+//
+//   1. x forward   src (real, device field layout)  -> S     (ptc_fft.hip)
+//   2. z forward   S -> S, tiles of T kx columns           (ptc_fft.hip)
+//   3. y solve     S -> S, one lane per (kx, kz) line      (ptc_tridiag.hip)
+//   4. z inverse   S -> S
+//   5. x inverse   S -> dst (real), times normfft = 1/(nx nz)
+//
+// Each pass reads and writes every point once: 16 bytes per grid point per
+// pass in FP64, 80 N bytes per solve.  The modified wavenumber of the complex
+// mode kx equals the reference's WaveNumX at the half-complex position of the
+// same frequency (the symbol is even in kx), so both solves diagonalise the
+// same operator.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ptc_internal.hpp"
+
+namespace ptc {
+
+bool fft_config_ok(const FftLen &L, int ncol, int nthr);
+int tri_plan(int ny, size_t lds_limit, size_t vbytes, int *P_out, int *mmax_out);
+
+static void make_twiddles(int n, std::vector<double2> &out) {
+  const double twopi = 6.283185307179586476925286766559005768;
+  out.resize((size_t)n);
+  for (int m = 0; m < n; ++m) {
+    const double a = twopi * (double)m / (double)n;
+    out[(size_t)m] = make_double2(std::cos(a), -std::sin(a));
+  }
+}
+
+int poisson_free_plan(ptc_ctx *c) {
+  PoissonPlan &P = c->pp;
+  if (P.d_tw) (void)hipFree(P.d_tw);
+  if (P.d_spec) (void)hipFree(P.d_spec);
+  if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
+  for (auto &e : P.ev)
+    if (e) (void)hipEventDestroy(e);
+  const int prec = P.prec, tri = P.tri, sweeps = P.sweeps, ztile = P.ztile;
+  P = PoissonPlan();
+  P.prec = prec;
+  P.tri = tri;
+  P.sweeps = sweeps;
+  P.ztile = ztile;
+  return PTC_OK;
+}
+
+int poisson_build_plan(ptc_ctx *c) {
+  PoissonPlan &P = c->pp;
+  if (P.ready) return PTC_OK;
+  PTC_CHECK(c->nx % 2 == 0 && c->nz % 2 == 0, PTC_E_ARG, "mod(nxc,2)/=0 || mod(nzc,2)/=0 WRONG");
+  P.h = c->nx / 2;
+  P.pitch = ((P.h + 1 + 15) / 16) * 16;
+  PTC_TRY(fft_make_len(P.h, &P.fx));
+  PTC_TRY(fft_make_len(c->nz, &P.fz));
+  const size_t lds_cap = std::min<size_t>(c->lds_limit, 65536);
+
+  // x pass: lines per workgroup so that a tile holds ~2048 complex values
+  P.x_lines = std::max(1, 2048 / P.h);
+  while (P.x_lines > 1 && sizeof(double2) * (size_t)P.h * P.x_lines > lds_cap) P.x_lines /= 2;
+  PTC_CHECK(sizeof(double2) * (size_t)P.h * P.x_lines <= lds_cap, PTC_E_UNSUPPORTED,
+            "nx too large for the LDS x-FFT");
+  P.x_threads = 0;
+  for (int t : {256, 512, 1024})
+    if (fft_config_ok(P.fx, P.x_lines, t)) { P.x_threads = t; break; }
+  PTC_CHECK(P.x_threads > 0, PTC_E_UNSUPPORTED, "no x-FFT launch shape fits");
+
+  // z pass: widest column tile (<= 16, divides pitch) that fits LDS
+  int T = P.ztile > 0 ? P.ztile : 16;
+  while (T > 1 && (sizeof(double2) * (size_t)c->nz * T > lds_cap || (size_t)T * c->nz > 4096)) T /= 2;
+  PTC_CHECK(P.pitch % T == 0, PTC_E_ARG, "z tile must divide the spectral pitch");
+  PTC_CHECK(sizeof(double2) * (size_t)c->nz * T <= lds_cap, PTC_E_UNSUPPORTED, "nz too large for the LDS z-FFT");
+  P.ztile = T;
+  P.z_threads = 0;
+  for (int t : {256, 512, 1024})
+    if (fft_config_ok(P.fz, T, t)) { P.z_threads = t; break; }
+  PTC_CHECK(P.z_threads > 0, PTC_E_UNSUPPORTED, "no z-FFT launch shape fits");
+
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, 16, &P.tri_chunks, &P.tri_mmax));
+
+  // twiddle tables: [fx: h][fz: nz][wx: h+1]
+  std::vector<double2> tx, tz, all;
+  make_twiddles(P.h, tx);
+  make_twiddles(c->nz, tz);
+  std::vector<double2> tnx;
+  make_twiddles(c->nx, tnx);
+  all.reserve(tx.size() + tz.size() + (size_t)P.h + 1);
+  all.insert(all.end(), tx.begin(), tx.end());
+  all.insert(all.end(), tz.begin(), tz.end());
+  all.insert(all.end(), tnx.begin(), tnx.begin() + P.h + 1);
+  PTC_HIP(hipMalloc(&P.d_tw, sizeof(double2) * all.size()));
+  PTC_HIP(hipMemcpy(P.d_tw, all.data(), sizeof(double2) * all.size(), hipMemcpyHostToDevice));
+  P.fx.tw = P.d_tw;
+  P.fz.tw = P.d_tw + tx.size();
+  P.d_wx = P.d_tw + tx.size() + tz.size();
+
+  P.spec_elems = (size_t)P.pitch * c->ny * c->nz;
+  if (hipMalloc(&P.d_spec, sizeof(double2) * P.spec_elems) != hipSuccess) {
+    P.d_spec = nullptr;
+    set_error("spectral buffer allocation failed");
+    return PTC_E_ALLOC;
+  }
+  PTC_HIP(hipMemsetAsync(P.d_spec, 0, sizeof(double2) * P.spec_elems, c->stream));
+  for (auto &e : P.ev) PTC_HIP(hipEventCreate(&e));
+  P.ready = true;
+  return PTC_OK;
+}
+
+static int ensure_thomas_scratch(ptc_ctx *c) {
+  PoissonPlan &P = c->pp;
+  if (P.tri != PTC_TRI_THOMAS || P.d_tri_scratch) return PTC_OK;
+  if (hipMalloc(&P.d_tri_scratch, sizeof(double) * P.spec_elems) != hipSuccess) {
+    P.d_tri_scratch = nullptr;
+    set_error("Thomas scratch allocation failed");
+    return PTC_E_ALLOC;
+  }
+  return PTC_OK;
+}
+
+int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
+  PTC_TRY(poisson_build_plan(c));
+  PoissonPlan &P = c->pp;
+  PTC_TRY(ensure_thomas_scratch(c));
+  const bool prof = P.profile;
+  if (prof) PTC_HIP(hipEventRecord(P.ev[0], c->stream));
+  PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
+  if (prof) PTC_HIP(hipEventRecord(P.ev[1], c->stream));
+  PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  if (prof) PTC_HIP(hipEventRecord(P.ev[2], c->stream));
+  PTC_TRY(launch_tri_poisson(c, P.d_spec));
+  if (prof) PTC_HIP(hipEventRecord(P.ev[3], c->stream));
+  PTC_TRY(launch_fft_z(c, P.d_spec, true));
+  if (prof) PTC_HIP(hipEventRecord(P.ev[4], c->stream));
+  PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
+  if (prof) {
+    PTC_HIP(hipEventRecord(P.ev[5], c->stream));
+    PTC_HIP(hipEventSynchronize(P.ev[5]));
+    for (int s = 0; s < 5; ++s) {
+      float ms = 0.f;
+      PTC_HIP(hipEventElapsedTime(&ms, P.ev[s], P.ev[s + 1]));
+      P.stage_ms[s] += ms;
+    }
+  }
+  return PTC_OK;
+}
+
+// ---- FP64 residual r = f - L phi, max |r| over the interior ---------------
+// L in x and z is the composed stencil D o G of the 4th-order staggered
+// operators: (1, -54, 783, -1460, 783, -54, 1) / 576 / dx^2, whose symbol is
+// the modified wavenumber of src/m_Poisson.f90:42,46.
+__global__ void __launch_bounds__(256) k_residual_max(const double *__restrict__ f, const double *__restrict__ phi,
+                                                     Geo g, double rdx2, double rdz2, double *__restrict__ part) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long total = (long long)nx * ny * nz;
+  double mx = 0.0;
+  const double *am = g.met + M_AM2PH * g.nyh, *ac = g.met + M_AC2PH * g.nyh, *ap = g.met + M_AP2PH * g.nyh;
+  constexpr double st[4] = {-1460.0 / 576.0, 783.0 / 576.0, -54.0 / 576.0, 1.0 / 576.0};
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % nx);
+    const long long r = q / nx;
+    const int j = (int)(r % ny) + 1;
+    const int k = (int)(r / ny);
+    auto at = [&](int ii, int jj, int kk) -> double {
+      ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
+      kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+      return phi[(long long)ii + (long long)nx * ((long long)jj + (long long)(ny + 2) * kk)];
+    };
+    const double c0 = at(i, j, k);
+    double lx = st[0] * c0, lz = st[0] * c0;
+#pragma unroll
+    for (int s = 1; s <= 3; ++s) {
+      lx += st[s] * (at(i - s, j, k) + at(i + s, j, k));
+      lz += st[s] * (at(i, j, k - s) + at(i, j, k + s));
+    }
+    const double ym = j > 1 ? at(i, j - 1, k) : c0;
+    const double yp = j < ny ? at(i, j + 1, k) : c0;
+    const double lap = rdx2 * lx + rdz2 * lz + (am[j] * ym + ac[j] * c0 + ap[j] * yp);
+    const double res = f[(long long)i + (long long)nx * ((long long)j + (long long)(ny + 2) * k)] - lap;
+    mx = fmax(mx, fabs(res));
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax) {
+  const int nblk = std::min(2048, c->n_cu * 8);
+  if (c->red_cap < nblk) {
+    if (c->d_red) (void)hipFree(c->d_red);
+    PTC_HIP(hipMalloc(&c->d_red, sizeof(double) * nblk));
+    c->red_cap = nblk;
+  }
+  const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
+  hipLaunchKernelGGL(k_residual_max, dim3(nblk), dim3(256), 0, c->stream, src, dst, c->geo, rdx2, rdz2, c->d_red);
+  PTC_HIP(hipGetLastError());
+  std::vector<double> h((size_t)nblk);
+  PTC_HIP(hipMemcpyAsync(h.data(), c->d_red, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  double m = 0.0;
+  for (double v : h) m = std::max(m, v);
+  *rmax = m;
+  return PTC_OK;
+}
+
+}  // namespace ptc

@@ -1,0 +1,653 @@
+// ptc_stencil.hip -- the explicit stencils of one RK substep on gfx950.
+//
+// clcRhsXYZ (src/m_TScheme.f90:113-360) builds six 4th-order midpoint
+// interpolations into haloed temporaries, refreshes their halos and then
+// assembles RhsX/Y/Z (and RhsC).  Here one kernel computes every right-hand
+// side of a grid point directly from ux, uy, uz, p (and the scalar): the
+// interpolated values a point needs are evaluated in registers from the
+// neighbouring velocities, so the six temporaries and their halo refreshes
+// never touch HBM.  The operand order of every interpolation and flux follows
+// the reference; x and z neighbours wrap periodically (one rank owns the whole
+// domain), y uses the stored ghost rows.
+//
+// clcPrSrc, FluidVelUpdate, PressureUpdate (src/m_TScheme.f90:48-107,512-532),
+// the wall boundary fills of m_BC_and_Halo (src/m_BC_and_Halo.f90:23-178),
+// Update_uy_ym (src/m_FlowCase.f90:112-122) and the monitors of m_Tools
+// (src/m_Tools.f90:24-110,256-301) follow.  Every reduction is a fixed-order
+// tree over a fixed grid, so results are run-to-run deterministic.
+#include <algorithm>
+#include <vector>
+
+#include "ptc_internal.hpp"
+
+namespace ptc {
+
+// device scalar slots in ctx->d_scal
+enum { S_DP1NS = 0, S_PRGRAD = 1, S_DIVMAX = 2, S_MON0 = 8 };
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+struct Ngh {
+  int ii[7];
+  long long kk[7];
+  long long nxl;
+  __device__ __forceinline__ Ngh(const Geo &g, int i, int k) {
+    nxl = g.nx;
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      ii[s] = wrapi(i + s - 3, g.nx);
+      kk[s] = (long long)wrapi(k + s - 3, g.nz) * g.plane;
+    }
+  }
+  // value of a at (i + di, j, k + dk)
+  __device__ __forceinline__ double operator()(const double *__restrict__ a, int di, int j, int dk) const {
+    return a[ii[di + 3] + nxl * j + kk[dk + 3]];
+  }
+};
+
+struct RhsCoef {
+  double gamma, theta, alpha, beta, beta_sc;
+  double xnu, ucrf, gx, gy, gz;
+  double sdiff, gev[3], fvv[3];
+  int scalar, s3_mode;  // s3_mode: 0 none, 1 viscous (non-scalar build), 2 ux (ScalarFlow build)
+};
+
+// Fixed-order tree reductions over all threads of a block (1-D or 2-D); the
+// block size must be a power of two no larger than the shared array (256).
+__device__ __forceinline__ int lin_tid() { return threadIdx.x + blockDim.x * threadIdx.y; }
+__device__ __forceinline__ int lin_size() { return blockDim.x * blockDim.y; }
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+  const int t = lin_tid();
+  sh[t] = v;
+  __syncthreads();
+  for (int w = lin_size() / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh) {
+  const int t = lin_tid();
+  sh[t] = v;
+  __syncthreads();
+  for (int w = lin_size() / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] = fmax(sh[t], sh[t + w]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// ---- clcRhsXYZ -------------------------------------------------------------
+// block (64, 4): i = bx*64 + tx, j = by*4 + ty + 1, k = bz
+__global__ void __launch_bounds__(256) k_rhs_xyz(Geo g, RhsCoef cf, const double *__restrict__ ux,
+                                                const double *__restrict__ uy, const double *__restrict__ uz,
+                                                const double *__restrict__ sc, const double *__restrict__ p,
+                                                double *__restrict__ rhsx, double *__restrict__ rhsy,
+                                                double *__restrict__ rhsz, double *__restrict__ rhsc,
+                                                double *__restrict__ hx, double *__restrict__ hy,
+                                                double *__restrict__ hz, double *__restrict__ hc,
+                                                double *__restrict__ s3part) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y + 1;
+  const int k = blockIdx.z;
+  double s3 = 0.0;
+  if (i < g.nx && j <= g.ny) {
+    const Ngh N(g, i, k);
+    const int jm = j - 1, jp = j + 1;
+    const int nyp = g.ny + 1;
+    const double *M = g.met;
+    const int W = g.nyh;
+    const double rdyp = M[M_RDYP * W + j], rdyc = M[M_RDYC * W + j];
+    const double IY1 = M[M_YINT * W + j], IY2 = 1.0 - IY1;
+    const double IY3 = M[M_YINT * W + jp], IY4 = 1.0 - IY3;
+    const long long o = N.ii[3] + N.nxl * j + N.kk[3];
+    const double c1 = g.ic1, c2 = g.ic2, c3 = g.ic3, c4 = g.ic4;
+    // interpolations (src/m_TScheme.f90:148-154) evaluated where needed
+    auto uxhx = [&](int di) {  // at (i+di, j, k)
+      return c1 * N(ux, di - 1, j, 0) + c2 * N(ux, di, j, 0) + c3 * N(ux, di + 1, j, 0) + c4 * N(ux, di + 2, j, 0);
+    };
+    auto uyhx = [&](int di, int jj) {  // at (i+di, jj, k); zero on row nyp (:157-160)
+      if (jj == nyp) return 0.0;
+      return c1 * N(uy, di - 2, jj, 0) + c2 * N(uy, di - 1, jj, 0) + c3 * N(uy, di, jj, 0) + c4 * N(uy, di + 1, jj, 0);
+    };
+    auto uzhx = [&](int di, int dk) {
+      return c1 * N(uz, di - 2, j, dk) + c2 * N(uz, di - 1, j, dk) + c3 * N(uz, di, j, dk) + c4 * N(uz, di + 1, j, dk);
+    };
+    auto uxhz = [&](int di, int dk) {
+      return c1 * N(ux, di, j, dk - 2) + c2 * N(ux, di, j, dk - 1) + c3 * N(ux, di, j, dk) + c4 * N(ux, di, j, dk + 1);
+    };
+    auto uyhz = [&](int dk, int jj) {
+      if (jj == nyp) return 0.0;
+      return c1 * N(uy, 0, jj, dk - 2) + c2 * N(uy, 0, jj, dk - 1) + c3 * N(uy, 0, jj, dk) + c4 * N(uy, 0, jj, dk + 1);
+    };
+    auto uzhz = [&](int dk) {
+      return c1 * N(uz, 0, j, dk - 1) + c2 * N(uz, 0, j, dk) + c3 * N(uz, 0, j, dk + 1) + c4 * N(uz, 0, j, dk + 2);
+    };
+    const double ucrf = cf.ucrf;
+
+    // ---- RhsX (:169-204)
+    double rx, cx, d22q1, dcq13x;
+    {
+      const double uxhyc = IY1 * N(ux, 0, jm, 0) + IY2 * N(ux, 0, j, 0);
+      const double uxhyp = IY3 * N(ux, 0, j, 0) + IY4 * N(ux, 0, jp, 0);
+      const double a0 = uxhx(-2), a1 = uxhx(-1), a2 = uxhx(0), a3 = uxhx(1);
+      const double h11 = g.dxc1 * a0 * (a0 + ucrf) + g.dxc2 * a1 * (a1 + ucrf) + g.dxc3 * a2 * (a2 + ucrf) +
+                         g.dxc4 * a3 * (a3 + ucrf);
+      const double h12 = (uyhx(0, jp) * uxhyp - uyhx(0, j) * uxhyc) * rdyp;
+      const double h13 = g.dzc1 * uxhz(0, -1) * uzhx(0, -1) + g.dzc2 * uxhz(0, 0) * uzhx(0, 0) +
+                         g.dzc3 * uxhz(0, 1) * uzhx(0, 1) + g.dzc4 * uxhz(0, 2) * uzhx(0, 2);
+      const double d11 = g.dxxc1 * N(ux, -2, j, 0) + g.dxxc2 * N(ux, -1, j, 0) + g.dxxc3 * N(ux, 0, j, 0) +
+                         g.dxxc4 * N(ux, 1, j, 0) + g.dxxc5 * N(ux, 2, j, 0);
+      d22q1 = M[M_AP2C * W + j] * N(ux, 0, jp, 0) + M[M_AC2C * W + j] * N(ux, 0, j, 0) +
+              M[M_AM2C * W + j] * N(ux, 0, jm, 0);
+      const double d33 = g.dzzc1 * N(ux, 0, j, -2) + g.dzzc2 * N(ux, 0, j, -1) + g.dzzc3 * N(ux, 0, j, 0) +
+                         g.dzzc4 * N(ux, 0, j, 1) + g.dzzc5 * N(ux, 0, j, 2);
+      dcq13x = d11 + d33;
+      cx = -h11 - h12 - h13 + cf.xnu * dcq13x + cf.gx;
+      const double gp = g.dxc1 * N(p, -2, j, 0) + g.dxc2 * N(p, -1, j, 0) + g.dxc3 * N(p, 0, j, 0) +
+                        g.dxc4 * N(p, 1, j, 0);
+      rx = cf.gamma * cx + cf.theta * hx[o] - cf.alpha * gp + 2.0 * cf.beta * d22q1;
+    }
+    if (cf.s3_mode == 1) s3 = (dcq13x + d22q1) * M[M_DYP * W + j];
+    else if (cf.s3_mode == 2) s3 = N(ux, 0, j, 0) * M[M_DYP * W + j];
+
+    // ---- RhsY (:206-236)
+    double ry, cy;
+    {
+      const double qs = 0.25 * rdyc;
+      const double uxhym = IY1 * N(ux, -1, jm, 0) + IY2 * N(ux, -1, j, 0) + ucrf;
+      const double uxhyc = IY1 * N(ux, 0, jm, 0) + IY2 * N(ux, 0, j, 0) + ucrf;
+      const double uxhyp = IY1 * N(ux, 1, jm, 0) + IY2 * N(ux, 1, j, 0) + ucrf;
+      const double uxhyu = IY1 * N(ux, 2, jm, 0) + IY2 * N(ux, 2, j, 0) + ucrf;
+      const double uzhym = IY1 * N(uz, 0, jm, -1) + IY2 * N(uz, 0, j, -1);
+      const double uzhyc = IY1 * N(uz, 0, jm, 0) + IY2 * N(uz, 0, j, 0);
+      const double uzhyp = IY1 * N(uz, 0, jm, 1) + IY2 * N(uz, 0, j, 1);
+      const double uzhyu = IY1 * N(uz, 0, jm, 2) + IY2 * N(uz, 0, j, 2);
+      const double h21 = g.dxc1 * uyhx(-1, j) * uxhym + g.dxc2 * uyhx(0, j) * uxhyc + g.dxc3 * uyhx(1, j) * uxhyp +
+                         g.dxc4 * uyhx(2, j) * uxhyu;
+      const double s1 = N(uy, 0, jp, 0) + N(uy, 0, j, 0);
+      const double s2 = N(uy, 0, j, 0) + N(uy, 0, jm, 0);
+      const double h22 = (s1 * s1 - s2 * s2) * qs;
+      const double h23 = g.dzc1 * uyhz(-1, j) * uzhym + g.dzc2 * uyhz(0, j) * uzhyc + g.dzc3 * uyhz(1, j) * uzhyp +
+                         g.dzc4 * uyhz(2, j) * uzhyu;
+      const double d11 = g.dxxc1 * N(uy, -2, j, 0) + g.dxxc2 * N(uy, -1, j, 0) + g.dxxc3 * N(uy, 0, j, 0) +
+                         g.dxxc4 * N(uy, 1, j, 0) + g.dxxc5 * N(uy, 2, j, 0);
+      const double d22 = M[M_AP2P * W + j] * N(uy, 0, jp, 0) + M[M_AC2P * W + j] * N(uy, 0, j, 0) +
+                         M[M_AM2P * W + j] * N(uy, 0, jm, 0);
+      const double d33 = g.dzzc1 * N(uy, 0, j, -2) + g.dzzc2 * N(uy, 0, j, -1) + g.dzzc3 * N(uy, 0, j, 0) +
+                         g.dzzc4 * N(uy, 0, j, 1) + g.dzzc5 * N(uy, 0, j, 2);
+      cy = -h21 - h22 - h23 + cf.xnu * (d11 + d33) + cf.gy;
+      const double gp = (N(p, 0, j, 0) - N(p, 0, jm, 0)) * rdyc;
+      ry = cf.gamma * cy + cf.theta * hy[o] - cf.alpha * gp + 2.0 * cf.beta * d22;
+    }
+
+    // ---- RhsZ (:238-267)
+    double rz, cz;
+    {
+      const double uzhyc = IY1 * N(uz, 0, jm, 0) + IY2 * N(uz, 0, j, 0);
+      const double uzhyp = IY3 * N(uz, 0, j, 0) + IY4 * N(uz, 0, jp, 0);
+      const double h31 = g.dxc1 * uzhx(-1, 0) * (uxhz(-1, 0) + ucrf) + g.dxc2 * uzhx(0, 0) * (uxhz(0, 0) + ucrf) +
+                         g.dxc3 * uzhx(1, 0) * (uxhz(1, 0) + ucrf) + g.dxc4 * uzhx(2, 0) * (uxhz(2, 0) + ucrf);
+      const double h32 = (uyhz(0, jp) * uzhyp - uyhz(0, j) * uzhyc) * rdyp;
+      const double b0 = uzhz(-2), b1 = uzhz(-1), b2 = uzhz(0), b3 = uzhz(1);
+      const double h33 = g.dzc1 * b0 * b0 + g.dzc2 * b1 * b1 + g.dzc3 * b2 * b2 + g.dzc4 * b3 * b3;
+      const double d11 = g.dxxc1 * N(uz, -2, j, 0) + g.dxxc2 * N(uz, -1, j, 0) + g.dxxc3 * N(uz, 0, j, 0) +
+                         g.dxxc4 * N(uz, 1, j, 0) + g.dxxc5 * N(uz, 2, j, 0);
+      const double d22 = M[M_AP2C * W + j] * N(uz, 0, jp, 0) + M[M_AC2C * W + j] * N(uz, 0, j, 0) +
+                         M[M_AM2C * W + j] * N(uz, 0, jm, 0);
+      const double d33 = g.dzzc1 * N(uz, 0, j, -2) + g.dzzc2 * N(uz, 0, j, -1) + g.dzzc3 * N(uz, 0, j, 0) +
+                         g.dzzc4 * N(uz, 0, j, 1) + g.dzzc5 * N(uz, 0, j, 2);
+      cz = -h31 - h32 - h33 + cf.xnu * (d11 + d33) + cf.gz;
+      const double gp = g.dzc1 * N(p, 0, j, -2) + g.dzc2 * N(p, 0, j, -1) + g.dzc3 * N(p, 0, j, 0) +
+                        g.dzc4 * N(p, 0, j, 1);
+      rz = cf.gamma * cz + cf.theta * hz[o] - cf.alpha * gp + 2.0 * cf.beta * d22;
+    }
+
+    if (cf.scalar) {
+      // ---- scalar transport (:302-357)
+      auto shx = [&](int di) {
+        return c1 * N(sc, di - 2, j, 0) + c2 * N(sc, di - 1, j, 0) + c3 * N(sc, di, j, 0) + c4 * N(sc, di + 1, j, 0);
+      };
+      auto shz = [&](int dk) {
+        return c1 * N(sc, 0, j, dk - 2) + c2 * N(sc, 0, j, dk - 1) + c3 * N(sc, 0, j, dk) + c4 * N(sc, 0, j, dk + 1);
+      };
+      const double shyc = IY1 * N(sc, 0, jm, 0) + IY2 * N(sc, 0, j, 0);
+      const double shyp = IY3 * N(sc, 0, j, 0) + IY4 * N(sc, 0, jp, 0);
+      const double x0 = shx(0), z0 = shz(0);
+      const double e1 = cf.gev[0] * x0, e2 = cf.gev[1] * shyc, e3 = cf.gev[2] * z0;
+      rx = rx + cf.gamma * e1;
+      ry = ry + cf.gamma * e2;
+      rz = rz + cf.gamma * e3;
+      cx = cx + e1;
+      cy = cy + e2;
+      cz = cz + e3;
+      const double f0 = cf.fvv[0], f1 = cf.fvv[1], f2 = cf.fvv[2];
+      const double hc1 = g.dxc1 * (N(ux, -1, j, 0) + f0) * shx(-1) + g.dxc2 * (N(ux, 0, j, 0) + f0) * x0 +
+                         g.dxc3 * (N(ux, 1, j, 0) + f0) * shx(1) + g.dxc4 * (N(ux, 2, j, 0) + f0) * shx(2);
+      const double hc2 = ((N(uy, 0, jp, 0) + f1) * shyp - (N(uy, 0, j, 0) + f1) * shyc) * rdyp;
+      const double hc3 = g.dzc1 * (N(uz, 0, j, -1) + f2) * shz(-1) + g.dzc2 * (N(uz, 0, j, 0) + f2) * z0 +
+                         g.dzc3 * (N(uz, 0, j, 1) + f2) * shz(1) + g.dzc4 * (N(uz, 0, j, 2) + f2) * shz(2);
+      const double d11 = g.dxxc1 * N(sc, -2, j, 0) + g.dxxc2 * N(sc, -1, j, 0) + g.dxxc3 * N(sc, 0, j, 0) +
+                         g.dxxc4 * N(sc, 1, j, 0) + g.dxxc5 * N(sc, 2, j, 0);
+      const double d22 = M[M_AP2CSC * W + j] * N(sc, 0, jp, 0) + M[M_AC2CSC * W + j] * N(sc, 0, j, 0) +
+                         M[M_AM2CSC * W + j] * N(sc, 0, jm, 0);
+      const double d33 = g.dzzc1 * N(sc, 0, j, -2) + g.dzzc2 * N(sc, 0, j, -1) + g.dzzc3 * N(sc, 0, j, 0) +
+                         g.dzzc4 * N(sc, 0, j, 1) + g.dzzc5 * N(sc, 0, j, 2);
+      const double cc = -hc1 - hc2 - hc3 + cf.sdiff * (d11 + d33);
+      rhsc[o] = cf.gamma * cc + cf.theta * hc[o] + 2.0 * cf.beta_sc * d22;
+      hc[o] = cc;
+    }
+    rhsx[o] = rx;
+    rhsy[o] = ry;
+    rhsz[o] = rz;
+    hx[o] = cx;
+    hy[o] = cy;
+    hz[o] = cz;
+  }
+  if (cf.s3_mode) {
+    const double t = block_sum(s3, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+      s3part[b] = t;
+    }
+  }
+}
+
+// constant flow rate (:269-300): sums the partials in a fixed order and stores
+// the shift added to RhsX and the PrGradAver increment
+__global__ void __launch_bounds__(256) k_flowrate(const double *__restrict__ part, int n, double *__restrict__ scal,
+                                                 int scalar_build, double xnu, double ubulk, double ucrf,
+                                                 double nxnz, double yly, double alpha, double alphaC, double dt) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    if (scalar_build) {
+      const double s3 = tot / nxnz / yly;
+      const double dpm = ubulk - ucrf - s3;
+      scal[S_DP1NS] = alphaC * dpm;
+      scal[S_PRGRAD] = scal[S_PRGRAD] + dpm * alphaC / dt;
+    } else {
+      const double dpm = xnu * tot / nxnz / yly;
+      scal[S_DP1NS] = -(alpha * dpm);
+      scal[S_PRGRAD] = scal[S_PRGRAD] + dpm * alphaC;
+    }
+  }
+}
+
+// RhsX += shift over the interior (the standalone clcRhsXYZ result)
+__global__ void k_add_shift(double *__restrict__ a, const double *__restrict__ scal, Geo g) {
+  const long long total = (long long)g.nx * g.ny * g.nz;
+  const double s = scal[S_DP1NS];
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long long)gridDim.x * blockDim.x) {
+    const long long i = q % g.nx, r = q / g.nx;
+    const long long j = r % g.ny + 1, k = r / g.ny;
+    const long long o = i + (long long)g.nx * (j + (long long)g.nyh * k);
+    a[o] = a[o] + s;
+  }
+}
+
+// ---- clcPrSrc (:48-77) ------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prsrc(Geo g, const double *__restrict__ ux, const double *__restrict__ uy,
+                                              const double *__restrict__ uz, double *__restrict__ prsrc,
+                                              double sudtal, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y + 1;
+  const int k = blockIdx.z;
+  double dm = 0.0;
+  if (i < g.nx && j <= g.ny) {
+    const Ngh N(g, i, k);
+    const double rdyp = g.met[M_RDYP * g.nyh + j];
+    const double dudx = g.dxc1 * N(ux, -1, j, 0) + g.dxc2 * N(ux, 0, j, 0) + g.dxc3 * N(ux, 1, j, 0) +
+                        g.dxc4 * N(ux, 2, j, 0);
+    const double dvdy = (N(uy, 0, j + 1, 0) - N(uy, 0, j, 0)) * rdyp;
+    const double dwdz = g.dzc1 * N(uz, 0, j, -1) + g.dzc2 * N(uz, 0, j, 0) + g.dzc3 * N(uz, 0, j, 1) +
+                        g.dzc4 * N(uz, 0, j, 2);
+    const double rdiv = dudx + dvdy + dwdz;
+    dm = fabs(rdiv);
+    if (prsrc) prsrc[N.ii[3] + N.nxl * j + N.kk[3]] = sudtal * rdiv;
+  }
+  const double t = block_max(dm, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+}
+
+__global__ void __launch_bounds__(256) k_final_max(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
+  const double t = block_max(m, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+__global__ void __launch_bounds__(256) k_final_sum(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+// ---- FluidVelUpdate (:82-107) and PressureUpdate (:512-532) ---------------
+__global__ void __launch_bounds__(256) k_vel_p_update(Geo g, const double *__restrict__ phi, double *__restrict__ ux,
+                                                     double *__restrict__ uy, double *__restrict__ uz,
+                                                     double *__restrict__ p, double alpha, double beta, int do_vel,
+                                                     int do_p) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y + 1;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j > g.ny) return;
+  const Ngh N(g, i, k);
+  const long long o = N.ii[3] + N.nxl * j + N.kk[3];
+  const int W = g.nyh;
+  if (do_vel) {
+    const double rdyc = g.met[M_RDYC * W + j];
+    const double dpdx = g.dxc1 * N(phi, -2, j, 0) + g.dxc2 * N(phi, -1, j, 0) + g.dxc3 * N(phi, 0, j, 0) +
+                        g.dxc4 * N(phi, 1, j, 0);
+    const double dpdy = (N(phi, 0, j, 0) - N(phi, 0, j - 1, 0)) * rdyc;
+    const double dpdz = g.dzc1 * N(phi, 0, j, -2) + g.dzc2 * N(phi, 0, j, -1) + g.dzc3 * N(phi, 0, j, 0) +
+                        g.dzc4 * N(phi, 0, j, 1);
+    ux[o] = ux[o] - alpha * dpdx;
+    uy[o] = uy[o] - alpha * dpdy;
+    uz[o] = uz[o] - alpha * dpdz;
+  }
+  if (do_p) {
+    const double bp = -beta * g.met[M_AP2PH * W + j];
+    const double bc = -beta * g.met[M_AC2PH * W + j] + 1.0;
+    const double bm = -beta * g.met[M_AM2PH * W + j];
+    p[o] = p[o] + bp * N(phi, 0, j + 1, 0) + bc * N(phi, 0, j, 0) + bm * N(phi, 0, j - 1, 0);
+  }
+}
+
+// ---- wall boundary conditions (m_BC_and_Halo) on the (i, k) plane ---------
+// mode 0: SetBC_and_UpdateHalo, 1: ...ForPrSrc, 2: _pr, 3: _scalar, 4: Update_uy_ym
+struct BcArgs {
+  int mode, flow_type;
+  double uxb_m, uxb_p, uyb_p, uzb_m, uzb_p;
+  double r1m, r2m, r1p, r2p;
+};
+
+__global__ void k_bc(Geo g, BcArgs b, double *__restrict__ a0, double *__restrict__ a1, double *__restrict__ a2,
+                     double *__restrict__ pl0, double *__restrict__ pl1) {
+  const long long n = (long long)g.nx * g.nz;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+    const long long i = q % g.nx, k = q / g.nx;
+    const long long base = i + g.plane * k;
+    const long long r0 = base, r1 = base + g.nx, rn = base + (long long)g.nx * g.ny,
+                    rnp = base + (long long)g.nx * (g.ny + 1);
+    switch (b.mode) {
+      case 0:  // src/m_BC_and_Halo.f90:32-58
+        if (b.flow_type == 1) {
+          a0[rnp] = b.uxb_p * 2.0 - a0[rn];
+          a1[rnp] = b.uyb_p;
+          a2[rnp] = b.uzb_p * 2.0 - a2[rn];
+        } else {
+          a0[rnp] = a0[rn];
+          a1[rnp] = 0.0;
+          a2[rnp] = a2[rn];
+        }
+        a0[r0] = b.uxb_m * 2.0 - a0[r1];
+        a1[r1] = pl0[q];
+        a2[r0] = b.uzb_m * 2.0 - a2[r1];
+        break;
+      case 1:  // :78-98
+        a1[rnp] = (b.flow_type == 1) ? b.uyb_p : 0.0;
+        a1[r1] = pl0[q];
+        break;
+      case 2:  // :115-120
+        a0[r0] = a0[r1];
+        a0[rnp] = a0[rn];
+        break;
+      case 3:  // :137-176
+        a0[r0] = b.r1m * a0[r1] + b.r2m;
+        a0[rnp] = b.r1p * a0[rn] + b.r2p;
+        break;
+      case 4: {  // Update_uy_ym: duy = uy_ym(old) -> uy_ym = 0 -> duy = uy_ym - duy
+        const double old = pl0[q];
+        pl0[q] = 0.0;
+        pl1[q] = pl0[q] - old;
+      } break;
+      default: break;
+    }
+  }
+}
+
+// ---- monitors (m_Tools) -------------------------------------------------------
+// which: 0 CFL max (:24-49), 1 divergence max (:54-81), 2..4 |ux|,|uy|,|uz| max (:86-110),
+//        5 sum ux*dyp (:256-275), 6..8 sums dyp*(ux, c, c^2) (:281-301)
+__global__ void __launch_bounds__(256) k_monitor(Geo g, int which, const double *__restrict__ ux,
+                                                const double *__restrict__ uy, const double *__restrict__ uz,
+                                                const double *__restrict__ sc, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y + 1;
+  const int k = blockIdx.z;
+  double v = 0.0;
+  if (i < g.nx && j <= g.ny) {
+    const Ngh N(g, i, k);
+    const int W = g.nyh;
+    switch (which) {
+      case 0:
+        v = fabs(N(ux, 0, j, 0) + N(ux, 1, j, 0)) * g.rdx + fabs(N(uy, 0, j, 0) + N(uy, 0, j + 1, 0)) * g.met[M_RDYP * W + j] +
+            fabs(N(uz, 0, j, 0) + N(uz, 0, j, 1)) * g.rdz;
+        break;
+      case 1: {
+        const double dudx = g.dxc1 * N(ux, -1, j, 0) + g.dxc2 * N(ux, 0, j, 0) + g.dxc3 * N(ux, 1, j, 0) +
+                            g.dxc4 * N(ux, 2, j, 0);
+        const double dvdy = (N(uy, 0, j + 1, 0) - N(uy, 0, j, 0)) * g.met[M_RDYP * W + j];
+        const double dwdz = g.dzc1 * N(uz, 0, j, -1) + g.dzc2 * N(uz, 0, j, 0) + g.dzc3 * N(uz, 0, j, 1) +
+                            g.dzc4 * N(uz, 0, j, 2);
+        v = fabs(dudx + dvdy + dwdz);
+      } break;
+      case 2: v = fabs(N(ux, 0, j, 0)); break;
+      case 3: v = fabs(N(uy, 0, j, 0)); break;
+      case 4: v = fabs(N(uz, 0, j, 0)); break;
+      case 5:
+      case 6: v = N(ux, 0, j, 0) * g.met[M_DYP * W + j]; break;
+      case 7: v = g.met[M_DYP * W + j] * N(sc, 0, j, 0); break;
+      case 8: {
+        const double s = N(sc, 0, j, 0);
+        v = g.met[M_DYP * W + j] * s * s;
+      } break;
+      default: break;
+    }
+  }
+  const double t = (which <= 4) ? block_max(v, sh) : block_sum(v, sh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+}
+
+// ---- host launchers ----------------------------------------------------------
+static dim3 pt_grid(const ptc_ctx *c) {
+  return dim3((unsigned)((c->nx + 63) / 64), (unsigned)((c->ny + 3) / 4), (unsigned)c->nz);
+}
+static int pt_nblocks(const ptc_ctx *c) {
+  const dim3 gr = pt_grid(c);
+  return (int)(gr.x * gr.y * gr.z);
+}
+
+static int ensure_red(ptc_ctx *c, int n) {
+  if (c->red_cap < n) {
+    if (c->d_red) (void)hipFree(c->d_red);
+    c->d_red = nullptr;
+    PTC_HIP(hipMalloc(&c->d_red, sizeof(double) * (size_t)n));
+    c->red_cap = n;
+  }
+  return PTC_OK;
+}
+
+int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *sc, double *rhsx,
+               double *rhsy, double *rhsz, double *rhsc, double *hx, double *hy, double *hz, double *hc,
+               const double *p) {
+  RhsCoef cf{};
+  cf.gamma = c->pmGamma;
+  cf.theta = c->pmTheta;
+  cf.alpha = c->pmAlpha;
+  cf.beta = c->pmBeta;
+  cf.beta_sc = c->pmBetaSc;
+  cf.xnu = c->prm.xnu;
+  cf.ucrf = c->prm.ucrf;
+  cf.gx = c->prm.gravity[0];
+  cf.gy = c->prm.gravity[1];
+  cf.gz = c->prm.gravity[2];
+  cf.sdiff = c->prm.sdiff;
+  for (int q = 0; q < 3; ++q) {
+    cf.gev[q] = c->prm.gev[q];
+    cf.fvv[q] = c->prm.fvv[q];
+  }
+  cf.scalar = c->prm.scalar ? 1 : 0;
+  cf.s3_mode = c->prm.is_ux_const ? (c->prm.scalar ? 2 : 1) : 0;
+  const int nb = pt_nblocks(c);
+  PTC_TRY(ensure_red(c, nb));
+  hipLaunchKernelGGL(k_rhs_xyz, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, cf, ux, uy, uz, sc, p, rhsx, rhsy,
+                     rhsz, rhsc, hx, hy, hz, hc, c->d_red);
+  PTC_HIP(hipGetLastError());
+  if (cf.s3_mode) {
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,
+                       c->prm.xnu, c->prm.ubulk, c->prm.ucrf, (double)(c->nx * c->nz), c->yly, c->pmAlpha,
+                       c->pmAlphaC, c->dt);
+    PTC_HIP(hipGetLastError());
+  }
+  return PTC_OK;
+}
+
+int launch_add_shift(ptc_ctx *c, double *rhsx) {
+  hipLaunchKernelGGL(k_add_shift, dim3(c->n_cu * 8), dim3(256), 0, c->stream, rhsx, c->d_scal, c->geo);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
+
+int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
+                 double *divmax_host) {
+  const int nb = pt_nblocks(c);
+  PTC_TRY(ensure_red(c, nb));
+  hipLaunchKernelGGL(k_prsrc, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, ux, uy, uz, prsrc, 1.0 / c->pmAlpha,
+                     c->d_red);
+  PTC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_DIVMAX);
+  PTC_HIP(hipGetLastError());
+  if (divmax_host) {
+    PTC_HIP(hipMemcpyAsync(c->h_pinned, c->d_scal + S_DIVMAX, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PTC_HIP(hipStreamSynchronize(c->stream));
+    *divmax_host = c->h_pinned[0];
+    c->divmax_prsrc = c->h_pinned[0];
+  }
+  return PTC_OK;
+}
+
+int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p) {
+  const int dv = (ux && uy && uz) ? 1 : 0, dp = p ? 1 : 0;
+  hipLaunchKernelGGL(k_vel_p_update, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, phi, ux, uy, uz, p, c->pmAlpha,
+                     c->pmBeta, dv, dp);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_vel_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz) {
+  return launch_vel_p_update(c, phi, ux, uy, uz, nullptr);
+}
+
+int launch_pressure_update(ptc_ctx *c, double *p, const double *phi) {
+  return launch_vel_p_update(c, phi, nullptr, nullptr, nullptr, p);
+}
+
+static int launch_bc(ptc_ctx *c, const BcArgs &b, double *a0, double *a1, double *a2, double *p0, double *p1) {
+  const long long n = (long long)c->nx * c->nz;
+  const int nblk = (int)std::min<long long>((n + 255) / 256, (long long)c->n_cu * 8);
+  hipLaunchKernelGGL(k_bc, dim3(std::max(1, nblk)), dim3(256), 0, c->stream, c->geo, b, a0, a1, a2, p0, p1);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, bool prsrc_only) {
+  BcArgs b{};
+  b.mode = prsrc_only ? 1 : 0;
+  b.flow_type = c->prm.flow_type;
+  // uxBcValue(ym_dir = 3), (yp_dir = 4) -> indices 2, 3
+  b.uxb_m = c->prm.uxbc[2];
+  b.uxb_p = c->prm.uxbc[3];
+  b.uyb_p = c->prm.uybc[3];
+  b.uzb_m = c->prm.uzbc[2];
+  b.uzb_p = c->prm.uzbc[3];
+  return launch_bc(c, b, ux, uy, uz, const_cast<double *>(uy_ym), nullptr);
+}
+
+int launch_bc_pr(ptc_ctx *c, double *p) {
+  BcArgs b{};
+  b.mode = 2;
+  return launch_bc(c, b, p, nullptr, nullptr, nullptr, nullptr);
+}
+
+int launch_bc_scalar(ptc_ctx *c, double *s) {
+  BcArgs b{};
+  b.mode = 3;
+  const double *v = c->prm.sbc_val;
+  const double dyc1 = c->h_met[(size_t)M_DYC * (c->ny + 2) + 1];
+  const double dycp = c->h_met[(size_t)M_DYC * (c->ny + 2) + c->ny + 1];
+  // src/m_BC_and_Halo.f90:137-171
+  switch (c->prm.sbc_opt[0]) {
+    case -1: b.r1m = -1.0; b.r2m = 2.0 * v[0]; break;
+    case -2: b.r1m = 1.0; b.r2m = -dyc1 * v[2]; break;
+    case -3:
+      b.r1m = (2.0 + v[0] * dyc1) / (2.0 - v[0] * dyc1);
+      b.r2m = -2.0 * v[2] * dyc1 / (2.0 - v[0] * dyc1);
+      break;
+    default: set_error("ScalarBcOption(1) wrong !!!"); return PTC_E_ARG;
+  }
+  switch (c->prm.sbc_opt[1]) {
+    case -1: b.r1p = -1.0; b.r2p = 2.0 * v[1]; break;
+    case -2: b.r1p = 1.0; b.r2p = dycp * v[3]; break;
+    case -3:
+      b.r1p = (2.0 - v[1] * dycp) / (2.0 + v[1] * dycp);
+      b.r2p = 2.0 * v[3] * dycp / (2.0 + v[1] * dycp);
+      break;
+    default: set_error("ScalarBcOption(2) wrong !!!"); return PTC_E_ARG;
+  }
+  return launch_bc(c, b, s, nullptr, nullptr, nullptr, nullptr);
+}
+
+int launch_uy_ym(ptc_ctx *c, double *uy_ym, double *duy_ym) {
+  BcArgs b{};
+  b.mode = 4;
+  return launch_bc(c, b, nullptr, nullptr, nullptr, uy_ym, duy_ym);
+}
+
+// result to host (synchronises)
+int launch_monitor(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *s,
+                   double *out) {
+  const int nb = pt_nblocks(c);
+  PTC_TRY(ensure_red(c, nb));
+  hipLaunchKernelGGL(k_monitor, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, which, ux, uy, uz, s, c->d_red);
+  PTC_HIP(hipGetLastError());
+  if (which <= 4)
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
+  else
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
+  PTC_HIP(hipGetLastError());
+  PTC_HIP(hipMemcpyAsync(c->h_pinned, c->d_scal + S_MON0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  *out = c->h_pinned[0];
+  return PTC_OK;
+}
+
+int set_prgrad(ptc_ctx *c, double v) {
+  c->h_pinned[1] = v;
+  PTC_HIP(hipMemcpyAsync(c->d_scal + S_PRGRAD, c->h_pinned + 1, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  return PTC_OK;
+}
+
+int get_device_scalar(ptc_ctx *c, int slot, double *out) {
+  PTC_HIP(hipMemcpyAsync(c->h_pinned + 2, c->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  *out = c->h_pinned[2];
+  return PTC_OK;
+}
+
+}  // namespace ptc

@@ -1,0 +1,204 @@
+"""Host-side mirror of the reference's hot-path interface over the HIP C-ABI.
+
+One ``Solver`` is one case: it stands for the module globals of m_Parameters,
+m_MeshAndMetries and m_Variables, and exposes the reference procedures under
+their own names (m_Poisson: InitPoissonSolver / clcPPE /
+Destory_Poisson_FFT_Plan; m_TScheme: clcRhsXYZ, clcU1Hat, ...; m_BC_and_Halo:
+SetBC_and_UpdateHalo*, ...).  Arguments are field ids (``capi.F_*``) naming
+device-resident arrays, so callers alias buffers as the reference does.  Errors
+raise ``capi.PtcError`` where the reference aborts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check
+
+
+class Solver:
+    def __init__(self, case, vel_limit: float = 0.0, div_limit: float = 0.0):
+        self.L = capi.load()
+        self.case = case
+        self.nx, self.ny, self.nz = case.nx, case.ny, case.nz
+        g, p = capi.params_from_case(case, vel_limit, div_limit)
+        self.yp = np.ascontiguousarray(case.yp(), dtype=np.float64)
+        h = C.c_void_p()
+        check(self.L.ptc_create(C.byref(g), C.byref(p), _dp(self.yp), C.byref(h)), "ptc_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ptc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- transfers ---------------------------------------------------------
+    def shape(self, layout):
+        if layout == capi.L_HALO:
+            return (self.nx + 4, self.ny + 2, self.nz + 4)
+        if layout == capi.L_INTERIOR:
+            return (self.nx, self.ny, self.nz)
+        return (self.nx, self.nz)
+
+    def upload(self, fid: int, arr: np.ndarray, layout: int = capi.L_HALO):
+        a = np.asfortranarray(arr, dtype=np.float64)
+        assert a.shape == self.shape(layout), (a.shape, self.shape(layout))
+        check(self.L.ptc_upload(self.h, fid, _dp(a), layout), "ptc_upload")
+
+    def download(self, fid: int, layout: int = capi.L_HALO) -> np.ndarray:
+        a = np.zeros(self.shape(layout), order="F")
+        check(self.L.ptc_download(self.h, fid, _dp(a), layout), "ptc_download")
+        return a
+
+    def fill(self, fid: int, value: float = 0.0):
+        check(self.L.ptc_fill(self.h, fid, value), "ptc_fill")
+
+    def copy(self, dst: int, src: int):
+        check(self.L.ptc_copy(self.h, dst, src), "ptc_copy")
+
+    def synchronize(self):
+        check(self.L.ptc_synchronize(self.h), "ptc_synchronize")
+
+    def metric(self, which: int) -> np.ndarray:
+        out = np.zeros(self.ny + 2)
+        check(self.L.ptc_get_metric(self.h, which, _dp(out)), "ptc_get_metric")
+        return out
+
+    # ---- m_Parameters --------------------------------------------------------
+    def PMcoeUpdate(self, ns: int, dt: float):
+        check(self.L.ptc_pm_coe_update(self.h, ns, dt), "PMcoeUpdate")
+
+    def coe(self) -> np.ndarray:
+        o = np.zeros(9)
+        check(self.L.ptc_get_coe(self.h, _dp(o)), "ptc_get_coe")
+        return o
+
+    def set_coe(self, o):
+        o = np.ascontiguousarray(o, dtype=np.float64)
+        check(self.L.ptc_set_coe(self.h, _dp(o)), "ptc_set_coe")
+
+    # ---- m_Poisson ----------------------------------------------------------
+    def InitPoissonSolver(self, prsrc: int = capi.F_ARR1, prphiHalo: int = capi.F_HALO):
+        check(self.L.ptc_init_poisson_solver(self.h, prsrc, prphiHalo), "InitPoissonSolver")
+
+    def clcPPE(self, prsrc: int = capi.F_ARR1, prphiHalo: int = capi.F_HALO):
+        check(self.L.ptc_clc_ppe(self.h, prsrc, prphiHalo), "clcPPE")
+
+    def Destory_Poisson_FFT_Plan(self):
+        check(self.L.ptc_destroy_poisson_plan(self.h), "Destory_Poisson_FFT_Plan")
+
+    def poisson_config(self, prec=capi.PREC_FP64, tri=capi.TRI_AUTO, tile=0, sweeps=2):
+        check(self.L.ptc_poisson_config(self.h, prec, tri, tile, sweeps), "ptc_poisson_config")
+
+    def poisson_residual(self, src: int, dst: int) -> float:
+        r = C.c_double()
+        check(self.L.ptc_poisson_residual(self.h, src, dst, C.byref(r)), "ptc_poisson_residual")
+        return r.value
+
+    def poisson_stage_times(self, enable: int = -1):
+        ms = (C.c_float * 5)()
+        check(self.L.ptc_poisson_stage_times(self.h, enable, ms), "ptc_poisson_stage_times")
+        return list(ms)
+
+    # ---- m_TScheme ----------------------------------------------------------
+    def InitTimeScheme(self):
+        check(self.L.ptc_init_time_scheme(self.h), "InitTimeScheme")
+
+    def clcRhsXYZ(self, ux, uy, uz, scalar, rhsx, rhsy, rhsz, rhsc, histx, histy, histz, histc, pressure):
+        check(self.L.ptc_clc_rhs_xyz(self.h, ux, uy, uz, scalar, rhsx, rhsy, rhsz, rhsc, histx, histy,
+                                     histz, histc, pressure), "clcRhsXYZ")
+
+    def clcU1Hat(self, ux, rhsx):
+        check(self.L.ptc_clc_u1hat(self.h, ux, rhsx), "clcU1Hat")
+
+    def clcU2Hat(self, uy, rhsy, duy_ym):
+        check(self.L.ptc_clc_u2hat(self.h, uy, rhsy, duy_ym), "clcU2Hat")
+
+    def clcU3Hat(self, uz, rhsz):
+        check(self.L.ptc_clc_u3hat(self.h, uz, rhsz), "clcU3Hat")
+
+    def clcScalar(self, scalar, rhsc):
+        check(self.L.ptc_clc_scalar(self.h, scalar, rhsc), "clcScalar")
+
+    def clcPrSrc(self, ux, uy, uz, prsrc) -> float:
+        d = C.c_double()
+        check(self.L.ptc_clc_prsrc(self.h, ux, uy, uz, prsrc, C.byref(d)), "clcPrSrc")
+        return d.value
+
+    def FluidVelUpdate(self, phi, ux, uy, uz):
+        check(self.L.ptc_fluid_vel_update(self.h, phi, ux, uy, uz), "FluidVelUpdate")
+
+    def PressureUpdate(self, pressure, phi):
+        check(self.L.ptc_pressure_update(self.h, pressure, phi), "PressureUpdate")
+
+    # ---- m_BC_and_Halo ------------------------------------------------------
+    def SetBC_and_UpdateHalo(self, ux, uy, uz, uy_ym):
+        check(self.L.ptc_set_bc_and_update_halo(self.h, ux, uy, uz, uy_ym), "SetBC_and_UpdateHalo")
+
+    def SetBC_and_UpdateHaloForPrSrc(self, ux, uy, uz, uy_ym):
+        check(self.L.ptc_set_bc_and_update_halo_for_prsrc(self.h, ux, uy, uz, uy_ym),
+              "SetBC_and_UpdateHaloForPrSrc")
+
+    def SetBC_and_UpdateHalo_pr(self, p):
+        check(self.L.ptc_set_bc_and_update_halo_pr(self.h, p), "SetBC_and_UpdateHalo_pr")
+
+    def SetBC_and_UpdateHalo_scalar(self, s):
+        check(self.L.ptc_set_bc_and_update_halo_scalar(self.h, s), "SetBC_and_UpdateHalo_scalar")
+
+    def Update_uy_ym(self, uy_ym, duy_ym):
+        check(self.L.ptc_update_uy_ym(self.h, uy_ym, duy_ym), "Update_uy_ym")
+
+    # ---- m_Tools monitors ---------------------------------------------------
+    def CalcMaxCFL(self, ux, uy, uz) -> float:
+        d = C.c_double()
+        check(self.L.ptc_calc_max_cfl(self.h, ux, uy, uz, C.byref(d)), "CalcMaxCFL")
+        return d.value
+
+    def CheckDivergence(self, ux, uy, uz) -> float:
+        d = C.c_double()
+        check(self.L.ptc_check_divergence(self.h, ux, uy, uz, C.byref(d)), "CheckDivergence")
+        return d.value
+
+    def CalcVmax(self, ux, uy, uz) -> np.ndarray:
+        v = np.zeros(3)
+        check(self.L.ptc_calc_vmax(self.h, ux, uy, uz, _dp(v)), "CalcVmax")
+        return v
+
+    def CalcUxAver(self, ux) -> float:
+        d = C.c_double()
+        check(self.L.ptc_calc_ux_aver(self.h, ux, C.byref(d)), "CalcUxAver")
+        return d.value
+
+    def ClcMeanValue(self, ux, scalar) -> np.ndarray:
+        v = np.zeros(3)
+        check(self.L.ptc_clc_mean_value(self.h, ux, scalar, _dp(v)), "ClcMeanValue")
+        return v
+
+    # ---- m_ChannelSystem ----------------------------------------------------
+    def ChannelIterate(self, nsteps: int = 1) -> np.ndarray:
+        mon = np.zeros(7)
+        check(self.L.ptc_channel_iterate(self.h, nsteps, _dp(mon)), "ChannelIterate")
+        return mon
+
+    def scalar(self, which: int) -> float:
+        d = C.c_double()
+        check(self.L.ptc_get_scalar(self.h, which, C.byref(d)), "ptc_get_scalar")
+        return d.value
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))

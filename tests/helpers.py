@@ -1,0 +1,51 @@
+"""Shared helpers of the test suite (inputs, comparisons)."""
+from __future__ import annotations
+
+import numpy as np
+
+from paratc_amd import rng
+
+
+def rel_err(a, b) -> float:
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = max(float(np.abs(b).max()), 1e-300)
+    return float(np.abs(a - b).max()) / scale
+
+
+def inner(h):
+    """Interior (x 1..nx, y 1..ny, z 1..nz) of a haloed (nx+4, ny+2, nz+4) array."""
+    return h[2:-2, 1:-1, 2:-2]
+
+
+def inner_y(h):
+    """Interior in x and z with the y ghost rows (y 0..ny+1)."""
+    return h[2:-2, :, 2:-2]
+
+
+def random_interior(shape, seed, stream=0):
+    n = int(np.prod(shape))
+    return rng.normal(seed, n, stream).reshape(shape, order="F")
+
+
+def compatible_rhs(orc_or_dyp, f):
+    """Remove the dyp-weighted mean so that sum_j dyp_j <f>_j = 0 (Neumann compatibility,
+    src/m_MeshAndMetries.f90:239-247)."""
+    dyp = orc_or_dyp
+    ny = f.shape[1]
+    w = dyp[1:ny + 1]
+    m = (f * w[None, :, None]).sum() / (w.sum() * f.shape[0] * f.shape[2])
+    return np.asfortranarray(f - m)
+
+
+def random_halo(shape_interior, seed, amp=1.0, stream=0):
+    """A haloed array whose interior and y-ghost rows are random; x/z halos are the
+    periodic images (as after a full myupdate_halo)."""
+    nx, ny, nz = shape_interior
+    core = amp * rng.uniform(seed, nx * (ny + 2) * nz, stream).reshape((nx, ny + 2, nz), order="F")
+    core = 2.0 * core - amp
+    h = np.zeros((nx + 4, ny + 2, nz + 4), order="F")
+    ii = (np.arange(-1, nx + 3) - 1) % nx
+    kk = (np.arange(-1, nz + 3) - 1) % nz
+    h[:, :, :] = core[ii][:, :, kk]
+    return h

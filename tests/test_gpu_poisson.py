@@ -1,0 +1,114 @@
+"""GPU parity of the Poisson solve (m_Poisson) against the CPU oracle.
+
+The oracle restates clcPPE_x1 (src/m_Poisson.f90:352-452) with FFTW's R2HC /
+HC2R semantics; the HIP path uses complex half spectra.  Both diagonalise the
+same operator, so they agree to rounding: the bar is 1e-12 relative (FP64).
+"""
+import numpy as np
+import pytest
+
+from helpers import inner, rel_err
+from oracle.oracle import Oracle
+from paratc_amd import capi, case as CS, ic
+from paratc_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+# (nx, ny, nz, gamma): radix-2 and the odd radices 3, 5, 7 in x/2 and z
+GRIDS = [
+    (16, 8, 16, 1.5),
+    (32, 24, 16, 2.0),
+    (48, 20, 30, 1.8),     # nx/2 = 24 = 8*3, nz = 30 = 2*3*5
+    (28, 16, 14, 2.0),     # nx/2 = 14 = 2*7,  nz = 14 = 2*7
+    (40, 12, 70, 1.6),     # nx/2 = 20 = 4*5,  nz = 70 = 2*5*7
+    (128, 96, 128, 1.8),   # the 128x96x128 channel grid
+    (96, 140, 50, 2.2),    # ny = 140 as in the large case
+]
+
+
+def solve_both(grid, tri, seed=1, compat=True):
+    nx, ny, nz, gam = grid
+    case = CS.case_poisson(nx, ny, nz, gam)
+    f = ic.poisson_rhs(case, seed=seed) if compat else np.asfortranarray(
+        np.random.default_rng(seed).standard_normal((nx, ny, nz)))
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    am, ac, ap = (orc.metric(n)[1:ny + 1] for n in ("am2ph", "ac2ph", "ap2ph"))
+    rdx2, rdz2 = (nx / case.xlx) ** 2, (nz / case.zlz) ** 2
+    lnorm = float((np.abs(am) + np.abs(ac) + np.abs(ap)).max() + (1460 + 2 * 838) / 576 * (rdx2 + rdz2))
+    with Solver(case) as s:
+        s.poisson_config(capi.PREC_FP64, tri, 0, 0)
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        out = s.download(capi.F_HALO, capi.L_INTERIOR)
+        src_after = s.download(capi.F_ARR1, capi.L_INTERIOR)
+        res = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+    return f, inner(ref), out, src_after, res / lnorm
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g[:3])))
+@pytest.mark.parametrize("tri", [capi.TRI_PARTITIONED, capi.TRI_THOMAS], ids=["part", "thomas"])
+def test_poisson_matches_oracle(grid, tri):
+    f, ref, out, src_after, res = solve_both(grid, tri)
+    print(f"grid={grid} tri={tri} rel={rel_err(out, ref):.3e} res={res:.3e}", flush=True)
+    assert rel_err(out, ref) <= TOL
+    # the input is not modified on the device
+    assert np.array_equal(src_after, f)
+    # pinned (0,0) line: the x-z mean of row j=1 is zero (src/m_Poisson.f90:400-404)
+    assert abs(out[:, 0, :].mean()) <= 1e-13 * np.abs(out).max()
+    # compatible RHS: residual at round-off relative to |L| |phi| (res is divided by |L|)
+    assert res <= 1e-13 * np.abs(out).max()
+
+
+def test_poisson_incompatible_rhs_matches_oracle():
+    """Without the mean removed, row j=1 of the (0,0) line absorbs the
+    inconsistency in both implementations."""
+    f, ref, out, _, _ = solve_both((32, 24, 16, 2.0), capi.TRI_PARTITIONED, seed=5, compat=False)
+    assert rel_err(out, ref) <= TOL
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
+def test_poisson_z_tiles_agree(tile):
+    case = CS.case_poisson(64, 32, 64, 2.0)
+    f = ic.poisson_rhs(case, seed=3)
+    outs = []
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        for t in (16, tile):
+            s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, t, 0)
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+            outs.append(s.download(capi.F_HALO, capi.L_INTERIOR))
+    assert rel_err(outs[1], outs[0]) <= 1e-14
+
+
+def test_init_poisson_solver_autotunes_and_zeroes():
+    case = CS.case_poisson(64, 48, 32, 2.0)
+    f = ic.poisson_rhs(case, seed=4)
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+        # src/m_Poisson.f90:174 -- both arrays are zero afterwards
+        assert not s.download(capi.F_ARR1, capi.L_INTERIOR).any()
+        assert not s.download(capi.F_HALO, capi.L_INTERIOR).any()
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.clcPPE()
+        out = s.download(capi.F_HALO, capi.L_INTERIOR)
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    assert rel_err(out, inner(ref)) <= TOL
+
+
+def test_poisson_deterministic():
+    case = CS.case_poisson(64, 32, 48, 2.0)
+    f = ic.poisson_rhs(case, seed=6)
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.clcPPE()
+        a = s.download(capi.F_HALO, capi.L_INTERIOR)
+        s.clcPPE()
+        b = s.download(capi.F_HALO, capi.L_INTERIOR)
+    assert np.array_equal(a, b)
