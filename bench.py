@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X hot path (one GPU).
+
+Workload (the metric's headline configuration): the Poisson drop-in alone,
+staggered L p = f on 1024 x 256 x 512 (periodic x/z, Neumann y, tanh-stretched
+y with gamma = 2.0), f ~ N(0,1) per cell from seed 1 minus its dyp-weighted
+mean, FP64.  One step = one clcPPE solve of a resident right-hand side; the
+solves alternate between two independent resident right-hand sides (seed 1,
+streams 0 and 1).  Inputs are in HBM before the timed region starts.
+
+The JSON line also carries
+  * roofline: the dominant pass of the solve (algorithmic bytes 16 B per grid
+    point per pass: one FP64 read and one write, complex half spectra) over its
+    mean launch time, measured with HIP events on the library's stream;
+  * cpu_baseline: the CPU oracle (a port of the reference's clcPPE_x1) on the
+    same right-hand side, timed on this host's cores over a bounded sample;
+  * rk_substep: grid points per second of one RK3 substep of the 128 x 96 x 128
+    closed channel (ChannelIterate on the device, fixed dt = 2e-3).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts/s per RK substep + Poisson HBM GB/s; FP32+refine vs FP64; ensemble p50/p99"
+HBM_PEAK_GBS = 8000.0
+STAGES = ["x_fwd", "z_fwd", "y_solve", "z_inv", "x_inv"]
+STAGE_KERNELS = ["k_fft_x_fwd", "k_fft_z<fwd>", "k_tri_poisson_part", "k_fft_z<inv>", "k_fft_x_inv"]
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's own stream)."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        vp = C.c_void_p
+        self.L.hipEventCreate.argtypes = [C.POINTER(vp)]
+        self.L.hipEventRecord.argtypes = [vp, vp]
+        self.L.hipEventSynchronize.argtypes = [vp]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        self.L.hipEventDestroy.argtypes = [vp]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.L.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.L.hipEventSynchronize(e1) == 0
+        t = C.c_float()
+        assert self.L.hipEventElapsedTime(C.byref(t), e0, e1) == 0
+        return float(t.value)
+
+
+def progress(msg: str):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def bench_poisson(args, ev):
+    from paratc_amd import capi, case as CS, ic
+    from paratc_amd.solver import Solver
+
+    case = CS.case_poisson(1024, 256, 512, 2.0)
+    n = case.npts
+    s = Solver(case)
+    t0 = time.time()
+    f1 = ic.poisson_rhs(case, seed=1, stream=0)
+    f2 = ic.poisson_rhs(case, seed=1, stream=1)
+    t_gen = time.time() - t0
+    progress(f"generated two 1024x256x512 right-hand sides in {t_gen:.1f} s")
+    # InitPoissonSolver times the variants (tile shape, Thomas vs partitioned)
+    # and zeroes both arrays, so it runs before the inputs are uploaded.
+    s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+    s.upload(capi.F_ARR1, f1, capi.L_INTERIOR)
+    s.upload(capi.F_WORK1, f2, capi.L_INTERIOR)
+    progress(f"autotuned plan {s.poisson_get_config()}, inputs resident")
+    srcs = [capi.F_ARR1, capi.F_WORK1]
+    if args.calib_copy:
+        # a plain device copy of one field (8 B read + 8 B write per point) to
+        # scale the FETCH_SIZE / WRITE_SIZE counters of the Poisson passes
+        s.copy(capi.F_WORK2, capi.F_ARR1)
+    for w in range(args.warmup):
+        s.clcPPE(srcs[w % 2], capi.F_HALO)
+    s.synchronize()
+    stream = s.stream()
+    s.poisson_stage_times(1)  # record per-pass events from here on
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for k in range(args.steps):
+        s.clcPPE(srcs[k % 2], capi.F_HALO)
+    ev.record(e1, stream)
+    total_ms = ev.elapsed_ms(e0, e1)
+    stage_ms, nsolves = s.poisson_stage_times(0)
+    s.synchronize()
+    progress(f"timed {args.steps} solves: {total_ms / args.steps:.4f} ms per solve")
+    assert nsolves == args.steps
+    # correctness guard on the last solve: FP64 residual relative to |L| |p|
+    res = s.poisson_residual(srcs[(args.steps - 1) % 2], capi.F_HALO)
+    ms_per = total_ms / args.steps
+    per_stage = [m / args.steps for m in stage_ms]
+    dom = int(np.argmax(per_stage))
+    bytes_stage = 16.0 * n  # one FP64 read + one FP64 write per grid point
+    achieved = bytes_stage / (per_stage[dom] * 1e-3) / 1e9
+    solve_gbs = 5 * bytes_stage / (ms_per * 1e-3) / 1e9
+    out = {
+        "case": case, "solver": s, "f1": f1, "ms_per": ms_per, "per_stage": per_stage, "dom": dom,
+        "achieved": achieved, "solve_gbs": solve_gbs, "residual": res, "t_gen": t_gen,
+        "plan": s.poisson_get_config(),
+    }
+    return out
+
+
+def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
+    from oracle.oracle import Oracle
+
+    cores = host_cores()
+    orc = Oracle(case, threads=cores)
+    out = orc.halo()
+    times = []
+    t_start = time.time()
+    while len(times) < max_solves and (time.time() - t_start) < budget_s:
+        src = f.copy(order="F")
+        t0 = time.time()
+        orc.clcPPE(src, out, 1)
+        times.append(time.time() - t0)
+        progress(f"cpu baseline solve {len(times)}: {times[-1]:.2f} s on {cores} cores")
+    t = float(np.mean(times))
+    return {
+        "value": round(case.npts / t / 1e6, 3),
+        "unit": "Mpts/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{len(times)} full FP64 Poisson solves (clcPPE_x1 restatement, OpenMP) at "
+                  f"{case.nx}x{case.ny}x{case.nz}, mean {t:.3f} s per solve",
+    }
+
+
+def bench_rk_substep(ev, steps=30, warmup=3):
+    from paratc_amd import capi, case as CS, ic
+    from paratc_amd.solver import Solver
+
+    case = CS.case_channel(128, 96, 128, 1.8)
+    s = Solver(case)
+    ux, uy, uz = ic.channel_fields(case, seed=0, amp=0.1)
+    s.upload(capi.F_UX, ux)
+    s.upload(capi.F_UY, uy)
+    s.upload(capi.F_UZ, uz)
+    s.SetBC_and_UpdateHalo(capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_UY_YM)
+    s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+    s.L.ptc_channel_iterate(s.h, warmup, None)
+    s.synchronize()
+    stream = s.stream()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    rc = s.L.ptc_channel_iterate(s.h, steps, None)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1)
+    assert rc == 0, s.L.ptc_last_error()
+    mon = s.ChannelIterate(1)
+    nsub = steps * 3
+    t_sub = ms / nsub
+    progress(f"rk substep 128x96x128: {t_sub:.4f} ms")
+    s.close()
+    return {
+        "workload": "closed channel 128x96x128, RK3, dt 2e-3, FP64",
+        "mpts_per_s": round(case.npts / (t_sub * 1e-3) / 1e6, 2),
+        "ms_per_substep": round(t_sub, 4),
+        "hbm_gbs_alg": round(368.0 * case.npts / (t_sub * 1e-3) / 1e9, 1),
+        "divmax_after": float(mon[1]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rk", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="launch one plain field copy before the solves (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the whole domain is resident on a single MI355X "
+              "and this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    ev = HipEvents()
+    r = bench_poisson(args, ev)
+    case = r["case"]
+    cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
+    rk = None if args.no_rk else bench_rk_substep(ev)
+    n = case.npts
+    line = {
+        "metric": METRIC,
+        "value": round(n / (r["ms_per"] * 1e-3) / 1e6, 1),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "poisson_fp64_1024x256x512", "grid": [1024, 256, 512], "gamma": 2.0,
+                   "rhs": "N(0,1) seed 1 minus dyp-weighted mean, 2 resident RHS alternating",
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": STAGE_KERNELS[r["dom"]], "bytes_per_launch": 16 * n,
+                     "launch_ms": round(r["per_stage"][r["dom"]], 4)},
+        "cpu_baseline": cpu,
+        "poisson": {"solves_per_s": round(1e3 / r["ms_per"], 2),
+                    "hbm_gbs_alg": round(r["solve_gbs"], 1),
+                    "frac_of_peak": round(r["solve_gbs"] / HBM_PEAK_GBS, 4),
+                    "stage_ms": {k: round(v, 4) for k, v in zip(STAGES, r["per_stage"])},
+                    "residual_max": r["residual"],
+                    "plan": r["plan"]},
+        "rk_substep": rk,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

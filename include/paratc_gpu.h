@@ -131,6 +131,9 @@ int ptc_copy(ptc_ctx *ctx, int dst, int src);
 /* raw device address of a field (nx*(ny+2)*nz doubles, x fastest, y rows 0..ny+1) */
 int ptc_field_device_ptr(ptc_ctx *ctx, int field, void **dptr, long long *count);
 int ptc_synchronize(ptc_ctx *ctx);
+/* the context's HIP stream (a hipStream_t), so a caller can order its own work
+ * or time the library's kernels with events on the same stream */
+int ptc_get_stream(ptc_ctx *ctx, void **stream);
 /* y-metric table, ny+2 values (same numbering as the oracle header) */
 int ptc_get_metric(ptc_ctx *ctx, int which, double *out);
 
@@ -155,10 +158,15 @@ int ptc_destroy_poisson_plan(ptc_ctx *ctx);
  * tri = PTC_TRI_*, tile = z-FFT column tile (0 = auto), sweeps = refinement
  * sweeps for PTC_PREC_FP32_REFINE */
 int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
+/* the plan in use: out4 = [prec, tri, z tile, sweeps] (after autotuning) */
+int ptc_poisson_get_config(ptc_ctx *ctx, int *out4);
 /* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
-/* per-stage kernel times in ms of the last profiled solve (5 stages) */
-int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5);
+/* Per-stage kernel times: with enable = 1 every later solve records HIP events
+ * around its five passes; ms5 (may be NULL) receives the summed times in ms and
+ * *nsolves (may be NULL) the number of solves since the last reset.  enable =
+ * 0 stops recording, -1 only reads.  enable >= 0 resets the sums after reading. */
+int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5, int *nsolves);
 
 /* ---- m_TScheme (src/m_TScheme.f90:14-19) */
 int ptc_init_time_scheme(ptc_ctx *ctx);

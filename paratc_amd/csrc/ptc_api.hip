@@ -18,6 +18,7 @@ const char *last_error();
 int upload(ptc_ctx *c, int id, const double *host, int layout);
 int download(ptc_ctx *c, int id, double *host, int layout);
 int fill_device(ptc_ctx *c, double *a, long long n, double v);
+int copy_device(ptc_ctx *c, double *dst, const double *src, long long n);
 size_t field_count(const ptc_ctx *c, int id);
 int launch_add_shift(ptc_ctx *c, double *rhsx);
 const double *flowrate_shift_ptr(ptc_ctx *c);
@@ -223,8 +224,7 @@ int ptc_copy(ptc_ctx *ctx, int dst, int src) {
   PTC_TRY(field(ctx, dst, &d));
   PTC_TRY(field(ctx, src, &s));
   PTC_CHECK(field_count(ctx, dst) == field_count(ctx, src), PTC_E_ARG, "ptc_copy: field shapes differ");
-  PTC_HIP(hipMemcpyAsync(d, s, sizeof(double) * field_count(ctx, src), hipMemcpyDeviceToDevice, ctx->stream));
-  return PTC_OK;
+  return copy_device(ctx, d, s, (long long)field_count(ctx, src));
 }
 
 int ptc_field_device_ptr(ptc_ctx *ctx, int f, void **dptr, long long *count) {
@@ -240,6 +240,13 @@ int ptc_field_device_ptr(ptc_ctx *ctx, int f, void **dptr, long long *count) {
 int ptc_synchronize(ptc_ctx *ctx) {
   PTC_CTX(ctx);
   PTC_HIP(hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
+}
+
+int ptc_get_stream(ptc_ctx *ctx, void **stream) {
+  PTC_CTX(ctx);
+  PTC_CHECK(stream != nullptr, PTC_E_ARG, "null output pointer");
+  *stream = (void *)ctx->stream;
   return PTC_OK;
 }
 
@@ -312,6 +319,16 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps) {
   return poisson_build_plan(ctx);
 }
 
+int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
+  PTC_CTX(ctx);
+  PTC_CHECK(o != nullptr, PTC_E_ARG, "null output");
+  o[0] = ctx->pp.prec;
+  o[1] = ctx->pp.tri;
+  o[2] = ctx->pp.ztile;
+  o[3] = ctx->pp.sweeps;
+  return PTC_OK;
+}
+
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax) {
   PTC_CTX(ctx);
   PTC_CHECK(rmax != nullptr, PTC_E_ARG, "null output");
@@ -321,13 +338,12 @@ int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax) {
   return poisson_residual(ctx, s, d, rmax);
 }
 
-int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5) {
+int ptc_poisson_stage_times(ptc_ctx *ctx, int enable, float *ms5, int *nsolves) {
   PTC_CTX(ctx);
-  if (ms5)
-    for (int s = 0; s < 5; ++s) ms5[s] = ctx->pp.stage_ms[s];
+  PTC_TRY(poisson_stage_times(ctx, ms5, nsolves));
   if (enable >= 0) {
     ctx->pp.profile = enable != 0;
-    for (float &v : ctx->pp.stage_ms) v = 0.f;
+    ctx->pp.prof_used = 0;
   }
   return PTC_OK;
 }

@@ -33,6 +33,28 @@ int fill_device(ptc_ctx *c, double *a, long long n, double v) {
   return PTC_OK;
 }
 
+// plain streaming copy (also the reference kernel for HBM counter calibration)
+__global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ d, const double2 *__restrict__ s, long long n2) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += (long long)gridDim.x * blockDim.x)
+    d[q] = s[q];
+}
+__global__ void k_copy_tail(double *__restrict__ d, const double *__restrict__ s, long long n) { d[n - 1] = s[n - 1]; }
+
+int copy_device(ptc_ctx *c, double *dst, const double *src, long long n) {
+  const long long n2 = n / 2;
+  const int nblk = (int)std::min<long long>((n2 + 255) / 256, (long long)c->n_cu * 32);
+  if (n2 > 0) {
+    hipLaunchKernelGGL(k_copy, dim3(std::max(1, nblk)), dim3(256), 0, c->stream, reinterpret_cast<double2 *>(dst),
+                       reinterpret_cast<const double2 *>(src), n2);
+    PTC_HIP(hipGetLastError());
+  }
+  if (n % 2) {
+    hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(1), 0, c->stream, dst, src, n);
+    PTC_HIP(hipGetLastError());
+  }
+  return PTC_OK;
+}
+
 static bool is_plane(int id) { return id == PTC_F_UY_YM || id == PTC_F_DUY_YM; }
 
 size_t field_count(const ptc_ctx *c, int id) { return is_plane(id) ? (size_t)c->nx * c->nz : c->n3; }

@@ -99,8 +99,8 @@ struct PoissonPlan {
   double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
   size_t spec_elems = 0;
   bool profile = false;
-  float stage_ms[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev[6] = {};
+  std::vector<hipEvent_t> prof_ev;  // 6 events per profiled solve
+  int prof_used = 0;                // profiled solves since the last read
 };
 
 }  // namespace ptc
@@ -146,6 +146,7 @@ int poisson_build_plan(ptc_ctx *c);
 int poisson_free_plan(ptc_ctx *c);
 int poisson_solve(ptc_ctx *c, const double *src, double *dst);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
+int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 
 // FFT launchers (ptc_fft.hip)
 int launch_fft_x_fwd(ptc_ctx *c, const double *src, double2 *spec);

@@ -45,14 +45,16 @@ int poisson_free_plan(ptc_ctx *c) {
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
   if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
-  for (auto &e : P.ev)
+  for (auto &e : P.prof_ev)
     if (e) (void)hipEventDestroy(e);
+  const bool prof = P.profile;
   const int prec = P.prec, tri = P.tri, sweeps = P.sweeps, ztile = P.ztile;
   P = PoissonPlan();
   P.prec = prec;
   P.tri = tri;
   P.sweeps = sweeps;
   P.ztile = ztile;
+  P.profile = prof;
   return PTC_OK;
 }
 
@@ -112,7 +114,6 @@ int poisson_build_plan(ptc_ctx *c) {
     return PTC_E_ALLOC;
   }
   PTC_HIP(hipMemsetAsync(P.d_spec, 0, sizeof(double2) * P.spec_elems, c->stream));
-  for (auto &e : P.ev) PTC_HIP(hipEventCreate(&e));
   P.ready = true;
   return PTC_OK;
 }
@@ -132,26 +133,46 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
   PTC_TRY(poisson_build_plan(c));
   PoissonPlan &P = c->pp;
   PTC_TRY(ensure_thomas_scratch(c));
-  const bool prof = P.profile;
-  if (prof) PTC_HIP(hipEventRecord(P.ev[0], c->stream));
-  PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
-  if (prof) PTC_HIP(hipEventRecord(P.ev[1], c->stream));
-  PTC_TRY(launch_fft_z(c, P.d_spec, false));
-  if (prof) PTC_HIP(hipEventRecord(P.ev[2], c->stream));
-  PTC_TRY(launch_tri_poisson(c, P.d_spec));
-  if (prof) PTC_HIP(hipEventRecord(P.ev[3], c->stream));
-  PTC_TRY(launch_fft_z(c, P.d_spec, true));
-  if (prof) PTC_HIP(hipEventRecord(P.ev[4], c->stream));
-  PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
-  if (prof) {
-    PTC_HIP(hipEventRecord(P.ev[5], c->stream));
-    PTC_HIP(hipEventSynchronize(P.ev[5]));
-    for (int s = 0; s < 5; ++s) {
-      float ms = 0.f;
-      PTC_HIP(hipEventElapsedTime(&ms, P.ev[s], P.ev[s + 1]));
-      P.stage_ms[s] += ms;
+  hipEvent_t *ev = nullptr;
+  if (P.profile) {
+    const size_t need = (size_t)(P.prof_used + 1) * 6;
+    while (P.prof_ev.size() < need) {
+      hipEvent_t e;
+      PTC_HIP(hipEventCreate(&e));
+      P.prof_ev.push_back(e);
     }
+    ev = P.prof_ev.data() + (size_t)P.prof_used * 6;
+    ++P.prof_used;
   }
+  if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
+  PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
+  if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
+  PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  if (ev) PTC_HIP(hipEventRecord(ev[2], c->stream));
+  PTC_TRY(launch_tri_poisson(c, P.d_spec));
+  if (ev) PTC_HIP(hipEventRecord(ev[3], c->stream));
+  PTC_TRY(launch_fft_z(c, P.d_spec, true));
+  if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
+  PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
+  if (ev) PTC_HIP(hipEventRecord(ev[5], c->stream));
+  return PTC_OK;
+}
+
+int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves) {
+  PoissonPlan &P = c->pp;
+  float acc[5] = {0, 0, 0, 0, 0};
+  if (P.prof_used > 0) {
+    PTC_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < P.prof_used; ++q)
+      for (int s = 0; s < 5; ++s) {
+        float ms = 0.f;
+        PTC_HIP(hipEventElapsedTime(&ms, P.prof_ev[(size_t)q * 6 + s], P.prof_ev[(size_t)q * 6 + s + 1]));
+        acc[s] += ms;
+      }
+  }
+  if (ms5)
+    for (int s = 0; s < 5; ++s) ms5[s] = acc[s];
+  if (nsolves) *nsolves = P.prof_used;
   return PTC_OK;
 }
 

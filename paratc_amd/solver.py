@@ -104,15 +104,29 @@ class Solver:
     def poisson_config(self, prec=capi.PREC_FP64, tri=capi.TRI_AUTO, tile=0, sweeps=2):
         check(self.L.ptc_poisson_config(self.h, prec, tri, tile, sweeps), "ptc_poisson_config")
 
+    def poisson_get_config(self) -> dict:
+        o = (C.c_int * 4)()
+        check(self.L.ptc_poisson_get_config(self.h, o), "ptc_poisson_get_config")
+        tri = {capi.TRI_THOMAS: "thomas", capi.TRI_PARTITIONED: "partitioned"}.get(o[1], str(o[1]))
+        return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
+                "z_tile": o[2], "sweeps": o[3]}
+
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()
         check(self.L.ptc_poisson_residual(self.h, src, dst, C.byref(r)), "ptc_poisson_residual")
         return r.value
 
     def poisson_stage_times(self, enable: int = -1):
+        """(summed ms of the five passes, number of solves) since the last reset."""
         ms = (C.c_float * 5)()
-        check(self.L.ptc_poisson_stage_times(self.h, enable, ms), "ptc_poisson_stage_times")
-        return list(ms)
+        n = C.c_int()
+        check(self.L.ptc_poisson_stage_times(self.h, enable, ms, C.byref(n)), "ptc_poisson_stage_times")
+        return list(ms), n.value
+
+    def stream(self) -> int:
+        st = C.c_void_p()
+        check(self.L.ptc_get_stream(self.h, C.byref(st)), "ptc_get_stream")
+        return st.value or 0
 
     # ---- m_TScheme ----------------------------------------------------------
     def InitTimeScheme(self):

@@ -1,0 +1,102 @@
+"""CPU checks of the C-ABI boundary and the host-side helpers (no GPU compute)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from paratc_amd import capi, case as CS, ic, rng
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "paratc_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ptc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.load()
+    syms = header_symbols()
+    assert len(syms) >= 40
+    assert sorted(capi.EXPORTS) == syms
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (ptc_[a-z0-9_]+)", out))
+    assert set(syms) <= exported
+
+
+def test_abi_version_and_error_text():
+    L = capi.load()
+    assert L.ptc_abi_version() == 1
+    assert isinstance(L.ptc_last_error(), bytes)
+
+
+def _create(case_mod):
+    L = capi.load()
+    g, p = capi.params_from_case(CS.case_channel(32, 24, 16))
+    case_mod(g, p)
+    yp = np.ascontiguousarray(CS.case_channel(32, 24, 16).yp())
+    h = C.c_void_p()
+    rc = L.ptc_create(C.byref(g), C.byref(p), yp.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h))
+    return rc, L.ptc_last_error().decode(), h
+
+
+@pytest.mark.parametrize("mod,msg", [
+    (lambda g, p: setattr(g, "nx", 33), "mod(nxc,2)"),
+    (lambda g, p: setattr(g, "ny", 23), "mod(nyc,2)"),
+    (lambda g, p: setattr(p, "ischeme", 4), "Time scheme"),
+    (lambda g, p: setattr(g, "nx", 44), "factor into 2, 3, 5 and 7"),
+    (lambda g, p: setattr(p, "flow_type", 3), "FlowType"),
+])
+def test_create_rejects_bad_input_before_touching_a_device(mod, msg):
+    rc, err, h = _create(mod)
+    assert rc == capi.PTC_E_ARG or rc == capi.PTC_E_UNSUPPORTED
+    assert msg in err
+    assert not h.value
+
+
+def test_null_context_is_an_error():
+    L = capi.load()
+    assert L.ptc_clc_ppe(None, capi.F_ARR1, capi.F_HALO) == capi.PTC_E_ARG
+    assert b"null context" in L.ptc_last_error()
+
+
+def test_splitmix64_known_answers():
+    # SplitMix64 from seed 0 (Steele, Lea, Flood 2014; Vigna's reference outputs)
+    assert [int(v) for v in rng.splitmix64(0, 3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    # counter form: any window equals the sequence
+    assert np.array_equal(rng.splitmix64(5, 4, start=6), rng.splitmix64(5, 10)[6:])
+    z = rng.normal(1, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_poisson_rhs_is_compatible_and_seeded():
+    case = CS.case_poisson(32, 24, 16, 2.0)
+    f = ic.poisson_rhs(case, seed=1)
+    yp = case.yp()
+    dyp = np.diff(yp)
+    assert abs((f.sum(axis=(0, 2)) * dyp).sum()) < 1e-12 * np.abs(f).sum()
+    assert np.array_equal(f, ic.poisson_rhs(case, seed=1))
+    assert not np.array_equal(f, ic.poisson_rhs(case, seed=1, stream=1))
+
+
+def test_channel_fields_profile():
+    case = CS.case_channel(32, 24, 16, 1.8)
+    ux, uy, uz = ic.channel_fields(case, seed=0, amp=0.0)
+    yc = 0.5 * (case.yp()[1:] + case.yp()[:-1])
+    assert np.allclose(ux[5, 1:-1, 5], 1.5 * (1 - (yc - 1) ** 2))
+    assert not uy.any() and not uz.any()
+    ux, uy, uz = ic.channel_fields(case, seed=0, amp=0.1)
+    assert np.all(uy[:, 1, :] == 0.0) and np.abs(uz).max() <= 0.1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
