@@ -155,11 +155,14 @@ int ptc_clc_ppe(ptc_ctx *ctx, int src, int dst);
 /* Destory_Poisson_FFT_Plan (:339-347) */
 int ptc_destroy_poisson_plan(ptc_ctx *ctx);
 /* explicit variant selection (overrides the autotuner); prec = PTC_PREC_*,
- * tri = PTC_TRI_*, tile = z-FFT column tile (0 = auto), sweeps = refinement
+ * tri = PTC_TRI_*, tile = 0 selects the compile-time specialised FFT kernels
+ * where they exist for nx/2 and nz (generic kernels otherwise), tile = 1..16
+ * forces the generic kernels with that z column tile; sweeps = refinement
  * sweeps for PTC_PREC_FP32_REFINE */
 int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
-/* the plan in use: out4 = [prec, tri, z tile, sweeps] (after autotuning) */
-int ptc_poisson_get_config(ptc_ctx *ctx, int *out4);
+/* the plan in use: out5 = [prec, tri, z tile, sweeps, specialised-kernel
+ * mask (1 = x pass, 2 = z pass)] (after autotuning) */
+int ptc_poisson_get_config(ptc_ctx *ctx, int *out5);
 /* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
 /* Per-stage kernel times: with enable = 1 every later solve records HIP events

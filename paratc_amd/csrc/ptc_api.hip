@@ -87,15 +87,15 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   };
   std::vector<Cand> cands;
   for (int tri : {PTC_TRI_PARTITIONED, PTC_TRI_THOMAS})
-    for (int tile : {16, 8, 4}) cands.push_back({tri, tile});
+    for (int tile : {0, 16, 8, 4}) cands.push_back({tri, tile});
   double best = 1e300;
   Cand pick{PTC_TRI_PARTITIONED, 0};
   for (const Cand &cd : cands) {
     poisson_free_plan(c);
     c->pp.tri = cd.tri;
-    c->pp.ztile = cd.tile;
+    c->pp.ztile_req = cd.tile;
     if (poisson_build_plan(c) != PTC_OK) continue;
-    if (c->pp.ztile != cd.tile) continue;  // tile clipped by LDS: already covered
+    if (cd.tile > 0 && c->pp.ztile != cd.tile) continue;  // tile clipped by LDS: already covered
     double ms = 0;
     if (time_solve(c, s, d, 3, &ms) != PTC_OK) continue;
     if (ms < best) {
@@ -105,7 +105,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   }
   poisson_free_plan(c);
   c->pp.tri = pick.tri;
-  c->pp.ztile = pick.tile;
+  c->pp.ztile_req = pick.tile;
   PTC_TRY(poisson_build_plan(c));
   // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
   PTC_TRY(fill_device(c, s, (long long)c->n3, 0.0));
@@ -314,7 +314,7 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps) {
   poisson_free_plan(ctx);
   ctx->pp.prec = prec;
   ctx->pp.tri = tri == PTC_TRI_AUTO ? PTC_TRI_PARTITIONED : tri;
-  ctx->pp.ztile = tile;
+  ctx->pp.ztile_req = tile;
   ctx->pp.sweeps = sweeps;
   return poisson_build_plan(ctx);
 }
@@ -326,6 +326,7 @@ int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
   o[1] = ctx->pp.tri;
   o[2] = ctx->pp.ztile;
   o[3] = ctx->pp.sweeps;
+  o[4] = (ctx->pp.x_fast ? 1 : 0) | (ctx->pp.z_fast ? 2 : 0);
   return PTC_OK;
 }
 

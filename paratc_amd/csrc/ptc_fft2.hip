@@ -1,0 +1,452 @@
+// ptc_fft2.hip -- compile-time specialised, register-resident FFT passes.
+//
+// For the transform lengths of the production grids the radix plan, the
+// number of stages and the threads per line are compile-time constants, so
+// every butterfly lives in registers and no private (scratch) memory is used.
+// Stage 0 reads its inputs straight from HBM and the last stage writes its
+// outputs straight to HBM; LDS only carries the exchanges between stages.
+// The algorithm is the same self-sorting Stockham transform as the generic
+// kernels of ptc_fft.hip (which remain the path for other lengths):
+//
+//   stage with radix R after a span Ns:  butterfly b reads x[b + r N/R],
+//   multiplies by w^(r (b mod Ns) N/(Ns R)), applies the R-point DFT and
+//   writes x[(b div Ns) Ns R + (b mod Ns) + r Ns].
+//
+// Twiddles: w^e for the first power comes from the host FP64 table; the
+// higher powers of the same stage are formed by complex products (at most six
+// products, error ~ 7 ulp), which replaces R-1 table loads per butterfly.
+#include <algorithm>
+
+#include "ptc_internal.hpp"
+
+namespace ptc {
+
+namespace f2 {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <bool INV>
+__device__ __forceinline__ double2 mul_mi(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// ---- compile-time radix plan: 8s first, then 4 or 2, then 3, 5, 7 ----
+struct PlanC {
+  int ns;
+  int r[12];
+};
+
+constexpr PlanC make_plan(int n) {
+  PlanC p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n;
+  while (m % 8 == 0) { p.r[p.ns++] = 8; m /= 8; }
+  if (m % 4 == 0) { p.r[p.ns++] = 4; m /= 4; }
+  if (m % 2 == 0) { p.r[p.ns++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.r[p.ns++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.r[p.ns++] = 5; m /= 5; }
+  while (m % 7 == 0) { p.r[p.ns++] = 7; m /= 7; }
+  if (m != 1) p.ns = -1;
+  return p;
+}
+
+constexpr int plan_rmax(int n) {
+  PlanC p = make_plan(n);
+  int mx = 1;
+  for (int s = 0; s < p.ns; ++s) mx = p.r[s] > mx ? p.r[s] : mx;
+  return mx;
+}
+
+constexpr int plan_span(int n, int s) {  // product of the radices before stage s
+  PlanC p = make_plan(n);
+  int v = 1;
+  for (int q = 0; q < s; ++q) v *= p.r[q];
+  return v;
+}
+
+// LDS padding of one line: one slot per 8 elements
+__device__ __forceinline__ int pad8(int m) { return m + (m >> 3); }
+constexpr int padded(int n) { return n + n / 8 + 1; }
+
+// ---- small DFTs ----
+template <int R, bool INV>
+struct Dft;
+template <bool INV>
+struct Dft<2, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <bool INV>
+struct Dft<4, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const double2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[2] = csub(t0, t2);
+    v[3] = csub(t1, t3);
+  }
+};
+template <bool INV>
+struct Dft<8, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    constexpr double r2 = 0.70710678118654752440;
+    double2 e[4] = {v[0], v[2], v[4], v[6]};
+    double2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, INV>::run(e);
+    Dft<4, INV>::run(o);
+    const double2 o1 = INV ? make_double2(r2 * (o[1].x - o[1].y), r2 * (o[1].x + o[1].y))
+                           : make_double2(r2 * (o[1].x + o[1].y), r2 * (o[1].y - o[1].x));
+    const double2 o2 = mul_mi<INV>(o[2]);
+    const double2 o3 = INV ? make_double2(-r2 * (o[3].x + o[3].y), r2 * (o[3].x - o[3].y))
+                           : make_double2(r2 * (o[3].y - o[3].x), -r2 * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+template <int R>
+struct Trig;
+template <>
+struct Trig<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <>
+struct Trig<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                                  0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                                  -0.95105651629515357212};
+};
+template <>
+struct Trig<7> {
+  static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624,
+                                  -0.90096886790241912624, -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048,
+                                  -0.43388373911755812048, -0.97492791218182360702, -0.78183148246802980871};
+};
+
+template <int R, bool INV>
+struct DftOdd {
+  static __device__ __forceinline__ void run(double2 *v) {
+    constexpr int H = (R - 1) / 2;
+    double2 a[H + 1], b[H + 1];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+      a[q] = cadd(v[q], v[R - q]);
+      b[q] = csub(v[q], v[R - q]);
+    }
+    double2 x0 = v[0];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) x0 = cadd(x0, a[q]);
+    double2 out[R];
+    out[0] = x0;
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      double2 A = v[0], B = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 1; q <= H; ++q) {
+        const double cq = Trig<R>::c[(q * r) % R], sq = Trig<R>::s[(q * r) % R];
+        A.x += a[q].x * cq;
+        A.y += a[q].y * cq;
+        B.x += b[q].x * sq;
+        B.y += b[q].y * sq;
+      }
+      if (!INV) {
+        out[r] = make_double2(A.x + B.y, A.y - B.x);
+        out[R - r] = make_double2(A.x - B.y, A.y + B.x);
+      } else {
+        out[r] = make_double2(A.x - B.y, A.y + B.x);
+        out[R - r] = make_double2(A.x + B.y, A.y - B.x);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = out[r];
+  }
+};
+template <bool INV>
+struct Dft<3, INV> : DftOdd<3, INV> {};
+template <bool INV>
+struct Dft<5, INV> : DftOdd<5, INV> {};
+template <bool INV>
+struct Dft<7, INV> : DftOdd<7, INV> {};
+
+// ---- one stage for the G threads of a line; Load(m) / Store(m, v) address the
+//      line's elements.  SYNC: a barrier separates this stage's loads from its
+//      stores (needed when both go to the same LDS buffer). ----
+template <int N, int S, int G, bool INV, bool SYNC, class Load, class Store>
+__device__ __forceinline__ void stage(int g, const double2 *__restrict__ tw, const Load &load, const Store &store) {
+  constexpr PlanC P = make_plan(N);
+  constexpr int R = P.r[S];
+  constexpr int NS = plan_span(N, S);
+  constexpr int NB = N / R;
+  constexpr int BPT = (NB + G - 1) / G;
+  double2 v[BPT][R];
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = g + i * G;
+    if (BPT * G == NB || b < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = load(b + r * NB);
+    }
+  }
+  if (SYNC) __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = g + i * G;
+    if (BPT * G == NB || b < NB) {
+      const int k = b % NS;
+      if (NS > 1) {
+        double2 w1 = tw[k * (N / (NS * R))];
+        if (INV) w1.y = -w1.y;
+        double2 w = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[i][r] = cmul(v[i][r], w);
+          if (r + 1 < R) w = cmul(w, w1);
+        }
+      }
+      Dft<R, INV>::run(v[i]);
+      const int idxD = (b / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) store(idxD + r * NS, v[i][r]);
+    }
+  }
+}
+
+// Full transform of one line group.  Stage 0 loads through load0, the last
+// stage stores through storeL; intermediate data lives in LDS through lds(m).
+// LAST_TO_LDS: the last stage stores into LDS as well (the caller reads the
+// result from LDS after a barrier).
+template <int N, int G, bool INV, class Load0, class LdsAt, class StoreL>
+__device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
+                                        const StoreL &storeL) {
+  constexpr PlanC P = make_plan(N);
+  auto ldsload = [&](int m) { return lds(m); };
+  auto ldsstore = [&](int m, double2 v) { lds(m) = v; };
+  if constexpr (P.ns == 1) {
+    stage<N, 0, G, INV, false>(g, tw, load0, storeL);
+  } else {
+    stage<N, 0, G, INV, false>(g, tw, load0, ldsstore);
+    __syncthreads();
+    if constexpr (P.ns > 2) {
+      stage<N, 1, G, INV, true>(g, tw, ldsload, ldsstore);
+      __syncthreads();
+    }
+    if constexpr (P.ns > 3) {
+      stage<N, 2, G, INV, true>(g, tw, ldsload, ldsstore);
+      __syncthreads();
+    }
+    if constexpr (P.ns > 4) {
+      stage<N, 3, G, INV, true>(g, tw, ldsload, ldsstore);
+      __syncthreads();
+    }
+    if constexpr (P.ns > 5) {
+      stage<N, 4, G, INV, true>(g, tw, ldsload, ldsstore);
+      __syncthreads();
+    }
+    if constexpr (P.ns > 6) {
+      stage<N, 5, G, INV, true>(g, tw, ldsload, ldsstore);
+      __syncthreads();
+    }
+    static_assert(P.ns <= 7, "too many stages");
+    stage<N, P.ns - 1, G, INV, true>(g, tw, ldsload, storeL);
+  }
+}
+
+}  // namespace f2
+
+// threads per line: the largest-radix stage has one butterfly per thread
+template <int N>
+constexpr int f2_threads() {
+  return N / f2::plan_rmax(N);
+}
+
+// ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
+template <int N>
+__global__ void __launch_bounds__(256) k2_fft_x_fwd(const double *__restrict__ src, double2 *__restrict__ spec,
+                                                    const double2 *__restrict__ tw, const double2 *__restrict__ wx,
+                                                    int ny, int nz, int pitch, long long rowlen) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  constexpr int PN = f2::padded(N);
+  __shared__ double2 sh[L * PN];
+  const int l = threadIdx.x / G, g = threadIdx.x - l * G;
+  const long long nlines = (long long)ny * nz;
+  const long long q = (long long)blockIdx.x * L + l;
+  const bool live = l < L && q < nlines;
+  const long long qq = live ? q : 0;
+  const long long j = qq % ny + 1, k = qq / ny;
+  const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (j + (long long)(ny + 2) * k));
+  double2 *ls = sh + (l < L ? l : 0) * PN;
+  auto load0 = [&](int m) { return live ? row[m] : make_double2(0.0, 0.0); };
+  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
+  f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
+  __syncthreads();
+  if (!live) return;
+  double2 *out = spec + q * pitch;
+  for (int kx = g; kx < pitch; kx += G) {
+    double2 X = make_double2(0.0, 0.0);
+    if (kx <= N) {
+      const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
+      double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
+      zc.y = -zc.y;
+      const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
+      const double2 wd = f2::cmul(wx[kx], d);
+      X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
+    }
+    out[kx] = X;
+  }
+}
+
+// ---- x inverse: half spectra -> real rows times scale ----------------------
+template <int N>
+__global__ void __launch_bounds__(256) k2_fft_x_inv(const double2 *__restrict__ spec, double *__restrict__ dst,
+                                                    const double2 *__restrict__ tw, const double2 *__restrict__ wx,
+                                                    int ny, int nz, int pitch, long long rowlen, double scale) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  constexpr int PN = f2::padded(N);
+  __shared__ double2 sh[L * PN];
+  const int l = threadIdx.x / G, g = threadIdx.x - l * G;
+  const long long nlines = (long long)ny * nz;
+  const long long q = (long long)blockIdx.x * L + l;
+  const bool live = l < L && q < nlines;
+  const long long qq = live ? q : 0;
+  const double2 *in = spec + qq * pitch;
+  double2 *ls = sh + (l < L ? l : 0) * PN;
+  auto load0 = [&](int m) {
+    if (!live) return make_double2(0.0, 0.0);
+    const double2 xk = in[m];
+    double2 xc = in[N - m];
+    xc.y = -xc.y;
+    const double2 s = f2::cadd(xk, xc), d = f2::csub(xk, xc);
+    double2 w = wx[m];
+    w.y = -w.y;
+    const double2 wd = f2::cmul(w, d);
+    return make_double2(s.x - wd.y, s.y + wd.x);
+  };
+  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  const long long j = qq % ny + 1, k = qq / ny;
+  double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (j + (long long)(ny + 2) * k));
+  auto store_out = [&](int m, double2 v) {
+    if (live) row[m] = make_double2(v.x * scale, v.y * scale);
+  };
+  f2::run_fft<N, G, true>(g, tw, load0, lds, store_out);
+}
+
+// ---- z pass: T columns x N rows, thread id = g * T + col ----------------
+template <int N, int T, bool INV>
+__global__ void __launch_bounds__(512) k2_fft_z(double2 *__restrict__ spec, const double2 *__restrict__ tw, int ny,
+                                                 int pitch) {
+  constexpr int G = f2_threads<N>();
+  __shared__ double2 sh[N * T];
+  const int col = threadIdx.x % T, g = threadIdx.x / T;
+  const long long zs = (long long)pitch * ny;
+  double2 *base = spec + (long long)blockIdx.y * pitch + (long long)blockIdx.x * T + col;
+  auto load0 = [&](int m) { return base[m * zs]; };
+  auto lds = [&](int m) -> double2 & { return sh[m * T + col]; };
+  auto store_out = [&](int m, double2 v) { base[m * zs] = v; };
+  f2::run_fft<N, G, INV>(g, tw, load0, lds, store_out);
+}
+
+// ---- dispatch ----------------------------------------------------------------
+template <int N>
+static int x_fwd_t(ptc_ctx *c, const double *src, double2 *spec) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  const long long nlines = (long long)c->ny * c->nz;
+  const unsigned nblk = (unsigned)((nlines + L - 1) / L);
+  hipLaunchKernelGGL(k2_fft_x_fwd<N>, dim3(nblk), dim3(L * G), 0, c->stream, src, spec, c->pp.fx.tw, c->pp.d_wx,
+                     c->ny, c->nz, c->pp.pitch, (long long)c->nx);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+template <int N>
+static int x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  const long long nlines = (long long)c->ny * c->nz;
+  const unsigned nblk = (unsigned)((nlines + L - 1) / L);
+  hipLaunchKernelGGL(k2_fft_x_inv<N>, dim3(nblk), dim3(L * G), 0, c->stream, spec, dst, c->pp.fx.tw, c->pp.d_wx,
+                     c->ny, c->nz, c->pp.pitch, (long long)c->nx, scale);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+template <int N, int T>
+static int z_t(ptc_ctx *c, double2 *spec, bool inverse) {
+  constexpr int G = f2_threads<N>();
+  dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)c->ny);
+  if (inverse)
+    hipLaunchKernelGGL((k2_fft_z<N, T, true>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny, c->pp.pitch);
+  else
+    hipLaunchKernelGGL((k2_fft_z<N, T, false>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny,
+                       c->pp.pitch);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+// x lengths h = nx/2 with a specialised kernel
+#define PTC_X_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(768)
+
+bool fft2_has_x(int h) {
+#define PTC_CASE(N) if (h == N) return true;
+  PTC_X_SIZES(PTC_CASE)
+#undef PTC_CASE
+  return false;
+}
+
+int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec) {
+  switch (c->pp.h) {
+#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec);
+    PTC_X_SIZES(PTC_CASE)
+#undef PTC_CASE
+    default: return PTC_E_UNSUPPORTED;
+  }
+}
+
+int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
+  switch (c->pp.h) {
+#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale);
+    PTC_X_SIZES(PTC_CASE)
+#undef PTC_CASE
+    default: return PTC_E_UNSUPPORTED;
+  }
+}
+
+// z lengths with a specialised kernel and their column tile (<= 64 KB LDS,
+// <= 1024 threads)
+#define PTC_Z_SIZES(Z) Z(16, 16) Z(32, 16) Z(64, 16) Z(96, 16) Z(128, 16) Z(192, 16) Z(256, 16) Z(512, 8)
+
+bool fft2_has_z(int nz, int *tile) {
+#define PTC_CASE(N, T) if (nz == N) { *tile = T; return true; }
+  PTC_Z_SIZES(PTC_CASE)
+#undef PTC_CASE
+  return false;
+}
+
+int fft2_z(ptc_ctx *c, double2 *spec, bool inverse) {
+  switch (c->nz) {
+#define PTC_CASE(N, T) case N: return z_t<N, T>(c, spec, inverse);
+    PTC_Z_SIZES(PTC_CASE)
+#undef PTC_CASE
+    default: return PTC_E_UNSUPPORTED;
+  }
+}
+
+}  // namespace ptc

@@ -86,12 +86,15 @@ struct PoissonPlan {
   int sweeps = 2;
   int h = 0;          // nx / 2
   int pitch = 0;      // complex row pitch of the spectral buffer (>= h + 1)
-  int ztile = 4;      // columns per z-FFT workgroup
+  int ztile_req = 0;  // requested z tile: 0 = automatic (specialised kernels where built)
+  int ztile = 4;      // columns per z-FFT workgroup in use
+  bool x_fast = false, z_fast = false;  // compile-time specialised FFT kernels in use
   int z_threads = 256;
   int x_lines = 4;    // lines per x-FFT workgroup
   int x_threads = 256;
   int tri_chunks = 1; // chunks (waves) per line in the partitioned solver
-  int tri_mmax = 16;  // register rows per thread
+  int tri_mmax = 16;  // register rows per thread (chunk length)
+  int tri_lpb = 64;   // lines per workgroup
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
@@ -153,6 +156,11 @@ int launch_fft_x_fwd(ptc_ctx *c, const double *src, double2 *spec);
 int launch_fft_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale);
 int launch_fft_z(ptc_ctx *c, double2 *spec, bool inverse);
 int fft_make_len(int n, FftLen *L);
+bool fft2_has_x(int h);
+bool fft2_has_z(int nz, int *tile);
+int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec);
+int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale);
+int fft2_z(ptc_ctx *c, double2 *spec, bool inverse);
 
 // tridiagonal launchers (ptc_tridiag.hip)
 int launch_tri_poisson(ptc_ctx *c, double2 *spec);

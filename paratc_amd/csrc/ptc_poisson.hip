@@ -29,7 +29,7 @@
 namespace ptc {
 
 bool fft_config_ok(const FftLen &L, int ncol, int nthr);
-int tri_plan(int ny, size_t lds_limit, size_t vbytes, int *P_out, int *mmax_out);
+int tri_plan(int ny, size_t lds_limit, bool poisson, int *P_out, int *mm_out, int *lpb_out);
 
 static void make_twiddles(int n, std::vector<double2> &out) {
   const double twopi = 6.283185307179586476925286766559005768;
@@ -47,14 +47,13 @@ int poisson_free_plan(ptc_ctx *c) {
   if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
   for (auto &e : P.prof_ev)
     if (e) (void)hipEventDestroy(e);
-  const bool prof = P.profile;
-  const int prec = P.prec, tri = P.tri, sweeps = P.sweeps, ztile = P.ztile;
-  P = PoissonPlan();
-  P.prec = prec;
-  P.tri = tri;
-  P.sweeps = sweeps;
-  P.ztile = ztile;
-  P.profile = prof;
+  PoissonPlan keep;
+  keep.prec = P.prec;
+  keep.tri = P.tri;
+  keep.sweeps = P.sweeps;
+  keep.ztile_req = P.ztile_req;
+  keep.profile = P.profile;
+  P = keep;
   return PTC_OK;
 }
 
@@ -78,18 +77,26 @@ int poisson_build_plan(ptc_ctx *c) {
     if (fft_config_ok(P.fx, P.x_lines, t)) { P.x_threads = t; break; }
   PTC_CHECK(P.x_threads > 0, PTC_E_UNSUPPORTED, "no x-FFT launch shape fits");
 
-  // z pass: widest column tile (<= 16, divides pitch) that fits LDS
-  int T = P.ztile > 0 ? P.ztile : 16;
-  while (T > 1 && (sizeof(double2) * (size_t)c->nz * T > lds_cap || (size_t)T * c->nz > 4096)) T /= 2;
-  PTC_CHECK(P.pitch % T == 0, PTC_E_ARG, "z tile must divide the spectral pitch");
-  PTC_CHECK(sizeof(double2) * (size_t)c->nz * T <= lds_cap, PTC_E_UNSUPPORTED, "nz too large for the LDS z-FFT");
-  P.ztile = T;
-  P.z_threads = 0;
-  for (int t : {256, 512, 1024})
-    if (fft_config_ok(P.fz, T, t)) { P.z_threads = t; break; }
-  PTC_CHECK(P.z_threads > 0, PTC_E_UNSUPPORTED, "no z-FFT launch shape fits");
+  // specialised kernels for the production lengths unless a tile is forced
+  int zt = 0;
+  P.x_fast = P.ztile_req == 0 && fft2_has_x(P.h);
+  P.z_fast = P.ztile_req == 0 && fft2_has_z(c->nz, &zt) && P.pitch % zt == 0;
+  if (P.z_fast) {
+    P.ztile = zt;
+  } else {
+    // generic z pass: widest column tile (<= 16, divides pitch) that fits LDS
+    int T = P.ztile_req > 0 ? P.ztile_req : 16;
+    while (T > 1 && (sizeof(double2) * (size_t)c->nz * T > lds_cap || (size_t)T * c->nz > 4096)) T /= 2;
+    PTC_CHECK(P.pitch % T == 0, PTC_E_ARG, "z tile must divide the spectral pitch");
+    PTC_CHECK(sizeof(double2) * (size_t)c->nz * T <= lds_cap, PTC_E_UNSUPPORTED, "nz too large for the LDS z-FFT");
+    P.ztile = T;
+    P.z_threads = 0;
+    for (int t : {256, 512, 1024})
+      if (fft_config_ok(P.fz, T, t)) { P.z_threads = t; break; }
+    PTC_CHECK(P.z_threads > 0, PTC_E_UNSUPPORTED, "no z-FFT launch shape fits");
+  }
 
-  PTC_TRY(tri_plan(c->ny, c->lds_limit, 16, &P.tri_chunks, &P.tri_mmax));
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, true, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb));
 
   // twiddle tables: [fx: h][fz: nz][wx: h+1]
   std::vector<double2> tx, tz, all;

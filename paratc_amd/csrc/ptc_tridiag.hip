@@ -3,23 +3,24 @@
 // Two algorithms for the same systems:
 //
 //  * Partitioned (default).  A line of ny rows is split into P chunks, one
-//    wavefront per chunk, one lane per line (64 consecutive lines per
-//    workgroup, so every row load is a contiguous 64-lane access).  Each lane
+//    wavefront per chunk, one lane per real line (LPB consecutive lines per
+//    workgroup, so every row load is a contiguous LPB-lane access).  Each lane
 //    keeps its chunk in registers and runs the reference's "modified Thomas"
 //    (src/m_Poisson.f90:497-531): afterwards row j reads
 //        x_j + a_j x_first + c_j x_last = f_j ,
 //    with the chunk's first row coupled to the previous chunk's last row.  The
 //    2P first/last rows form a tridiagonal system with unit diagonal
-//    (InverseTridiagonal_reduce, src/m_Poisson.f90:1045-1071) solved by wave 0
-//    in LDS; every lane then reconstructs its interior rows
+//    (InverseTridiagonal_reduce, src/m_Poisson.f90:1045-1071) solved by
+//    cyclic reduction in LDS; every lane then reconstructs its interior rows
 //    (src/m_Poisson.f90:665-675).  One HBM read and one write per point.
 //  * Thomas.  One thread per line: InverseTridiagonal (src/m_Tools.f90:224-251)
 //    with the forward-sweep factor kept in a scratch array.  It moves ~2.5x the
 //    bytes and exists as the reference algorithm and autotuner candidate.
 //
-// Poisson lines (kx, kz) carry complex right-hand sides and the diagonal
-// ac2ph(j) + WaveNumX(kx) + WaveNumZ(kz) (src/m_Poisson.f90:383-430); the
-// (0,0) line has row j=1 replaced by phi=0 (src/m_Poisson.f90:399-404).
+// Poisson lines (kx, kz) carry complex right-hand sides with real rows whose
+// diagonal is ac2ph(j) + WaveNumX(kx) + WaveNumZ(kz) (src/m_Poisson.f90:383-430);
+// the partitioned kernel solves the real and imaginary parts as two real lines.
+// The (0,0) line has row j=1 replaced by phi=0 (src/m_Poisson.f90:399-404).
 // Crank-Nicolson lines (i, k) of clcU1Hat/U2Hat/U3Hat/clcScalar
 // (src/m_TScheme.f90:365-506) are real with j-only coefficients.
 #include "ptc_internal.hpp"
@@ -33,136 +34,230 @@ __device__ __forceinline__ double2 vsub(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double vzero(double) { return 0.0; }
 __device__ __forceinline__ double2 vzero(double2) { return make_double2(0.0, 0.0); }
 
-// local elimination of one chunk (rows r = 0..m-1, reference row j = s + r + 1)
-template <int MMAX, typename V, typename G>
-__device__ __forceinline__ void local_eliminate(V *f, double *a, double *c, int m, int s, const G &getc) {
-#pragma unroll
-  for (int r = 0; r < MMAX; ++r) {
-    if (r < m) {
-      double aj, bj, cj;
-      getc(s + r + 1, aj, bj, cj, f[r]);
-      if (r < 2) {
-        const double inv = 1.0 / bj;
-        a[r] = aj * inv;
-        c[r] = cj * inv;
-        f[r] = vscale(inv, f[r]);
-      } else {
-        const double rr = 1.0 / (bj - aj * c[r - 1]);
-        f[r] = vscale(rr, vsub(f[r], vscale(aj, f[r - 1])));
-        c[r] = rr * cj;
-        a[r] = -rr * aj * a[r - 1];
-      }
-    }
-  }
-#pragma unroll
-  for (int r = MMAX - 3; r >= 1; --r) {
-    if (r <= m - 3) {
-      f[r] = vsub(f[r], vscale(c[r], f[r + 1]));
-      a[r] = a[r] - c[r] * a[r + 1];
-      c[r] = -c[r] * c[r + 1];
-    }
-  }
-  const double rr = 1.0 / (1.0 - a[1] * c[0]);
-  f[0] = vscale(rr, vsub(f[0], vscale(c[0], f[1])));
-  a[0] = rr * a[0];
-  c[0] = -rr * c[0] * c[1];
-}
-
-// reduced system of 2P rows per line in LDS, solved by wave 0, then every lane
-// rebuilds its chunk.  RA/RC/RF are [2P][64].
-template <int MMAX, typename V>
-__device__ __forceinline__ void reduce_and_rebuild(V *f, const double *a, const double *c, int m, int P, int p,
-                                                   int lane, double *RA, double *RC, V *RF) {
-  V fl = f[0];
-  double al = a[0], cl = c[0];
-#pragma unroll
-  for (int r = 1; r < MMAX; ++r)
-    if (r == m - 1) {
-      fl = f[r];
-      al = a[r];
-      cl = c[r];
-    }
-  RA[(2 * p) * 64 + lane] = a[0];
-  RC[(2 * p) * 64 + lane] = c[0];
-  RF[(2 * p) * 64 + lane] = f[0];
-  RA[(2 * p + 1) * 64 + lane] = al;
-  RC[(2 * p + 1) * 64 + lane] = cl;
-  RF[(2 * p + 1) * 64 + lane] = fl;
-  __syncthreads();
-  if (p == 0) {
-    const int n2 = 2 * P;
-    // forward (vecm starts at 1: unit diagonal)
-    V fp = RF[lane];
-    double cp = RC[lane];
-    for (int q = 1; q < n2; ++q) {
-      const double aq = RA[q * 64 + lane];
-      const double den = 1.0 - aq * cp;
-      const double inv = 1.0 / den;
-      cp = RC[q * 64 + lane] * inv;
-      fp = vscale(inv, vsub(RF[q * 64 + lane], vscale(aq, fp)));
-      RC[q * 64 + lane] = cp;
-      RF[q * 64 + lane] = fp;
-    }
-    V x = fp;
-    for (int q = n2 - 2; q >= 0; --q) {
-      x = vsub(RF[q * 64 + lane], vscale(RC[q * 64 + lane], x));
-      RF[q * 64 + lane] = x;
-    }
-  }
-  __syncthreads();
-  const V xs = RF[(2 * p) * 64 + lane];
-  const V xe = RF[(2 * p + 1) * 64 + lane];
-#pragma unroll
-  for (int r = 0; r < MMAX; ++r) {
-    if (r < m) {
-      if (r == 0) f[r] = xs;
-      else if (r == m - 1) f[r] = xe;
-      else f[r] = vsub(vsub(f[r], vscale(a[r], xs)), vscale(c[r], xe));
-    }
-  }
-}
-
-// ---- Poisson y-solve, partitioned ----------------------------------------
-template <int MMAX>
-__global__ void __launch_bounds__(1024) k_tri_poisson_part(double2 *__restrict__ spec, const double *__restrict__ met,
-                                                          int nyh, const double *__restrict__ wnx,
-                                                          const double *__restrict__ wnz, int ny, int pitch, int h,
-                                                          int P) {
-  extern __shared__ double smem[];
-  double *RA = smem;
-  double *RC = RA + 2 * P * 64;
-  double2 *RF = reinterpret_cast<double2 *>(RC + 2 * P * 64);
-  const int lane = threadIdx.x & 63;
-  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int kx = blockIdx.x * 64 + lane, kz = blockIdx.y;
-  const bool valid = kx <= h;
-  const int s = (p * ny) / P, m = ((p + 1) * ny) / P - s;
-  const double lam = wnx[valid ? kx : 0] + wnz[kz];
-  const bool pin = (kx == 0 && kz == 0);
-  double2 *col = spec + (long long)kz * ny * pitch + kx;
-  double2 f[MMAX];
-  double a[MMAX], c[MMAX];
-#pragma unroll
-  for (int r = 0; r < MMAX; ++r)
-    if (r < m) f[r] = valid ? col[(long long)(s + r) * pitch] : make_double2(0.0, 0.0);
-  const double *am = met + M_AM2PH * nyh, *ac = met + M_AC2PH * nyh, *ap = met + M_AP2PH * nyh;
-  auto getc = [&](int j, double &aj, double &bj, double &cj, double2 &fj) {
-    aj = am[j];
-    cj = ap[j];
-    bj = ac[j] + lam;
-    if (pin && j == 1) {
+// Row coefficients of one line.  A row replaced by an identity row (the
+// pinned Poisson row, the v wall row) gets its right-hand side from the kernel.
+struct PoissonRowsRe {  // ac2ph(j) + WaveNumX + WaveNumZ, pinned (0,0) line
+  const double *am, *ac, *ap;
+  double lam;
+  int ny;
+  bool pin;
+  __device__ __forceinline__ void operator()(int j, double &aj, double &bj, double &cj) const {
+    if (j > ny) {
       aj = 0.0;
       bj = 1.0;
       cj = 0.0;
-      fj = make_double2(0.0, 0.0);
+    } else {
+      aj = am[j];
+      cj = j < ny ? ap[j] : 0.0;
+      bj = ac[j] + lam;
+      if (pin && j == 1) {
+        aj = 0.0;
+        bj = 1.0;
+        cj = 0.0;
+      }
     }
-  };
-  local_eliminate<MMAX>(f, a, c, m, s, getc);
-  reduce_and_rebuild<MMAX>(f, a, c, m, P, p, lane, RA, RC, RF);
+  }
+};
+
+struct CnRows {  // (I - beta L_y) with the wall rows of the chosen table
+  const double *am, *ac, *ap;
+  double beta;
+  int ny, kind;
+  __device__ __forceinline__ void operator()(int j, double &aj, double &bj, double &cj) const {
+    if (j > ny) {
+      aj = 0.0;
+      bj = 1.0;
+      cj = 0.0;
+    } else if (kind == 1 && j == 1) {
+      aj = 0.0;
+      bj = 1.0;
+      cj = 0.0;
+    } else {
+      aj = -beta * am[j];
+      bj = -beta * ac[j] + 1.0;
+      cj = j < ny ? -beta * ap[j] : 0.0;
+    }
+  }
+};
+
+// Every chunk holds exactly MM rows (reference rows j = s+1 .. s+MM).  Rows
+// beyond ny are identity rows (a = c = 0, b = 1, f = 0) and the last real row
+// drops its super-diagonal, which InverseTridiagonal never reads either
+// (src/m_Tools.f90:239-245), so the padded system has the same solution on the
+// real rows and all control flow here is compile-time.
+//
+// Reduced rows of one chunk:
+//   first row:  aS x_{s-1} + x_s    + cS x_last   = fS
+//   last row:   aE x_s     + x_last + cE x_{e+1}  = fE
+// as left by the reference's modified Thomas (src/m_Poisson.f90:497-531).  The
+// last row comes from a downward elimination and the first row from an upward
+// one, each with O(1) state, so no per-row coefficient arrays are kept.
+template <int MM, typename V, typename G>
+__device__ __forceinline__ void chunk_reduced_rows(const V (&f)[MM], int s, G &getc, double &aS, double &cS,
+                                                   V &fS, double &aE, double &cE, V &fE) {
+  static_assert(MM >= 4, "chunks need at least four rows");
+  double aj, bj, cj;
+  getc(s + 2, aj, bj, cj);
+  double inv = 1.0 / bj;
+  double A = aj * inv, Cc = cj * inv;
+  V Fv = vscale(inv, f[1]);
+#pragma unroll
+  for (int r = 2; r < MM; ++r) {
+    getc(s + r + 1, aj, bj, cj);
+    const double rr = 1.0 / (bj - aj * Cc);
+    Fv = vscale(rr, vsub(f[r], vscale(aj, Fv)));
+    A = -rr * aj * A;
+    Cc = rr * cj;
+  }
+  aE = A;
+  cE = Cc;
+  fE = Fv;
+  getc(s + MM - 1, aj, bj, cj);
+  const double iv = 1.0 / bj;
+  double Ap = aj * iv, Cp = cj * iv;
+  V Fp = vscale(iv, f[MM - 2]);
+#pragma unroll
+  for (int r = MM - 3; r >= 1; --r) {
+    getc(s + r + 1, aj, bj, cj);
+    const double rr = 1.0 / (bj - cj * Ap);
+    Fp = vscale(rr, vsub(f[r], vscale(cj, Fp)));
+    Cp = -rr * cj * Cp;
+    Ap = rr * aj;
+  }
+  getc(s + 1, aj, bj, cj);
+  const double r0 = 1.0 / (bj - cj * Ap);
+  aS = r0 * aj;
+  cS = -r0 * cj * Cp;
+  fS = vscale(r0, vsub(f[0], vscale(cj, Fp)));
+}
+
+// With the chunk's end values known, solve its interior rows 1..MM-2 by
+// Thomas (InverseTridiagonal, src/m_Tools.f90:224-251); f receives the chunk.
+template <int MM, typename V, typename G>
+__device__ __forceinline__ void chunk_interior(V (&f)[MM], int s, G &getc, V xs, V xe) {
+  double cp[MM];
+  double aj, bj, cj;
+  getc(s + 2, aj, bj, cj);
+  double iv = 1.0 / bj;
+  cp[1] = cj * iv;
+  f[1] = vscale(iv, vsub(f[1], vscale(aj, xs)));
+#pragma unroll
+  for (int r = 2; r <= MM - 2; ++r) {
+    getc(s + r + 1, aj, bj, cj);
+    V fj = f[r];
+    if (r == MM - 2) fj = vsub(fj, vscale(cj, xe));
+    iv = 1.0 / (bj - aj * cp[r - 1]);
+    cp[r] = cj * iv;
+    f[r] = vscale(iv, vsub(fj, vscale(aj, f[r - 1])));
+  }
+#pragma unroll
+  for (int r = MM - 3; r >= 1; --r) f[r] = vsub(f[r], vscale(cp[r], f[r + 1]));
+  f[0] = xs;
+  f[MM - 1] = xe;
+}
+
+// Reduced system of 2P rows per line (row 2p: first row of chunk p, row 2p+1:
+// its last row), unit diagonal, solved by parallel cyclic reduction: at step d
+// every row eliminates its neighbours q-d and q+d, so after ceil(log2(2P))
+// steps each row stands alone.  Rows are renormalised to a unit diagonal after
+// every step; rows outside the system act as decoupled identity rows.  Each
+// thread owns the two rows of its chunk and keeps them in registers; LDS
+// ([2P][LPB] for a, c, f) only publishes them between steps.  The reference
+// solves the same reduced system by Thomas (InverseTridiagonal_reduce,
+// src/m_Poisson.f90:1045-1071); cyclic reduction lets every chunk's thread
+// take part instead of one wavefront.
+template <int LPB, typename V>
+__device__ __forceinline__ void pcr_row(int q, int d, int n2, int lane, const double *RA, const double *RC,
+                                        const V *RF, double &a, double &c, V &f) {
+  double am = 0.0, cm = 0.0, ap = 0.0, cp = 0.0;
+  V fm = vzero(f), fp = vzero(f);
+  if (q - d >= 0) {
+    am = RA[(q - d) * LPB + lane];
+    cm = RC[(q - d) * LPB + lane];
+    fm = RF[(q - d) * LPB + lane];
+  }
+  if (q + d < n2) {
+    ap = RA[(q + d) * LPB + lane];
+    cp = RC[(q + d) * LPB + lane];
+    fp = RF[(q + d) * LPB + lane];
+  }
+  // a x_{q-d} + x_q + c x_{q+d} = f, with rows q-d and q+d substituted
+  const double inv = 1.0 / (1.0 - a * cm - c * ap);
+  f = vscale(inv, vsub(vsub(f, vscale(a, fm)), vscale(c, fp)));
+  a = -inv * a * am;
+  c = -inv * c * cp;
+}
+
+template <int LPB, typename V>
+__device__ __forceinline__ void solve_reduced(double aS, double cS, V fS, double aE, double cE, V fE, int P, int p,
+                                              int lane, double *RA, double *RC, V *RF, V &xs, V &xe) {
+  const int n2 = 2 * P, q0 = 2 * p, q1 = 2 * p + 1;
+  for (int d = 1; d < n2; d *= 2) {
+    RA[q0 * LPB + lane] = aS;
+    RC[q0 * LPB + lane] = cS;
+    RF[q0 * LPB + lane] = fS;
+    RA[q1 * LPB + lane] = aE;
+    RC[q1 * LPB + lane] = cE;
+    RF[q1 * LPB + lane] = fE;
+    __syncthreads();
+    pcr_row<LPB>(q0, d, n2, lane, RA, RC, RF, aS, cS, fS);
+    pcr_row<LPB>(q1, d, n2, lane, RA, RC, RF, aE, cE, fE);
+    __syncthreads();
+  }
+  xs = fS;
+  xe = fE;
+}
+
+// chunk index of a thread; with 64 lines per block a chunk is one wavefront
+template <int LPB>
+__device__ __forceinline__ int chunk_of_thread() {
+  if constexpr (LPB == 64) return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  return threadIdx.x / LPB;
+}
+
+// ---- Poisson y-solve, partitioned ----------------------------------------
+// chunks of >= 16 rows need more than 128 VGPRs: blocks of at most 512 threads
+template <int MM>
+constexpr int tri_block_cap() {
+  return MM >= 16 ? 512 : 1024;
+}
+
+// The rows are real, so the real and imaginary parts of a line are two
+// independent real systems.  Lane 2m handles the real part of line kx0+m,
+// lane 2m+1 its imaginary part: a row is still one contiguous 8-byte-per-lane
+// access, the right-hand side takes half the registers, and with LPB = 64 a
+// chunk is one wavefront so the row coefficients are wave-uniform.
+template <int MM, int LPB>
+__global__ void __launch_bounds__(tri_block_cap<MM>()) k_tri_poisson_split(double2 *__restrict__ spec, const double *__restrict__ met,
+                                                           int nyh, const double *__restrict__ wnx,
+                                                           const double *__restrict__ wnz, int ny, int pitch, int h,
+                                                           int P) {
+  extern __shared__ double smem[];
+  double *RA = smem;
+  double *RC = RA + 2 * P * LPB;
+  double *RF = RC + 2 * P * LPB;
+  const int lane = threadIdx.x % LPB;
+  const int p = chunk_of_thread<LPB>();
+  const int kx = blockIdx.x * (LPB / 2) + lane / 2, kz = blockIdx.y;
+  const bool valid = kx <= h;
+  const int s = p * MM;
+  const double lam = valid ? wnx[kx] + wnz[kz] : 1.0;
+  const bool pin = (kx == 0 && kz == 0);
+  double *col = reinterpret_cast<double *>(spec + (long long)kz * ny * pitch) + 2 * kx + (lane & 1);
+  const long long rs = 2LL * pitch;
+  double f[MM];
+#pragma unroll
+  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? col[(s + r) * rs] : 0.0;
+  if (pin && s == 0) f[0] = 0.0;  // phi(1) = 0 on the (0,0) line
+  PoissonRowsRe getc{met + M_AM2PH * nyh, met + M_AC2PH * nyh, met + M_AP2PH * nyh, lam, ny, pin};
+  double aS, cS, aE, cE, fS, fE, xs, xe;
+  chunk_reduced_rows<MM>(f, s, getc, aS, cS, fS, aE, cE, fE);
+  solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
+  chunk_interior<MM>(f, s, getc, xs, xe);
   if (valid) {
 #pragma unroll
-    for (int r = 0; r < MMAX; ++r)
-      if (r < m) col[(long long)(s + r) * pitch] = f[r];
+    for (int r = 0; r < MM; ++r)
+      if (s + r < ny) col[(s + r) * rs] = f[r];
   }
 }
 
@@ -211,47 +306,36 @@ __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict_
 
 // ---- Crank-Nicolson y-solves (u += du), partitioned -----------------------
 // kind: 0 = u/w (ForCN tables), 1 = v (row 1 Dirichlet), 2 = scalar (ScCN tables)
-template <int MMAX>
-__global__ void __launch_bounds__(1024) k_tri_cn_part(double *__restrict__ u, const double *__restrict__ rhs,
+template <int MM, int LPB>
+__global__ void __launch_bounds__(tri_block_cap<MM>()) k_tri_cn_part(double *__restrict__ u, const double *__restrict__ rhs,
                                                      const double *__restrict__ duy_ym,
                                                      const double *__restrict__ shift, const double *__restrict__ met,
                                                      int nyh, int rowA, int rowB, int rowC, double beta, int kind,
                                                      int nx, int ny, int P) {
   extern __shared__ double smem[];
   double *RA = smem;
-  double *RC = RA + 2 * P * 64;
-  double *RF = RC + 2 * P * 64;
-  const int lane = threadIdx.x & 63;
-  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = blockIdx.x * 64 + lane, k = blockIdx.y;
+  double *RC = RA + 2 * P * LPB;
+  double *RF = RC + 2 * P * LPB;
+  const int lane = threadIdx.x % LPB;
+  const int p = chunk_of_thread<LPB>();
+  const int i = blockIdx.x * LPB + lane, k = blockIdx.y;
   const bool valid = i < nx;
-  const int s = (p * ny) / P, m = ((p + 1) * ny) / P - s;
+  const int s = p * MM;
   const long long col = (long long)(valid ? i : 0) + (long long)nx * ((long long)(ny + 2) * k);
   const double sh = shift ? shift[0] : 0.0;
-  double f[MMAX], a[MMAX], c[MMAX];
+  double f[MM];
 #pragma unroll
-  for (int r = 0; r < MMAX; ++r)
-    if (r < m) f[r] = valid ? rhs[col + (long long)nx * (s + r + 1)] + sh : 0.0;
-  const double *am = met + rowA * nyh, *ac = met + rowB * nyh, *ap = met + rowC * nyh;
-  const double dym = (kind == 1 && valid) ? duy_ym[i + (long long)nx * k] : 0.0;
-  auto getc = [&](int j, double &aj, double &bj, double &cj, double &fj) {
-    if (kind == 1 && j == 1) {
-      aj = 0.0;
-      bj = 1.0;
-      cj = 0.0;
-      fj = dym;
-    } else {
-      aj = -beta * am[j];
-      bj = -beta * ac[j] + 1.0;
-      cj = -beta * ap[j];
-    }
-  };
-  local_eliminate<MMAX>(f, a, c, m, s, getc);
-  reduce_and_rebuild<MMAX>(f, a, c, m, P, p, lane, RA, RC, RF);
+  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? rhs[col + (long long)nx * (s + r + 1)] + sh : 0.0;
+  if (kind == 1 && s == 0) f[0] = valid ? duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
+  CnRows getc{met + rowA * nyh, met + rowB * nyh, met + rowC * nyh, beta, ny, kind};
+  double aS, cS, aE, cE, fS, fE, xs, xe;
+  chunk_reduced_rows<MM>(f, s, getc, aS, cS, fS, aE, cE, fE);
+  solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
+  chunk_interior<MM>(f, s, getc, xs, xe);
   if (valid) {
 #pragma unroll
-    for (int r = 0; r < MMAX; ++r)
-      if (r < m) {
+    for (int r = 0; r < MM; ++r)
+      if (s + r < ny) {
         const long long o = col + (long long)nx * (s + r + 1);
         u[o] = u[o] + f[r];
       }
@@ -260,27 +344,39 @@ __global__ void __launch_bounds__(1024) k_tri_cn_part(double *__restrict__ u, co
 
 // ---- host launchers -------------------------------------------------------
 
-// chunks per line for ny rows with at most mmax rows per chunk, each >= 4 rows
-static int choose_chunks(int ny, int mmax) {
-  int P = (ny + mmax - 1) / mmax;
-  if (P < 1) P = 1;
-  while (P > 1 && ny / P < 4) --P;
-  return P;
-}
+// Launch shape of the partitioned solver: chunk length MM (compile-time), lines
+// per block LPB and chunk count P = ceil(ny / MM).  The candidates are the
+// instantiations that run without private-memory spills; the first one with
+// at most 1/8 padded rows whose block (LPB * P threads) and LDS fit wins.
+struct TriShape {
+  int mm, lpb;
+};
+static const TriShape kPoissonShapes[] = {{4, 64}, {8, 64}, {12, 64}, {16, 64}};
+static const TriShape kCnShapes[] = {{4, 64}, {8, 64}, {8, 32}, {16, 32}};
 
-int tri_plan(int ny, size_t lds_limit, size_t vbytes, int *P_out, int *mmax_out) {
-  for (int mmax : {8, 16, 24, 32}) {
-    const int P = choose_chunks(ny, mmax);
-    if ((ny + P - 1) / P > mmax) continue;
-    if (P > 16) continue;
-    const size_t lds = (size_t)2 * P * 64 * (16 + vbytes);
-    if (lds > lds_limit) continue;
-    *P_out = P;
-    *mmax_out = mmax;
-    return PTC_OK;
-  }
+static int tri_choose(int ny, size_t lds_limit, const TriShape *shapes, int nshapes, int *P_out,
+                      int *mm_out, int *lpb_out) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int q = 0; q < nshapes; ++q) {
+      const int mm = shapes[q].mm, lpb = shapes[q].lpb;
+      const int P = (ny + mm - 1) / mm;
+      if (lpb * P > (mm >= 16 ? 512 : 1024)) continue;
+      if (2 * P * lpb > 1 << 16) continue;
+      if (pass == 0 && (P * mm - ny) * 8 > ny) continue;  // prefer little padding
+      const size_t lds = (size_t)2 * P * lpb * 24;  // a, c, f of the reduced rows
+      if (lds > lds_limit) continue;
+      *P_out = P;
+      *mm_out = mm;
+      *lpb_out = lpb;
+      return PTC_OK;
+    }
   set_error("ny = " + std::to_string(ny) + " is outside the partitioned tridiagonal kernels");
   return PTC_E_UNSUPPORTED;
+}
+
+int tri_plan(int ny, size_t lds_limit, bool poisson, int *P_out, int *mm_out, int *lpb_out) {
+  return poisson ? tri_choose(ny, lds_limit, kPoissonShapes, 4, P_out, mm_out, lpb_out)
+                 : tri_choose(ny, lds_limit, kCnShapes, 4, P_out, mm_out, lpb_out);
 }
 
 int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
@@ -293,54 +389,51 @@ int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
     PTC_HIP(hipGetLastError());
     return PTC_OK;
   }
-  const int nP = P.tri_chunks;
-  dim3 grid((unsigned)((P.h + 1 + 63) / 64), (unsigned)c->nz);
-  const size_t lds = (size_t)2 * nP * 64 * 32;
-  switch (P.tri_mmax) {
-    case 8:
-      hipLaunchKernelGGL(k_tri_poisson_part<8>, grid, dim3(64 * nP), lds, c->stream, spec, c->d_met, c->ny + 2,
-                         c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);
-      break;
-    case 16:
-      hipLaunchKernelGGL(k_tri_poisson_part<16>, grid, dim3(64 * nP), lds, c->stream, spec, c->d_met, c->ny + 2,
-                         c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);
-      break;
-    case 24:
-      hipLaunchKernelGGL(k_tri_poisson_part<24>, grid, dim3(64 * nP), lds, c->stream, spec, c->d_met, c->ny + 2,
-                         c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);
-      break;
-    default:
-      hipLaunchKernelGGL(k_tri_poisson_part<32>, grid, dim3(64 * nP), lds, c->stream, spec, c->d_met, c->ny + 2,
-                         c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);
-      break;
+  const int nP = P.tri_chunks, lpb = P.tri_lpb;
+  // lpb lanes per block = lpb/2 complex lines
+  dim3 grid((unsigned)((2 * (P.h + 1) + lpb - 1) / lpb), (unsigned)c->nz);
+  const size_t lds = (size_t)2 * nP * lpb * 24;
+#define PTC_CASE(MM, LPB)                                                                                    \
+  if (P.tri_mmax == MM && lpb == LPB) {                                                                      \
+    hipLaunchKernelGGL((k_tri_poisson_split<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, spec, c->d_met,  \
+                       c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);                              \
+    PTC_HIP(hipGetLastError());                                                                              \
+    return PTC_OK;                                                                                           \
   }
-  PTC_HIP(hipGetLastError());
-  return PTC_OK;
+  PTC_CASE(4, 64)
+  PTC_CASE(8, 64)
+  PTC_CASE(12, 64)
+  PTC_CASE(16, 64)
+#undef PTC_CASE
+  set_error("bad tridiagonal launch shape");
+  return PTC_E_STATE;
 }
 
 // kind 0 (u/w), 1 (v), 2 (scalar); shift: optional device scalar added to rhs
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
                   const double *shift) {
-  int nP = 1, mmax = 16;
-  PTC_TRY(tri_plan(c->ny, c->lds_limit, 8, &nP, &mmax));
+  int nP = 1, mmax = 16, lpb = 64;
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, &nP, &mmax, &lpb));
   int rA, rB, rC;
   if (kind == 0) { rA = M_AM2CCN; rB = M_AC2CCN; rC = M_AP2CCN; }
   else if (kind == 1) { rA = M_AM2P; rB = M_AC2P; rC = M_AP2P; }
   else { rA = M_AM2CSCCN; rB = M_AC2CSCCN; rC = M_AP2CSCCN; }
-  dim3 grid((unsigned)((c->nx + 63) / 64), (unsigned)c->nz);
-  const size_t lds = (size_t)2 * nP * 64 * 24;
-#define PTC_CN(MM)                                                                                          \
-  hipLaunchKernelGGL(k_tri_cn_part<MM>, grid, dim3(64 * nP), lds, c->stream, u, rhs, duy_ym, shift, c->d_met, \
-                     c->ny + 2, rA, rB, rC, beta, kind, c->nx, c->ny, nP)
-  switch (mmax) {
-    case 8: PTC_CN(8); break;
-    case 16: PTC_CN(16); break;
-    case 24: PTC_CN(24); break;
-    default: PTC_CN(32); break;
+  dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz);
+  const size_t lds = (size_t)2 * nP * lpb * 24;
+#define PTC_CN(MM, LPB)                                                                                       \
+  if (mmax == MM && lpb == LPB) {                                                                             \
+    hipLaunchKernelGGL((k_tri_cn_part<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, u, rhs, duy_ym, shift,  \
+                       c->d_met, c->ny + 2, rA, rB, rC, beta, kind, c->nx, c->ny, nP);                        \
+    PTC_HIP(hipGetLastError());                                                                               \
+    return PTC_OK;                                                                                            \
   }
+  PTC_CN(4, 64)
+  PTC_CN(8, 64)
+  PTC_CN(8, 32)
+  PTC_CN(16, 32)
 #undef PTC_CN
-  PTC_HIP(hipGetLastError());
-  return PTC_OK;
+  set_error("bad tridiagonal launch shape");
+  return PTC_E_STATE;
 }
 
 }  // namespace ptc

@@ -105,11 +105,11 @@ class Solver:
         check(self.L.ptc_poisson_config(self.h, prec, tri, tile, sweeps), "ptc_poisson_config")
 
     def poisson_get_config(self) -> dict:
-        o = (C.c_int * 4)()
+        o = (C.c_int * 5)()
         check(self.L.ptc_poisson_get_config(self.h, o), "ptc_poisson_get_config")
         tri = {capi.TRI_THOMAS: "thomas", capi.TRI_PARTITIONED: "partitioned"}.get(o[1], str(o[1]))
         return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
-                "z_tile": o[2], "sweeps": o[3]}
+                "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2)}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()

@@ -100,3 +100,20 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_restart_roundtrip_and_size_check(tmp_path):
+    from paratc_amd import restart
+
+    r = np.random.default_rng(0)
+    f = [r.standard_normal((6, 4, 8)) for _ in range(5)]
+    path = str(tmp_path / restart.restart_name("", "Cha", 12).lstrip("/"))
+    assert path.endswith("RestartForCha0000000012")
+    restart.write_restart(path, *f[:4], scalar=f[4], ucrf=0.25)
+    back = restart.read_restart(path, 6, 4, 8, scalar=True, ucrf=0.25)
+    assert np.allclose(back["ux"], f[0]) and np.array_equal(back["scalar"], f[4])
+    assert os.path.getsize(path) == 8 * 6 * 4 * 8 * 5
+    with open(path, "ab") as fh:  # truncated / padded file -> "file byte wrong"
+        fh.write(b"\0" * 8)
+    with pytest.raises(ValueError, match="file byte wrong2"):
+        restart.read_restart(path, 6, 4, 8, scalar=True)

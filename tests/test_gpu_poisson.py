@@ -112,3 +112,31 @@ def test_poisson_deterministic():
         s.clcPPE()
         b = s.download(capi.F_HALO, capi.L_INTERIOR)
     assert np.array_equal(a, b)
+
+
+# lengths with compile-time specialised FFT kernels: x/2 in {8,16,32,48,64,96,128,256,512,768},
+# z in {16,32,64,96,128,192,256,512}
+FAST_GRIDS = [(64, 16, 32), (128, 8, 64), (256, 12, 96), (512, 8, 128), (1024, 8, 192),
+              (1536, 8, 256), (96, 8, 512), (16, 20, 16), (192, 24, 128)]
+
+
+@pytest.mark.parametrize("grid", FAST_GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_specialised_fft_kernels_match_generic_and_oracle(grid):
+    nx, ny, nz = grid
+    case = CS.case_poisson(nx, ny, nz, 2.0)
+    f = ic.poisson_rhs(case, seed=8)
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
+        cfg = s.poisson_get_config()
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        fast = s.download(capi.F_HALO, capi.L_INTERIOR)
+        s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 4, 0)
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        generic = s.download(capi.F_HALO, capi.L_INTERIOR)
+    assert cfg["x_fast"] and cfg["z_fast"], cfg
+    assert rel_err(fast, generic) <= 1e-13
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    assert rel_err(fast, inner(ref)) <= TOL
