@@ -12,6 +12,8 @@ The JSON line also carries
   * roofline: the dominant pass of the solve (algorithmic bytes 16 B per grid
     point per pass: one FP64 read and one write, complex half spectra) over its
     mean launch time, measured with HIP events on the library's stream;
+    traffic is that kernel's HBM bytes per launch from the FETCH_SIZE /
+    WRITE_SIZE summary in profiles/ (null if the summary lacks the kernel);
   * cpu_baseline: the CPU oracle (a port of the reference's clcPPE_x1) on the
     same right-hand side, timed on this host's cores over a bounded sample;
   * rk_substep: grid points per second of one RK3 substep of the 128 x 96 x 128
@@ -34,7 +36,15 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpts/s per RK substep + Poisson HBM GB/s; FP32+refine vs FP64; ensemble p50/p99"
 HBM_PEAK_GBS = 8000.0
 STAGES = ["x_fwd", "z_fwd", "y_solve", "z_inv", "x_inv"]
-STAGE_KERNELS = ["k_fft_x_fwd", "k_fft_z<fwd>", "k_tri_poisson_part", "k_fft_z<inv>", "k_fft_x_inv"]
+
+
+def stage_kernels(plan: dict) -> list:
+    """Kernel behind each pass for the plan in use (as rocprofv3 names them)."""
+    xf = "k2_fft_x_fwd" if plan["x_fast"] else "k_fft_x_fwd"
+    xi = "k2_fft_x_inv" if plan["x_fast"] else "k_fft_x_inv"
+    z = "k2_fft_z" if plan["z_fast"] else "k_fft_z"
+    y = "k_tri_poisson_split" if plan["tri"] == "partitioned" else "k_tri_poisson_thomas"
+    return [xf, z + "<fwd>", y, z + "<inv>", xi]
 
 
 class HipEvents:
@@ -210,6 +220,11 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
     n = case.npts
+    kernel = stage_kernels(r["plan"])[r["dom"]]
+    # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
+    # WRITE_SIZE summary of this workload (null when it does not cover it)
+    from paratc_amd import profsum
+    traffic = profsum.read_traffic(os.path.join(ROOT, "profiles", "poisson_1024x256x512_hbm_traffic.md"), kernel)
     line = {
         "metric": METRIC,
         "value": round(n / (r["ms_per"] * 1e-3) / 1e6, 1),
@@ -227,8 +242,8 @@ def main():
                    "rhs": "N(0,1) seed 1 minus dyp-weighted mean, 2 resident RHS alternating",
                    "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": STAGE_KERNELS[r["dom"]], "bytes_per_launch": 16 * n,
+                     "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kernel, "bytes_per_launch": 16 * n,
                      "launch_ms": round(r["per_stage"][r["dom"]], 4)},
         "cpu_baseline": cpu,
         "poisson": {"solves_per_s": round(1e3 / r["ms_per"], 2),

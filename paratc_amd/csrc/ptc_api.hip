@@ -82,29 +82,46 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   double *s = nullptr, *d = nullptr;
   PTC_TRY(field(c, src, &s));
   PTC_TRY(field(c, dst, &d));
+  // Generalises the reference's timed search over its four schemes
+  // (src/m_Poisson.f90:50-171): every candidate plan is built and timed over
+  // three solves of the (scratch) src array; the fastest is kept.  First the
+  // FFT kernels (specialised or a generic z tile) with the default y-solve,
+  // then the y-solve (partitioned chunk length, or Thomas) with that FFT.
   struct Cand {
-    int tri, tile;
+    int tri, mm, tile;
   };
-  std::vector<Cand> cands;
-  for (int tri : {PTC_TRI_PARTITIONED, PTC_TRI_THOMAS})
-    for (int tile : {0, 16, 8, 4}) cands.push_back({tri, tile});
-  double best = 1e300;
-  Cand pick{PTC_TRI_PARTITIONED, 0};
-  for (const Cand &cd : cands) {
+  auto run = [&](const Cand &cd, double *ms) -> int {
     poisson_free_plan(c);
-    c->pp.tri = cd.tri;
+    c->pp.tri_req = cd.tri;
+    c->pp.tri_mm_req = cd.mm;
     c->pp.ztile_req = cd.tile;
-    if (poisson_build_plan(c) != PTC_OK) continue;
-    if (cd.tile > 0 && c->pp.ztile != cd.tile) continue;  // tile clipped by LDS: already covered
+    if (poisson_build_plan(c) != PTC_OK) return PTC_E_UNSUPPORTED;
+    if (cd.tile > 0 && c->pp.ztile != cd.tile) return PTC_E_UNSUPPORTED;  // clipped: covered elsewhere
+    return time_solve(c, s, d, 3, ms);
+  };
+  double best = 1e300;
+  Cand pick{PTC_TRI_AUTO, 0, 0};
+  for (int tile : {0, 16, 8, 4}) {
     double ms = 0;
-    if (time_solve(c, s, d, 3, &ms) != PTC_OK) continue;
-    if (ms < best) {
+    const Cand cd{PTC_TRI_AUTO, 0, tile};
+    if (run(cd, &ms) == PTC_OK && ms < best) {
+      best = ms;
+      pick = cd;
+    }
+  }
+  const int tile = pick.tile;
+  for (const Cand &cd : {Cand{PTC_TRI_PARTITIONED, 4, tile}, Cand{PTC_TRI_PARTITIONED, 8, tile},
+                         Cand{PTC_TRI_PARTITIONED, 12, tile}, Cand{PTC_TRI_PARTITIONED, 16, tile},
+                         Cand{PTC_TRI_THOMAS, 0, tile}}) {
+    double ms = 0;
+    if (run(cd, &ms) == PTC_OK && ms < best) {
       best = ms;
       pick = cd;
     }
   }
   poisson_free_plan(c);
-  c->pp.tri = pick.tri;
+  c->pp.tri_req = pick.tri;
+  c->pp.tri_mm_req = pick.mm;
   c->pp.ztile_req = pick.tile;
   PTC_TRY(poisson_build_plan(c));
   // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
@@ -313,9 +330,19 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps) {
   PTC_HIP(hipStreamSynchronize(ctx->stream));
   poisson_free_plan(ctx);
   ctx->pp.prec = prec;
-  ctx->pp.tri = tri == PTC_TRI_AUTO ? PTC_TRI_PARTITIONED : tri;
+  ctx->pp.tri_req = tri;
   ctx->pp.ztile_req = tile;
   ctx->pp.sweeps = sweeps;
+  return poisson_build_plan(ctx);
+}
+
+int ptc_poisson_chunk_rows(ptc_ctx *ctx, int rows) {
+  PTC_CTX(ctx);
+  PTC_CHECK(rows == 0 || rows == 4 || rows == 8 || rows == 12 || rows == 16, PTC_E_ARG,
+            "chunk rows must be 0, 4, 8, 12 or 16");
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  poisson_free_plan(ctx);
+  ctx->pp.tri_mm_req = rows;
   return poisson_build_plan(ctx);
 }
 
@@ -327,6 +354,10 @@ int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
   o[2] = ctx->pp.ztile;
   o[3] = ctx->pp.sweeps;
   o[4] = (ctx->pp.x_fast ? 1 : 0) | (ctx->pp.z_fast ? 2 : 0);
+  const bool part = ctx->pp.tri == PTC_TRI_PARTITIONED;
+  o[5] = part ? ctx->pp.tri_mmax : 0;
+  o[6] = part ? ctx->pp.tri_chunks : 0;
+  o[7] = part ? ctx->pp.tri_lpb : 0;
   return PTC_OK;
 }
 

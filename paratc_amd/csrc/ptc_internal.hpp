@@ -82,7 +82,9 @@ constexpr int kMaxRadix = 8;
 struct PoissonPlan {
   bool ready = false;
   int prec = PTC_PREC_FP64;
-  int tri = PTC_TRI_PARTITIONED;
+  int tri_req = PTC_TRI_AUTO;      // requested variant
+  int tri = PTC_TRI_PARTITIONED;   // variant in use
+  int tri_mm_req = 0;              // requested chunk rows (0 = automatic)
   int sweeps = 2;
   int h = 0;          // nx / 2
   int pitch = 0;      // complex row pitch of the spectral buffer (>= h + 1)

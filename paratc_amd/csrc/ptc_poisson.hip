@@ -29,7 +29,7 @@
 namespace ptc {
 
 bool fft_config_ok(const FftLen &L, int ncol, int nthr);
-int tri_plan(int ny, size_t lds_limit, bool poisson, int *P_out, int *mm_out, int *lpb_out);
+int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out);
 
 static void make_twiddles(int n, std::vector<double2> &out) {
   const double twopi = 6.283185307179586476925286766559005768;
@@ -49,7 +49,8 @@ int poisson_free_plan(ptc_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   PoissonPlan keep;
   keep.prec = P.prec;
-  keep.tri = P.tri;
+  keep.tri_req = P.tri_req;
+  keep.tri_mm_req = P.tri_mm_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.profile = P.profile;
@@ -96,7 +97,16 @@ int poisson_build_plan(ptc_ctx *c) {
     PTC_CHECK(P.z_threads > 0, PTC_E_UNSUPPORTED, "no z-FFT launch shape fits");
   }
 
-  PTC_TRY(tri_plan(c->ny, c->lds_limit, true, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb));
+  // y pass: partitioned unless Thomas is asked for, or (automatic choice) no
+  // partitioned shape fits this ny
+  P.tri = P.tri_req == PTC_TRI_THOMAS ? PTC_TRI_THOMAS : PTC_TRI_PARTITIONED;
+  if (P.tri == PTC_TRI_PARTITIONED) {
+    const int rc = tri_plan(c->ny, c->lds_limit, true, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb);
+    if (rc != PTC_OK) {
+      if (P.tri_req == PTC_TRI_PARTITIONED) return rc;
+      P.tri = PTC_TRI_THOMAS;
+    }
+  }
 
   // twiddle tables: [fx: h][fz: nz][wx: h+1]
   std::vector<double2> tx, tz, all;

@@ -216,9 +216,11 @@ __device__ __forceinline__ int chunk_of_thread() {
 }
 
 // ---- Poisson y-solve, partitioned ----------------------------------------
-// chunks of >= 16 rows need more than 128 VGPRs: blocks of at most 512 threads
+// Largest block of each kernel family: the real-valued Poisson kernels stay
+// within 128 VGPRs up to MM = 16; Crank-Nicolson chunks of 16 rows need more,
+// so their blocks hold at most 512 threads.
 template <int MM>
-constexpr int tri_block_cap() {
+constexpr int tri_cn_block_cap() {
   return MM >= 16 ? 512 : 1024;
 }
 
@@ -228,7 +230,7 @@ constexpr int tri_block_cap() {
 // access, the right-hand side takes half the registers, and with LPB = 64 a
 // chunk is one wavefront so the row coefficients are wave-uniform.
 template <int MM, int LPB>
-__global__ void __launch_bounds__(tri_block_cap<MM>()) k_tri_poisson_split(double2 *__restrict__ spec, const double *__restrict__ met,
+__global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict__ spec, const double *__restrict__ met,
                                                            int nyh, const double *__restrict__ wnx,
                                                            const double *__restrict__ wnz, int ny, int pitch, int h,
                                                            int P) {
@@ -307,7 +309,7 @@ __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict_
 // ---- Crank-Nicolson y-solves (u += du), partitioned -----------------------
 // kind: 0 = u/w (ForCN tables), 1 = v (row 1 Dirichlet), 2 = scalar (ScCN tables)
 template <int MM, int LPB>
-__global__ void __launch_bounds__(tri_block_cap<MM>()) k_tri_cn_part(double *__restrict__ u, const double *__restrict__ rhs,
+__global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *__restrict__ u, const double *__restrict__ rhs,
                                                      const double *__restrict__ duy_ym,
                                                      const double *__restrict__ shift, const double *__restrict__ met,
                                                      int nyh, int rowA, int rowB, int rowC, double beta, int kind,
@@ -349,18 +351,19 @@ __global__ void __launch_bounds__(tri_block_cap<MM>()) k_tri_cn_part(double *__r
 // instantiations that run without private-memory spills; the first one with
 // at most 1/8 padded rows whose block (LPB * P threads) and LDS fit wins.
 struct TriShape {
-  int mm, lpb;
+  int mm, lpb, cap;  // cap: the instantiation's launch bound
 };
-static const TriShape kPoissonShapes[] = {{4, 64}, {8, 64}, {12, 64}, {16, 64}};
-static const TriShape kCnShapes[] = {{4, 64}, {8, 64}, {8, 32}, {16, 32}};
+static const TriShape kPoissonShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {12, 64, 1024}, {16, 64, 1024}};
+static const TriShape kCnShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {8, 32, 1024}, {16, 32, 512}};
 
-static int tri_choose(int ny, size_t lds_limit, const TriShape *shapes, int nshapes, int *P_out,
+static int tri_choose(int ny, size_t lds_limit, int mm_req, const TriShape *shapes, int nshapes, int *P_out,
                       int *mm_out, int *lpb_out) {
   for (int pass = 0; pass < 2; ++pass)
     for (int q = 0; q < nshapes; ++q) {
       const int mm = shapes[q].mm, lpb = shapes[q].lpb;
+      if (mm_req > 0 && mm != mm_req) continue;
       const int P = (ny + mm - 1) / mm;
-      if (lpb * P > (mm >= 16 ? 512 : 1024)) continue;
+      if (lpb * P > shapes[q].cap) continue;
       if (2 * P * lpb > 1 << 16) continue;
       if (pass == 0 && (P * mm - ny) * 8 > ny) continue;  // prefer little padding
       const size_t lds = (size_t)2 * P * lpb * 24;  // a, c, f of the reduced rows
@@ -370,13 +373,14 @@ static int tri_choose(int ny, size_t lds_limit, const TriShape *shapes, int nsha
       *lpb_out = lpb;
       return PTC_OK;
     }
-  set_error("ny = " + std::to_string(ny) + " is outside the partitioned tridiagonal kernels");
+  set_error("ny = " + std::to_string(ny) + (mm_req > 0 ? " with " + std::to_string(mm_req) + "-row chunks" : "") +
+            " is outside the partitioned tridiagonal kernels");
   return PTC_E_UNSUPPORTED;
 }
 
-int tri_plan(int ny, size_t lds_limit, bool poisson, int *P_out, int *mm_out, int *lpb_out) {
-  return poisson ? tri_choose(ny, lds_limit, kPoissonShapes, 4, P_out, mm_out, lpb_out)
-                 : tri_choose(ny, lds_limit, kCnShapes, 4, P_out, mm_out, lpb_out);
+int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out) {
+  return poisson ? tri_choose(ny, lds_limit, mm_req, kPoissonShapes, 4, P_out, mm_out, lpb_out)
+                 : tri_choose(ny, lds_limit, mm_req, kCnShapes, 4, P_out, mm_out, lpb_out);
 }
 
 int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
@@ -413,7 +417,7 @@ int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
                   const double *shift) {
   int nP = 1, mmax = 16, lpb = 64;
-  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, &nP, &mmax, &lpb));
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, &nP, &mmax, &lpb));
   int rA, rB, rC;
   if (kind == 0) { rA = M_AM2CCN; rB = M_AC2CCN; rC = M_AP2CCN; }
   else if (kind == 1) { rA = M_AM2P; rB = M_AC2P; rC = M_AP2P; }

@@ -1,0 +1,103 @@
+"""Summaries of rocprofv3 output for the kernels of this library.
+
+    python -m paratc_amd.profsum trace <run_results.db>
+        per-kernel dispatch count and durations from a --kernel-trace run
+    python -m paratc_amd.profsum traffic <fetch.csv> <write.csv> <copy_bytes>
+        per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
+        --pmc runs, scaled to HBM bytes by what the same counters read for
+        k_copy, the plain device copy of copy_bytes bytes (read once, written
+        once; bench.py --calib-copy copies one field of nx * (ny + 2) * nz
+        doubles)
+
+The traffic table is what bench.py reads to fill roofline.traffic.  Scaling:
+read_factor = copy bytes / copy FETCH_SIZE, write_factor = copy bytes /
+copy WRITE_SIZE.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import sqlite3
+import sys
+from typing import Dict, List, Optional, Tuple
+
+TRAFFIC_HEADER = "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | read bytes | write bytes | traffic bytes |"
+
+
+def short_name(name: str) -> str:
+    """Kernel name without the argument list (template arguments kept)."""
+    depth = 0
+    for q, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            name = name[:q]
+            break
+    for pre in ("void ", "ptc::"):
+        name = name.replace(pre, "")
+    return name.strip()
+
+
+def trace_table(db: str) -> str:
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, count(*), sum(end-start), avg(end-start), min(end-start), max(end-start) "
+                       "from kernels group by name order by sum(end-start) desc").fetchall()
+    tot = sum(r[2] for r in rows) or 1
+    out = ["| kernel | calls | total us | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for name, n, s, a, lo, hi in rows:
+        out.append(f"| {short_name(name)} | {n} | {s / 1e3:.1f} | {a / 1e3:.2f} | {lo / 1e3:.2f} | {hi / 1e3:.2f} "
+                   f"| {100 * s / tot:.1f} |")
+    return "\n".join(out)
+
+
+def _counter_means(path: str) -> Dict[str, Tuple[int, float]]:
+    acc: Dict[str, List[float]] = collections.defaultdict(list)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
+    fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
+    if "k_copy" not in fetch or "k_copy" not in write:
+        raise SystemExit("k_copy is not in the counter files: profile bench.py --calib-copy")
+    rf = nbytes / (fetch["k_copy"][1] * 1024.0)
+    wf = nbytes / (write["k_copy"][1] * 1024.0)
+    out = [f"read_factor {rf:.4f} write_factor {wf:.4f} (k_copy of {nbytes:.0f} B each way)", "",
+           TRAFFIC_HEADER, "|---|---:|---:|---:|---:|---:|---:|"]
+    for k in sorted(fetch, key=lambda k: -fetch[k][1]):
+        if k not in write:
+            continue
+        fr, wr = fetch[k][1], write[k][1]
+        rb, wb = fr * 1024.0 * rf, wr * 1024.0 * wf
+        out.append(f"| {k} | {fetch[k][0]} | {fr:.1f} | {wr:.1f} | {rb:.0f} | {wb:.0f} | {rb + wb:.0f} |")
+    return "\n".join(out)
+
+
+def read_traffic(md_path: str, kernel: str) -> Optional[float]:
+    """Traffic bytes per launch of `kernel` (a short-name prefix) from a traffic table, or None."""
+    try:
+        lines = open(md_path).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].startswith(kernel) and cells[6].replace(".", "").isdigit():
+            return float(cells[6])
+    return None
+
+
+def main(argv: List[str]) -> None:
+    if len(argv) >= 2 and argv[0] == "trace":
+        print(trace_table(argv[1]))
+    elif len(argv) >= 4 and argv[0] == "traffic":
+        print(traffic_table(argv[1], argv[2], float(argv[3])))
+    else:
+        raise SystemExit(__doc__)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -104,12 +104,16 @@ class Solver:
     def poisson_config(self, prec=capi.PREC_FP64, tri=capi.TRI_AUTO, tile=0, sweeps=2):
         check(self.L.ptc_poisson_config(self.h, prec, tri, tile, sweeps), "ptc_poisson_config")
 
+    def poisson_chunk_rows(self, rows: int = 0):
+        check(self.L.ptc_poisson_chunk_rows(self.h, rows), "ptc_poisson_chunk_rows")
+
     def poisson_get_config(self) -> dict:
-        o = (C.c_int * 5)()
+        o = (C.c_int * 8)()
         check(self.L.ptc_poisson_get_config(self.h, o), "ptc_poisson_get_config")
         tri = {capi.TRI_THOMAS: "thomas", capi.TRI_PARTITIONED: "partitioned"}.get(o[1], str(o[1]))
         return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
-                "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2)}
+                "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2),
+                "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7]}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()

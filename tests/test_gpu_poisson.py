@@ -25,10 +25,12 @@ GRIDS = [
     (40, 12, 70, 1.6),     # nx/2 = 20 = 4*5,  nz = 70 = 2*5*7
     (128, 96, 128, 1.8),   # the 128x96x128 channel grid
     (96, 140, 50, 2.2),    # ny = 140 as in the large case
+    (32, 192, 16, 1.6),    # ny = 192 as in the gravity current
+    (64, 256, 32, 2.0),    # ny = 256 as in the 1024x256x512 solve
 ]
 
 
-def solve_both(grid, tri, seed=1, compat=True):
+def solve_both(grid, tri, seed=1, compat=True, rows=0):
     nx, ny, nz, gam = grid
     case = CS.case_poisson(nx, ny, nz, gam)
     f = ic.poisson_rhs(case, seed=seed) if compat else np.asfortranarray(
@@ -41,6 +43,10 @@ def solve_both(grid, tri, seed=1, compat=True):
     lnorm = float((np.abs(am) + np.abs(ac) + np.abs(ap)).max() + (1460 + 2 * 838) / 576 * (rdx2 + rdz2))
     with Solver(case) as s:
         s.poisson_config(capi.PREC_FP64, tri, 0, 0)
+        if rows:
+            s.poisson_chunk_rows(rows)
+            cfg = s.poisson_get_config()
+            assert cfg["tri"] == "partitioned" and cfg["chunk_rows"] == rows, cfg
         s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         out = s.download(capi.F_HALO, capi.L_INTERIOR)
@@ -61,6 +67,27 @@ def test_poisson_matches_oracle(grid, tri):
     assert abs(out[:, 0, :].mean()) <= 1e-13 * np.abs(out).max()
     # compatible RHS: residual at round-off relative to |L| |phi| (res is divided by |L|)
     assert res <= 1e-13 * np.abs(out).max()
+
+
+@pytest.mark.parametrize("ny", [64, 60, 50])
+@pytest.mark.parametrize("rows", [4, 8, 12, 16])
+def test_partitioned_chunk_lengths_match_oracle(ny, rows):
+    """Every compiled chunk length, with and without identity padding rows."""
+    f, ref, out, _, res = solve_both((32, ny, 24, 1.9), capi.TRI_PARTITIONED, seed=2, rows=rows)
+    assert rel_err(out, ref) <= TOL
+    assert res <= 1e-13 * np.abs(out).max()
+
+
+def test_automatic_variant_falls_back_to_thomas():
+    """ny beyond every partitioned shape: AUTO runs Thomas, PARTITIONED refuses."""
+    case = CS.case_poisson(16, 272, 8, 2.0)
+    with Solver(case) as s:
+        s.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
+        assert s.poisson_get_config()["tri"] == "thomas"
+        with pytest.raises(capi.PtcError, match="partitioned"):
+            s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
+    f, ref, out, _, _ = solve_both((16, 272, 8, 2.0), capi.TRI_AUTO, seed=3)
+    assert rel_err(out, ref) <= TOL
 
 
 def test_poisson_incompatible_rhs_matches_oracle():

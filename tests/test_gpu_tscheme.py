@@ -248,41 +248,49 @@ def run_iterate(case, nsteps, seed=0, choice=1):
     return out, ref, mons, coe, orc.coe()
 
 
-def reference_pressure_spread(case, nsteps):
-    """Spread of p between the reference's own Poisson variants (Choice-1 vs
-    Choice-3, src/m_Poisson.f90:352-695): same operator, different rounding.
-    The solve's smooth low modes are ill-conditioned, so p carries this spread
-    while the projected velocity does not."""
-    ps = []
-    for choice in (1, 3):
-        orc = Oracle(case)
-        st = State(orc)
-        if case.scalar:
-            st.ux, st.uy, st.uz, st.scalar = ic.gravity_current_fields(case, seed=0, amp=1e-2)
-        else:
-            st.ux, st.uy, st.uz = ic.channel_fields(case, seed=0, amp=0.1)
-        orc.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
-        orc.SetBC_and_UpdateHalo_pr(st.p)
-        if case.scalar:
-            orc.SetBC_and_UpdateHalo_scalar(st.scalar)
-        for _ in range(nsteps):
-            orc.ChannelIterate(st, choice)
-        ps.append(inner_y(st.p).copy())
-    return rel_err(ps[1], ps[0])
+def reference_pressure_spread(case, nsteps, p_choice1):
+    """Spread of p between the reference's own Poisson variants (Choice-1, the
+    oracle run already made, vs Choice-3, src/m_Poisson.f90:352-695): same
+    operator, different rounding.  The solve's smooth low modes are
+    ill-conditioned, so p carries this spread while the projected velocity
+    does not."""
+    orc = Oracle(case)
+    st = State(orc)
+    if case.scalar:
+        st.ux, st.uy, st.uz, st.scalar = ic.gravity_current_fields(case, seed=0, amp=1e-2)
+    else:
+        st.ux, st.uy, st.uz = ic.channel_fields(case, seed=0, amp=0.1)
+    orc.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
+    orc.SetBC_and_UpdateHalo_pr(st.p)
+    if case.scalar:
+        orc.SetBC_and_UpdateHalo_scalar(st.scalar)
+    for _ in range(nsteps):
+        orc.ChannelIterate(st, 3)
+    return rel_err(inner_y(st.p), inner_y(p_choice1))
 
 
+def c0_channel():
+    """The closed channel of the 128 x 96 x 128 workload: 4pi x 2 x 4pi/3,
+    gamma 1.8, Re_b 2800, constant flow rate, dt 2e-3."""
+    return CS.case_channel(128, 96, 128, 1.8)
+
+
+# the bar after 100 steps is 1e-9 relative, after one step 1e-12 (p: or twice
+# the spread of the reference's own Poisson variants, whichever is larger)
 @pytest.mark.parametrize("kind,nsteps,tol", [("channel", 1, 1e-12), ("channel", 10, 1e-10),
-                                             ("gravity", 1, 1e-12), ("gravity", 5, 1e-10)])
+                                             ("channel", 100, 1e-9),
+                                             ("gravity", 1, 1e-12), ("gravity", 5, 1e-10),
+                                             ("c0", 1, 1e-12), ("c0", 100, 1e-9)])
 def test_channel_iterate(kind, nsteps, tol):
-    case = small_channel() if kind == "channel" else small_gravity()
+    case = {"channel": small_channel, "gravity": small_gravity, "c0": c0_channel}[kind]()
     out, ref, mons, coe, ocoe = run_iterate(case, nsteps)
-    spread = reference_pressure_spread(case, nsteps)
+    spread = reference_pressure_spread(case, nsteps, ref[F.F_PR])
     for fid in ref:
         e = rel_err(inner_y(out[fid]), inner_y(ref[fid]))
         bar = tol if fid != F.F_PR else max(tol, 2.0 * spread)
         print(kind, nsteps, fid, f"{e:.3e} (bar {bar:.1e}, reference variant spread {spread:.1e})", flush=True)
         assert e <= bar, fid
-    assert spread <= 5e-11
+    assert spread <= 5e-10
     mo, mg = mons[-1]
     assert rel_err(mg[1:5], mo[1:5]) <= 1e-9
     # post-projection divergence at round-off
