@@ -160,14 +160,22 @@ int ptc_destroy_poisson_plan(ptc_ctx *ctx);
  * forces the generic kernels with that z column tile; sweeps = refinement
  * sweeps for PTC_PREC_FP32_REFINE */
 int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
-/* rows per chunk of the partitioned y-solve (4, 8, 12 or 16; 0 = the
- * shortest chunk whose block fits); rebuilds the plan.  With tri =
- * PTC_TRI_AUTO a grid that no partitioned shape fits falls back to Thomas. */
-int ptc_poisson_chunk_rows(ptc_ctx *ctx, int rows);
-/* the plan in use: out8 = [prec, tri, z tile, sweeps, specialised-kernel
+/* Further plan knobs (each rebuilds the plan; InitPoissonSolver's search sets
+ * them too):
+ *   PTC_POISSON_CHUNK_ROWS  rows per chunk of the partitioned y-solve (4, 8,
+ *                           12 or 16; 0 = the shortest chunk whose workgroup
+ *                           fits).  With tri = PTC_TRI_AUTO a grid no
+ *                           partitioned shape fits falls back to Thomas.
+ *   PTC_POISSON_SLAB_PLANES run the x and z passes slab by slab of this many
+ *                           y planes (0 = whole field; specialised FFT
+ *                           kernels only), so each z pass finds its input in
+ *                           the last-level cache. */
+enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2 };
+int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
+/* the plan in use: out9 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per
- * workgroup] (after autotuning; the last three are 0 for Thomas) */
-int ptc_poisson_get_config(ptc_ctx *ctx, int *out8);
+ * workgroup, slab planes] (after autotuning; chunk fields are 0 for Thomas) */
+int ptc_poisson_get_config(ptc_ctx *ctx, int *out9);
 /* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
 /* Per-stage kernel times: with enable = 1 every later solve records HIP events

@@ -383,7 +383,7 @@ static int launch_x_fwd_t(ptc_ctx *c, const double *src, double2 *spec) {
 }
 
 int launch_fft_x_fwd(ptc_ctx *c, const double *src, double2 *spec) {
-  if (c->pp.x_fast) return fft2_x_fwd(c, src, spec);
+  if (c->pp.x_fast) return fft2_x_fwd(c, src, spec, 0, c->ny);
   return stage_fits(c->pp.fx, c->pp.x_lines, c->pp.x_threads, 16) ? launch_x_fwd_t<16>(c, src, spec)
                                                                   : launch_x_fwd_t<32>(c, src, spec);
 }
@@ -401,7 +401,7 @@ static int launch_x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double s
 }
 
 int launch_fft_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
-  if (c->pp.x_fast) return fft2_x_inv(c, spec, dst, scale);
+  if (c->pp.x_fast) return fft2_x_inv(c, spec, dst, scale, 0, c->ny);
   return stage_fits(c->pp.fx, c->pp.x_lines, c->pp.x_threads, 16) ? launch_x_inv_t<16>(c, spec, dst, scale)
                                                                   : launch_x_inv_t<32>(c, spec, dst, scale);
 }
@@ -418,7 +418,7 @@ static int launch_z_t(ptc_ctx *c, double2 *spec) {
 }
 
 int launch_fft_z(ptc_ctx *c, double2 *spec, bool inverse) {
-  if (c->pp.z_fast) return fft2_z(c, spec, inverse);
+  if (c->pp.z_fast) return fft2_z(c, spec, inverse, 0, c->ny);
   const bool small = stage_fits(c->pp.fz, c->pp.ztile, c->pp.z_threads, 16);
   if (inverse) return small ? launch_z_t<16, true>(c, spec) : launch_z_t<32, true>(c, spec);
   return small ? launch_z_t<16, false>(c, spec) : launch_z_t<32, false>(c, spec);

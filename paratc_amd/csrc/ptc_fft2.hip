@@ -228,9 +228,9 @@ __device__ __forceinline__ void stage(int g, const double2 *__restrict__ tw, con
 
 // Full transform of one line group.  Stage 0 loads through load0, the last
 // stage stores through storeL; intermediate data lives in LDS through lds(m).
-// LAST_TO_LDS: the last stage stores into LDS as well (the caller reads the
-// result from LDS after a barrier).
-template <int N, int G, bool INV, class Load0, class LdsAt, class StoreL>
+// LOAD0_LDS: load0 itself reads the LDS buffer, so stage 0 needs a barrier
+// between its loads and its stores.
+template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class LdsAt, class StoreL>
 __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
                                         const StoreL &storeL) {
   constexpr PlanC P = make_plan(N);
@@ -239,7 +239,7 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
   if constexpr (P.ns == 1) {
     stage<N, 0, G, INV, false>(g, tw, load0, storeL);
   } else {
-    stage<N, 0, G, INV, false>(g, tw, load0, ldsstore);
+    stage<N, 0, G, INV, LOAD0_LDS>(g, tw, load0, ldsstore);
     __syncthreads();
     if constexpr (P.ns > 2) {
       stage<N, 1, G, INV, true>(g, tw, ldsload, ldsstore);
@@ -274,22 +274,30 @@ constexpr int f2_threads() {
   return N / f2::plan_rmax(N);
 }
 
+// x-pass workgroup: 256 threads (several lines) or one line of G > 256 threads
+template <int N>
+constexpr int f2_xblock() {
+  return f2_threads<N>() > 256 ? f2_threads<N>() : 256;
+}
+
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
 template <int N>
-__global__ void __launch_bounds__(256) k2_fft_x_fwd(const double *__restrict__ src, double2 *__restrict__ spec,
+__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, double2 *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
-                                                    int ny, int nz, int pitch, long long rowlen) {
+                                                    int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = f2::padded(N);
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
-  const long long nlines = (long long)ny * nz;
+  // lines (j, k) of the planes j0 .. j0+nj-1 (0-based), all k
+  const long long nlines = (long long)nj * nz;
   const long long q = (long long)blockIdx.x * L + l;
   const bool live = l < L && q < nlines;
   const long long qq = live ? q : 0;
-  const long long j = qq % ny + 1, k = qq / ny;
-  const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (j + (long long)(ny + 2) * k));
+  const long long jl = j0 + qq % nj, k = qq / nj;
+  const long long sq = jl + (long long)ny * k;  // spectral row S[k][jl]
+  const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   double2 *ls = sh + (l < L ? l : 0) * PN;
   auto load0 = [&](int m) { return live ? row[m] : make_double2(0.0, 0.0); };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
@@ -297,7 +305,7 @@ __global__ void __launch_bounds__(256) k2_fft_x_fwd(const double *__restrict__ s
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
   __syncthreads();
   if (!live) return;
-  double2 *out = spec + q * pitch;
+  double2 *out = spec + sq * pitch;
   for (int kx = g; kx < pitch; kx += G) {
     double2 X = make_double2(0.0, 0.0);
     if (kx <= N) {
@@ -314,24 +322,29 @@ __global__ void __launch_bounds__(256) k2_fft_x_fwd(const double *__restrict__ s
 
 // ---- x inverse: half spectra -> real rows times scale ----------------------
 template <int N>
-__global__ void __launch_bounds__(256) k2_fft_x_inv(const double2 *__restrict__ spec, double *__restrict__ dst,
+__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const double2 *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
-                                                    int ny, int nz, int pitch, long long rowlen, double scale) {
+                                                    int ny, int nz, int j0, int nj, int pitch, long long rowlen,
+                                                    double scale) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = f2::padded(N);
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
-  const long long nlines = (long long)ny * nz;
+  const long long nlines = (long long)nj * nz;
   const long long q = (long long)blockIdx.x * L + l;
   const bool live = l < L && q < nlines;
   const long long qq = live ? q : 0;
-  const double2 *in = spec + qq * pitch;
+  const long long jl = j0 + qq % nj, k = qq / nj;
+  const double2 *in = spec + (jl + (long long)ny * k) * pitch;
   double2 *ls = sh + (l < L ? l : 0) * PN;
+  // The re-tangle of element m needs X_m and X_{N-m}: the line's N+1 values
+  // go through LDS once (coalesced) instead of being fetched twice.
+  for (int m = g; m <= N; m += G) ls[f2::pad8(m)] = live ? in[m] : make_double2(0.0, 0.0);
+  __syncthreads();
   auto load0 = [&](int m) {
-    if (!live) return make_double2(0.0, 0.0);
-    const double2 xk = in[m];
-    double2 xc = in[N - m];
+    const double2 xk = ls[f2::pad8(m)];
+    double2 xc = ls[f2::pad8(N - m)];
     xc.y = -xc.y;
     const double2 s = f2::cadd(xk, xc), d = f2::csub(xk, xc);
     double2 w = wx[m];
@@ -340,23 +353,22 @@ __global__ void __launch_bounds__(256) k2_fft_x_inv(const double2 *__restrict__ 
     return make_double2(s.x - wd.y, s.y + wd.x);
   };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
-  const long long j = qq % ny + 1, k = qq / ny;
-  double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (j + (long long)(ny + 2) * k));
+  double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   auto store_out = [&](int m, double2 v) {
     if (live) row[m] = make_double2(v.x * scale, v.y * scale);
   };
-  f2::run_fft<N, G, true>(g, tw, load0, lds, store_out);
+  f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
 }
 
 // ---- z pass: T columns x N rows, thread id = g * T + col ----------------
 template <int N, int T, bool INV>
 __global__ void __launch_bounds__(512) k2_fft_z(double2 *__restrict__ spec, const double2 *__restrict__ tw, int ny,
-                                                 int pitch) {
+                                                 int pitch, int j0) {
   constexpr int G = f2_threads<N>();
   __shared__ double2 sh[N * T];
   const int col = threadIdx.x % T, g = threadIdx.x / T;
   const long long zs = (long long)pitch * ny;
-  double2 *base = spec + (long long)blockIdx.y * pitch + (long long)blockIdx.x * T + col;
+  double2 *base = spec + (long long)(j0 + blockIdx.y) * pitch + (long long)blockIdx.x * T + col;
   auto load0 = [&](int m) { return base[m * zs]; };
   auto lds = [&](int m) -> double2 & { return sh[m * T + col]; };
   auto store_out = [&](int m, double2 v) { base[m * zs] = v; };
@@ -365,44 +377,47 @@ __global__ void __launch_bounds__(512) k2_fft_z(double2 *__restrict__ spec, cons
 
 // ---- dispatch ----------------------------------------------------------------
 template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, double2 *spec) {
+static int x_fwd_t(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  const long long nlines = (long long)c->ny * c->nz;
+  const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
   hipLaunchKernelGGL(k2_fft_x_fwd<N>, dim3(nblk), dim3(L * G), 0, c->stream, src, spec, c->pp.fx.tw, c->pp.d_wx,
-                     c->ny, c->nz, c->pp.pitch, (long long)c->nx);
+                     c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
 template <int N>
-static int x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
+static int x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  const long long nlines = (long long)c->ny * c->nz;
+  const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
   hipLaunchKernelGGL(k2_fft_x_inv<N>, dim3(nblk), dim3(L * G), 0, c->stream, spec, dst, c->pp.fx.tw, c->pp.d_wx,
-                     c->ny, c->nz, c->pp.pitch, (long long)c->nx, scale);
+                     c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, scale);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
 template <int N, int T>
-static int z_t(ptc_ctx *c, double2 *spec, bool inverse) {
+static int z_t(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj) {
   constexpr int G = f2_threads<N>();
-  dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)c->ny);
+  dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)nj);
   if (inverse)
-    hipLaunchKernelGGL((k2_fft_z<N, T, true>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny, c->pp.pitch);
+    hipLaunchKernelGGL((k2_fft_z<N, T, true>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny, c->pp.pitch,
+                       j0);
   else
     hipLaunchKernelGGL((k2_fft_z<N, T, false>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny,
-                       c->pp.pitch);
+                       c->pp.pitch, j0);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
 // x lengths h = nx/2 with a specialised kernel
-#define PTC_X_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(768)
+// (2304 and 4608: the 4608- and 9216-point rows of the largest cases; their
+// lines take 41 KB and 83 KB of LDS, which gfx950's 160 KB allows)
+#define PTC_X_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(768) X(2304) X(4608)
 
 bool fft2_has_x(int h) {
 #define PTC_CASE(N) if (h == N) return true;
@@ -411,18 +426,18 @@ bool fft2_has_x(int h) {
   return false;
 }
 
-int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec) {
+int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec);
+#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;
   }
 }
 
-int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
+int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale);
+#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale, j0, nj);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;
@@ -430,8 +445,9 @@ int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale) {
 }
 
 // z lengths with a specialised kernel and their column tile (<= 64 KB LDS,
-// <= 1024 threads)
-#define PTC_Z_SIZES(Z) Z(16, 16) Z(32, 16) Z(64, 16) Z(96, 16) Z(128, 16) Z(192, 16) Z(256, 16) Z(512, 8)
+// <= 512 threads)
+#define PTC_Z_SIZES(Z) \
+  Z(16, 16) Z(32, 16) Z(64, 16) Z(96, 16) Z(128, 16) Z(192, 16) Z(256, 16) Z(512, 8) Z(700, 4) Z(1400, 2)
 
 bool fft2_has_z(int nz, int *tile) {
 #define PTC_CASE(N, T) if (nz == N) { *tile = T; return true; }
@@ -440,9 +456,9 @@ bool fft2_has_z(int nz, int *tile) {
   return false;
 }
 
-int fft2_z(ptc_ctx *c, double2 *spec, bool inverse) {
+int fft2_z(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj) {
   switch (c->nz) {
-#define PTC_CASE(N, T) case N: return z_t<N, T>(c, spec, inverse);
+#define PTC_CASE(N, T) case N: return z_t<N, T>(c, spec, inverse, j0, nj);
     PTC_Z_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;

@@ -85,6 +85,8 @@ struct PoissonPlan {
   int tri_req = PTC_TRI_AUTO;      // requested variant
   int tri = PTC_TRI_PARTITIONED;   // variant in use
   int tri_mm_req = 0;              // requested chunk rows (0 = automatic)
+  int slab_req = 0;                // requested planes per x/z slab (0 = whole field)
+  int slab = 0;                    // planes per slab in use (0 = unblocked)
   int sweeps = 2;
   int h = 0;          // nx / 2
   int pitch = 0;      // complex row pitch of the spectral buffer (>= h + 1)
@@ -160,9 +162,10 @@ int launch_fft_z(ptc_ctx *c, double2 *spec, bool inverse);
 int fft_make_len(int n, FftLen *L);
 bool fft2_has_x(int h);
 bool fft2_has_z(int nz, int *tile);
-int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec);
-int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale);
-int fft2_z(ptc_ctx *c, double2 *spec, bool inverse);
+// specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum
+int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj);
+int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj);
+int fft2_z(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj);
 
 // tridiagonal launchers (ptc_tridiag.hip)
 int launch_tri_poisson(ptc_ctx *c, double2 *spec);

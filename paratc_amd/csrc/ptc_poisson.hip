@@ -51,6 +51,7 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.prec = P.prec;
   keep.tri_req = P.tri_req;
   keep.tri_mm_req = P.tri_mm_req;
+  keep.slab_req = P.slab_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.profile = P.profile;
@@ -68,19 +69,21 @@ int poisson_build_plan(ptc_ctx *c) {
   PTC_TRY(fft_make_len(c->nz, &P.fz));
   const size_t lds_cap = std::min<size_t>(c->lds_limit, 65536);
 
-  // x pass: lines per workgroup so that a tile holds ~2048 complex values
-  P.x_lines = std::max(1, 2048 / P.h);
-  while (P.x_lines > 1 && sizeof(double2) * (size_t)P.h * P.x_lines > lds_cap) P.x_lines /= 2;
-  PTC_CHECK(sizeof(double2) * (size_t)P.h * P.x_lines <= lds_cap, PTC_E_UNSUPPORTED,
-            "nx too large for the LDS x-FFT");
-  P.x_threads = 0;
-  for (int t : {256, 512, 1024})
-    if (fft_config_ok(P.fx, P.x_lines, t)) { P.x_threads = t; break; }
-  PTC_CHECK(P.x_threads > 0, PTC_E_UNSUPPORTED, "no x-FFT launch shape fits");
-
   // specialised kernels for the production lengths unless a tile is forced
-  int zt = 0;
   P.x_fast = P.ztile_req == 0 && fft2_has_x(P.h);
+  if (!P.x_fast) {
+    // generic x pass: lines per workgroup so that a tile holds ~2048 complex values
+    P.x_lines = std::max(1, 2048 / P.h);
+    while (P.x_lines > 1 && sizeof(double2) * (size_t)P.h * P.x_lines > lds_cap) P.x_lines /= 2;
+    PTC_CHECK(sizeof(double2) * (size_t)P.h * P.x_lines <= lds_cap, PTC_E_UNSUPPORTED,
+              "nx too large for the LDS x-FFT");
+    P.x_threads = 0;
+    for (int t : {256, 512, 1024})
+      if (fft_config_ok(P.fx, P.x_lines, t)) { P.x_threads = t; break; }
+    PTC_CHECK(P.x_threads > 0, PTC_E_UNSUPPORTED, "no x-FFT launch shape fits");
+  }
+
+  int zt = 0;
   P.z_fast = P.ztile_req == 0 && fft2_has_z(c->nz, &zt) && P.pitch % zt == 0;
   if (P.z_fast) {
     P.ztile = zt;
@@ -96,6 +99,11 @@ int poisson_build_plan(ptc_ctx *c) {
       if (fft_config_ok(P.fz, T, t)) { P.z_threads = t; break; }
     PTC_CHECK(P.z_threads > 0, PTC_E_UNSUPPORTED, "no z-FFT launch shape fits");
   }
+
+  // x/z passes over slabs of y planes (specialised kernels only): the z pass
+  // then reads a slab the x pass has just written while it is still held in
+  // the device's last-level cache, and likewise on the way back
+  P.slab = (P.x_fast && P.z_fast && P.slab_req > 0 && P.slab_req < c->ny) ? P.slab_req : 0;
 
   // y pass: partitioned unless Thomas is asked for, or (automatic choice) no
   // partitioned shape fits this ny
@@ -162,15 +170,34 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
     ++P.prof_used;
   }
   if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
-  PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
-  if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
-  PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  if (P.slab == 0) {
+    PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
+    if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
+    PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  } else {
+    // slabbed: the x and z times are reported together under x_fwd / x_inv
+    for (int j0 = 0; j0 < c->ny; j0 += P.slab) {
+      const int nj = std::min(P.slab, c->ny - j0);
+      PTC_TRY(fft2_x_fwd(c, src, P.d_spec, j0, nj));
+      PTC_TRY(fft2_z(c, P.d_spec, false, j0, nj));
+    }
+    if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
+  }
   if (ev) PTC_HIP(hipEventRecord(ev[2], c->stream));
   PTC_TRY(launch_tri_poisson(c, P.d_spec));
   if (ev) PTC_HIP(hipEventRecord(ev[3], c->stream));
-  PTC_TRY(launch_fft_z(c, P.d_spec, true));
-  if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
-  PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
+  if (P.slab == 0) {
+    PTC_TRY(launch_fft_z(c, P.d_spec, true));
+    if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
+    PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
+  } else {
+    if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
+    for (int j0 = 0; j0 < c->ny; j0 += P.slab) {
+      const int nj = std::min(P.slab, c->ny - j0);
+      PTC_TRY(fft2_z(c, P.d_spec, true, j0, nj));
+      PTC_TRY(fft2_x_inv(c, P.d_spec, dst, c->normfft, j0, nj));
+    }
+  }
   if (ev) PTC_HIP(hipEventRecord(ev[5], c->stream));
   return PTC_OK;
 }

@@ -86,126 +86,121 @@ struct CnRows {  // (I - beta L_y) with the wall rows of the chosen table
 // (src/m_Tools.f90:239-245), so the padded system has the same solution on the
 // real rows and all control flow here is compile-time.
 //
-// Reduced rows of one chunk:
-//   first row:  aS x_{s-1} + x_s    + cS x_last   = fS
-//   last row:   aE x_s     + x_last + cE x_{e+1}  = fE
-// as left by the reference's modified Thomas (src/m_Poisson.f90:497-531).  The
-// last row comes from a downward elimination and the first row from an upward
-// one, each with O(1) state, so no per-row coefficient arrays are kept.
+// Chunk rows r = 0..MM-1 read a_r x_{r-1} + b_r x_r + c_r x_{r+1} = f_r.  The
+// downward elimination of rows 1..MM-1 keeps the chunk's first value x_0 as a
+// parameter, as the reference's modified Thomas does (src/m_Poisson.f90:497-531):
+//     x_r + C_r x_{r+1} = F_r + A_r x_0 ,
+// one division per row; F_r overwrites f_r.  Its last row is the chunk's
+// reduced last row
+//     aE x_0 + x_last + cE x_{next first} = fE ,
+// and an upward pass with O(1) state writes x_1 in terms of x_0 and x_last,
+// which turns row 0 into the reduced first row
+//     aS x_{prev last} + x_0 + cS x_last = fS .
 template <int MM, typename V, typename G>
-__device__ __forceinline__ void chunk_reduced_rows(const V (&f)[MM], int s, G &getc, double &aS, double &cS,
-                                                   V &fS, double &aE, double &cE, V &fE) {
+__device__ __forceinline__ void chunk_reduce(V (&f)[MM], double (&A)[MM], double (&C)[MM], int s, G &getc,
+                                             double &aS, double &cS, V &fS, double &aE, double &cE, V &fE) {
   static_assert(MM >= 4, "chunks need at least four rows");
   double aj, bj, cj;
   getc(s + 2, aj, bj, cj);
-  double inv = 1.0 / bj;
-  double A = aj * inv, Cc = cj * inv;
-  V Fv = vscale(inv, f[1]);
+  double rr = 1.0 / bj;
+  C[1] = cj * rr;
+  A[1] = -aj * rr;
+  f[1] = vscale(rr, f[1]);
 #pragma unroll
   for (int r = 2; r < MM; ++r) {
     getc(s + r + 1, aj, bj, cj);
-    const double rr = 1.0 / (bj - aj * Cc);
-    Fv = vscale(rr, vsub(f[r], vscale(aj, Fv)));
-    A = -rr * aj * A;
-    Cc = rr * cj;
+    rr = 1.0 / (bj - aj * C[r - 1]);
+    C[r] = cj * rr;
+    A[r] = -rr * aj * A[r - 1];
+    f[r] = vscale(rr, vsub(f[r], vscale(aj, f[r - 1])));
   }
-  aE = A;
-  cE = Cc;
-  fE = Fv;
-  getc(s + MM - 1, aj, bj, cj);
-  const double iv = 1.0 / bj;
-  double Ap = aj * iv, Cp = cj * iv;
-  V Fp = vscale(iv, f[MM - 2]);
+  aE = -A[MM - 1];
+  cE = C[MM - 1];
+  fE = f[MM - 1];
+  // x_r = g + al x_0 + ga x_last, from r = MM-2 down to r = 1
+  V g = f[MM - 2];
+  double al = A[MM - 2], ga = -C[MM - 2];
 #pragma unroll
   for (int r = MM - 3; r >= 1; --r) {
-    getc(s + r + 1, aj, bj, cj);
-    const double rr = 1.0 / (bj - cj * Ap);
-    Fp = vscale(rr, vsub(f[r], vscale(cj, Fp)));
-    Cp = -rr * cj * Cp;
-    Ap = rr * aj;
+    g = vsub(f[r], vscale(C[r], g));
+    al = A[r] - C[r] * al;
+    ga = -C[r] * ga;
   }
   getc(s + 1, aj, bj, cj);
-  const double r0 = 1.0 / (bj - cj * Ap);
+  const double r0 = 1.0 / (bj + cj * al);
   aS = r0 * aj;
-  cS = -r0 * cj * Cp;
-  fS = vscale(r0, vsub(f[0], vscale(cj, Fp)));
+  cS = r0 * cj * ga;
+  fS = vscale(r0, vsub(f[0], vscale(cj, g)));
 }
 
-// With the chunk's end values known, solve its interior rows 1..MM-2 by
-// Thomas (InverseTridiagonal, src/m_Tools.f90:224-251); f receives the chunk.
-template <int MM, typename V, typename G>
-__device__ __forceinline__ void chunk_interior(V (&f)[MM], int s, G &getc, V xs, V xe) {
-  double cp[MM];
-  double aj, bj, cj;
-  getc(s + 2, aj, bj, cj);
-  double iv = 1.0 / bj;
-  cp[1] = cj * iv;
-  f[1] = vscale(iv, vsub(f[1], vscale(aj, xs)));
-#pragma unroll
-  for (int r = 2; r <= MM - 2; ++r) {
-    getc(s + r + 1, aj, bj, cj);
-    V fj = f[r];
-    if (r == MM - 2) fj = vsub(fj, vscale(cj, xe));
-    iv = 1.0 / (bj - aj * cp[r - 1]);
-    cp[r] = cj * iv;
-    f[r] = vscale(iv, vsub(fj, vscale(aj, f[r - 1])));
-  }
-#pragma unroll
-  for (int r = MM - 3; r >= 1; --r) f[r] = vsub(f[r], vscale(cp[r], f[r + 1]));
-  f[0] = xs;
+// With x_0 = xs and x_last = xe known: x_r = F_r + A_r x_0 - C_r x_{r+1}
+// (no divisions); f receives the chunk.
+template <int MM, typename V>
+__device__ __forceinline__ void chunk_backsub(V (&f)[MM], const double (&A)[MM], const double (&C)[MM], V xs,
+                                              V xe) {
   f[MM - 1] = xe;
+#pragma unroll
+  for (int r = MM - 2; r >= 1; --r) f[r] = vsub(vsub(f[r], vscale(-A[r], xs)), vscale(C[r], f[r + 1]));
+  f[0] = xs;
 }
 
 // Reduced system of 2P rows per line (row 2p: first row of chunk p, row 2p+1:
 // its last row), unit diagonal, solved by parallel cyclic reduction: at step d
 // every row eliminates its neighbours q-d and q+d, so after ceil(log2(2P))
-// steps each row stands alone.  Rows are renormalised to a unit diagonal after
-// every step; rows outside the system act as decoupled identity rows.  Each
-// thread owns the two rows of its chunk and keeps them in registers; LDS
-// ([2P][LPB] for a, c, f) only publishes them between steps.  The reference
+// steps each row stands alone.  Rows outside the system act as decoupled
+// identity rows.  The rows live in LDS, double-buffered ([2][2P][LPB] for
+// each of a, c, f): a step reads one buffer and writes the other, so it needs
+// one barrier and holds nothing in registers across steps.  The reference
 // solves the same reduced system by Thomas (InverseTridiagonal_reduce,
-// src/m_Poisson.f90:1045-1071); cyclic reduction lets every chunk's thread
-// take part instead of one wavefront.
+// src/m_Poisson.f90:1045-1071); cyclic reduction lets every chunk's lane take
+// part instead of one wavefront.
 template <int LPB, typename V>
 __device__ __forceinline__ void pcr_row(int q, int d, int n2, int lane, const double *RA, const double *RC,
-                                        const V *RF, double &a, double &c, V &f) {
+                                        const V *RF, double *WA, double *WC, V *WF) {
+  const int o = q * LPB + lane;
+  const double a = RA[o], c = RC[o];
+  V f = RF[o];
   double am = 0.0, cm = 0.0, ap = 0.0, cp = 0.0;
   V fm = vzero(f), fp = vzero(f);
   if (q - d >= 0) {
-    am = RA[(q - d) * LPB + lane];
-    cm = RC[(q - d) * LPB + lane];
-    fm = RF[(q - d) * LPB + lane];
+    am = RA[o - d * LPB];
+    cm = RC[o - d * LPB];
+    fm = RF[o - d * LPB];
   }
   if (q + d < n2) {
-    ap = RA[(q + d) * LPB + lane];
-    cp = RC[(q + d) * LPB + lane];
-    fp = RF[(q + d) * LPB + lane];
+    ap = RA[o + d * LPB];
+    cp = RC[o + d * LPB];
+    fp = RF[o + d * LPB];
   }
   // a x_{q-d} + x_q + c x_{q+d} = f, with rows q-d and q+d substituted
   const double inv = 1.0 / (1.0 - a * cm - c * ap);
-  f = vscale(inv, vsub(vsub(f, vscale(a, fm)), vscale(c, fp)));
-  a = -inv * a * am;
-  c = -inv * c * cp;
+  WF[o] = vscale(inv, vsub(vsub(f, vscale(a, fm)), vscale(c, fp)));
+  WA[o] = -inv * a * am;
+  WC[o] = -inv * c * cp;
 }
 
 template <int LPB, typename V>
 __device__ __forceinline__ void solve_reduced(double aS, double cS, V fS, double aE, double cE, V fE, int P, int p,
                                               int lane, double *RA, double *RC, V *RF, V &xs, V &xe) {
   const int n2 = 2 * P, q0 = 2 * p, q1 = 2 * p + 1;
+  const int half = n2 * LPB;  // second buffer
+  RA[q0 * LPB + lane] = aS;
+  RC[q0 * LPB + lane] = cS;
+  RF[q0 * LPB + lane] = fS;
+  RA[q1 * LPB + lane] = aE;
+  RC[q1 * LPB + lane] = cE;
+  RF[q1 * LPB + lane] = fE;
+  int in = 0;
   for (int d = 1; d < n2; d *= 2) {
-    RA[q0 * LPB + lane] = aS;
-    RC[q0 * LPB + lane] = cS;
-    RF[q0 * LPB + lane] = fS;
-    RA[q1 * LPB + lane] = aE;
-    RC[q1 * LPB + lane] = cE;
-    RF[q1 * LPB + lane] = fE;
     __syncthreads();
-    pcr_row<LPB>(q0, d, n2, lane, RA, RC, RF, aS, cS, fS);
-    pcr_row<LPB>(q1, d, n2, lane, RA, RC, RF, aE, cE, fE);
-    __syncthreads();
+    const int out = half - in;
+    pcr_row<LPB>(q0, d, n2, lane, RA + in, RC + in, RF + in, RA + out, RC + out, RF + out);
+    pcr_row<LPB>(q1, d, n2, lane, RA + in, RC + in, RF + in, RA + out, RC + out, RF + out);
+    in = out;
   }
-  xs = fS;
-  xe = fE;
+  // each thread reads back only the rows it wrote itself
+  xs = RF[in + q0 * LPB + lane];
+  xe = RF[in + q1 * LPB + lane];
 }
 
 // chunk index of a thread; with 64 lines per block a chunk is one wavefront
@@ -236,8 +231,8 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict_
                                                            int P) {
   extern __shared__ double smem[];
   double *RA = smem;
-  double *RC = RA + 2 * P * LPB;
-  double *RF = RC + 2 * P * LPB;
+  double *RC = RA + 4 * P * LPB;
+  double *RF = RC + 4 * P * LPB;
   const int lane = threadIdx.x % LPB;
   const int p = chunk_of_thread<LPB>();
   const int kx = blockIdx.x * (LPB / 2) + lane / 2, kz = blockIdx.y;
@@ -245,21 +240,23 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict_
   const int s = p * MM;
   const double lam = valid ? wnx[kx] + wnz[kz] : 1.0;
   const bool pin = (kx == 0 && kz == 0);
-  double *col = reinterpret_cast<double *>(spec + (long long)kz * ny * pitch) + 2 * kx + (lane & 1);
+  // row base addresses are wave-uniform (scalar), the lane adds a 32-bit offset
+  double *base = reinterpret_cast<double *>(spec + (long long)kz * ny * pitch) + (long long)blockIdx.x * LPB;
   const long long rs = 2LL * pitch;
   double f[MM];
 #pragma unroll
-  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? col[(s + r) * rs] : 0.0;
+  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? base[(s + r) * rs + lane] : 0.0;
   if (pin && s == 0) f[0] = 0.0;  // phi(1) = 0 on the (0,0) line
   PoissonRowsRe getc{met + M_AM2PH * nyh, met + M_AC2PH * nyh, met + M_AP2PH * nyh, lam, ny, pin};
+  double A[MM], C[MM];
   double aS, cS, aE, cE, fS, fE, xs, xe;
-  chunk_reduced_rows<MM>(f, s, getc, aS, cS, fS, aE, cE, fE);
+  chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
   solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
-  chunk_interior<MM>(f, s, getc, xs, xe);
+  chunk_backsub<MM>(f, A, C, xs, xe);
   if (valid) {
 #pragma unroll
     for (int r = 0; r < MM; ++r)
-      if (s + r < ny) col[(s + r) * rs] = f[r];
+      if (s + r < ny) base[(s + r) * rs + lane] = f[r];
   }
 }
 
@@ -316,8 +313,8 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *
                                                      int nx, int ny, int P) {
   extern __shared__ double smem[];
   double *RA = smem;
-  double *RC = RA + 2 * P * LPB;
-  double *RF = RC + 2 * P * LPB;
+  double *RC = RA + 4 * P * LPB;
+  double *RF = RC + 4 * P * LPB;
   const int lane = threadIdx.x % LPB;
   const int p = chunk_of_thread<LPB>();
   const int i = blockIdx.x * LPB + lane, k = blockIdx.y;
@@ -330,10 +327,11 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *
   for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? rhs[col + (long long)nx * (s + r + 1)] + sh : 0.0;
   if (kind == 1 && s == 0) f[0] = valid ? duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
   CnRows getc{met + rowA * nyh, met + rowB * nyh, met + rowC * nyh, beta, ny, kind};
+  double A[MM], C[MM];
   double aS, cS, aE, cE, fS, fE, xs, xe;
-  chunk_reduced_rows<MM>(f, s, getc, aS, cS, fS, aE, cE, fE);
+  chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
   solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
-  chunk_interior<MM>(f, s, getc, xs, xe);
+  chunk_backsub<MM>(f, A, C, xs, xe);
   if (valid) {
 #pragma unroll
     for (int r = 0; r < MM; ++r)
@@ -366,7 +364,7 @@ static int tri_choose(int ny, size_t lds_limit, int mm_req, const TriShape *shap
       if (lpb * P > shapes[q].cap) continue;
       if (2 * P * lpb > 1 << 16) continue;
       if (pass == 0 && (P * mm - ny) * 8 > ny) continue;  // prefer little padding
-      const size_t lds = (size_t)2 * P * lpb * 24;  // a, c, f of the reduced rows
+      const size_t lds = (size_t)4 * P * lpb * 24;  // a, c, f of the reduced rows, two buffers
       if (lds > lds_limit) continue;
       *P_out = P;
       *mm_out = mm;
@@ -396,7 +394,7 @@ int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
   const int nP = P.tri_chunks, lpb = P.tri_lpb;
   // lpb lanes per block = lpb/2 complex lines
   dim3 grid((unsigned)((2 * (P.h + 1) + lpb - 1) / lpb), (unsigned)c->nz);
-  const size_t lds = (size_t)2 * nP * lpb * 24;
+  const size_t lds = (size_t)4 * nP * lpb * 24;
 #define PTC_CASE(MM, LPB)                                                                                    \
   if (P.tri_mmax == MM && lpb == LPB) {                                                                      \
     hipLaunchKernelGGL((k_tri_poisson_split<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, spec, c->d_met,  \
@@ -423,7 +421,7 @@ int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const doub
   else if (kind == 1) { rA = M_AM2P; rB = M_AC2P; rC = M_AP2P; }
   else { rA = M_AM2CSCCN; rB = M_AC2CSCCN; rC = M_AP2CSCCN; }
   dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz);
-  const size_t lds = (size_t)2 * nP * lpb * 24;
+  const size_t lds = (size_t)4 * nP * lpb * 24;
 #define PTC_CN(MM, LPB)                                                                                       \
   if (mmax == MM && lpb == LPB) {                                                                             \
     hipLaunchKernelGGL((k_tri_cn_part<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, u, rhs, duy_ym, shift,  \

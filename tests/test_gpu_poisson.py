@@ -78,6 +78,30 @@ def test_partitioned_chunk_lengths_match_oracle(ny, rows):
     assert res <= 1e-13 * np.abs(out).max()
 
 
+@pytest.mark.parametrize("slab", [1, 5, 8, 24])
+def test_slabbed_xz_passes_match_oracle(slab):
+    """x/z passes run slab by slab of y planes (ny = 24: slabs of 5 leave a
+    ragged last slab; 24 means unblocked)."""
+    case = CS.case_poisson(64, 24, 32, 1.8)
+    f = ic.poisson_rhs(case, seed=9)
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
+        s.poisson_slab_planes(slab)
+        cfg = s.poisson_get_config()
+        assert cfg["slab_planes"] == (slab if slab < 24 else 0), cfg
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        out = s.download(capi.F_HALO, capi.L_INTERIOR)
+        s.poisson_slab_planes(0)
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        whole = s.download(capi.F_HALO, capi.L_INTERIOR)
+    assert np.array_equal(out, whole)
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    assert rel_err(out, inner(ref)) <= TOL
+
+
 def test_automatic_variant_falls_back_to_thomas():
     """ny beyond every partitioned shape: AUTO runs Thomas, PARTITIONED refuses."""
     case = CS.case_poisson(16, 272, 8, 2.0)
@@ -141,10 +165,11 @@ def test_poisson_deterministic():
     assert np.array_equal(a, b)
 
 
-# lengths with compile-time specialised FFT kernels: x/2 in {8,16,32,48,64,96,128,256,512,768},
-# z in {16,32,64,96,128,192,256,512}
+# lengths with compile-time specialised FFT kernels: x/2 in {8,16,32,48,64,96,128,256,512,768,
+# 2304,4608}, z in {16,32,64,96,128,192,256,512,700,1400}
 FAST_GRIDS = [(64, 16, 32), (128, 8, 64), (256, 12, 96), (512, 8, 128), (1024, 8, 192),
-              (1536, 8, 256), (96, 8, 512), (16, 20, 16), (192, 24, 128)]
+              (1536, 8, 256), (96, 8, 512), (16, 20, 16), (192, 24, 128),
+              (4608, 8, 16), (9216, 6, 8), (16, 8, 700), (32, 6, 1400)]
 
 
 @pytest.mark.parametrize("grid", FAST_GRIDS, ids=lambda g: "x".join(map(str, g)))
@@ -158,11 +183,17 @@ def test_specialised_fft_kernels_match_generic_and_oracle(grid):
         cfg = s.poisson_get_config()
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         fast = s.download(capi.F_HALO, capi.L_INTERIOR)
-        s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 4, 0)
-        s.clcPPE(capi.F_ARR1, capi.F_HALO)
-        generic = s.download(capi.F_HALO, capi.L_INTERIOR)
+        try:  # the generic kernels hold a whole x line in at most 64 KB of LDS
+            s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 4, 0)
+        except capi.PtcError as e:
+            assert nx // 2 * 16 > 65536 and "too large" in str(e), e
+            generic = None
+        else:
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+            generic = s.download(capi.F_HALO, capi.L_INTERIOR)
     assert cfg["x_fast"] and cfg["z_fast"], cfg
-    assert rel_err(fast, generic) <= 1e-13
+    if generic is not None:
+        assert rel_err(fast, generic) <= 1e-13
     orc = Oracle(case)
     ref = orc.halo()
     orc.clcPPE(f.copy(order="F"), ref, 1)
