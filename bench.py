@@ -17,7 +17,10 @@ The JSON line also carries
   * cpu_baseline: the CPU oracle (a port of the reference's clcPPE_x1) on the
     same right-hand side, timed on this host's cores over a bounded sample;
   * rk_substep: grid points per second of one RK3 substep of the 128 x 96 x 128
-    closed channel (ChannelIterate on the device, fixed dt = 2e-3).
+    closed channel (ChannelIterate on the device, fixed dt = 2e-3);
+  * ensemble: 64 closed-channel cases of uneven grids served together under
+    Poisson arrivals (20/s, 500 steps each): throughput, p50/p99 per-step
+    latency, and the time the same cases take one at a time.
 """
 from __future__ import annotations
 
@@ -147,6 +150,49 @@ def bench_poisson(args, ev):
     return out
 
 
+def bench_fp32_refine(r, ev, steps=20, max_sweeps=4):
+    """FP32-storage solve + FP64 defect correction on the same resident
+    right-hand side: the fewest sweeps whose field error against the FP64
+    solution is <= 1e-9 (relative max), its time per solve and the ratio."""
+    from paratc_amd import capi
+
+    s = r["solver"]
+    s.clcPPE(capi.F_ARR1, capi.F_HALO)
+    p64 = s.download(capi.F_HALO, capi.L_INTERIOR)
+    scale = float(np.abs(p64).max())
+    plan64 = s.poisson_get_config()
+    chosen = None
+    for sweeps in range(max_sweeps + 1):
+        s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, sweeps)
+        if plan64["slab_planes"]:
+            s.poisson_slab_planes(plan64["slab_planes"])
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        err = float(np.abs(s.download(capi.F_HALO, capi.L_INTERIOR) - p64).max()) / scale
+        progress(f"fp32+refine, {sweeps} sweeps: field error {err:.2e} vs FP64")
+        if err <= 1e-9:
+            chosen = (sweeps, err)
+            break
+    if chosen is None:
+        return {"error": "no sweep count reached 1e-9", "last_error": err}
+    stream = s.stream()
+    s.clcPPE(capi.F_ARR1, capi.F_HALO)
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for _ in range(steps):
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1) / steps
+    res = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+    # back to the FP64 plan
+    s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED if plan64["tri"] == "partitioned" else capi.TRI_THOMAS,
+                     plan64["z_tile"] if not (plan64["x_fast"] and plan64["z_fast"]) else 0, 0)
+    return {"sweeps": chosen[0], "field_error_vs_fp64": chosen[1], "ms_per_solve": round(ms, 4),
+            "speed_vs_fp64": round(r["ms_per"] / ms, 3), "residual_max": res,
+            # first solve 48 B/pt (FP64 in, FP32 spectra, FP64 out); a sweep:
+            # residual 24 + correction 56 (its x-inverse adds into phi)
+            "bytes_per_point": 48 + 80 * chosen[0]}
+
+
 def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
     from oracle.oracle import Oracle
 
@@ -207,6 +253,43 @@ def bench_rk_substep(ev, steps=30, warmup=3):
     }
 
 
+def bench_ensemble(steps=500, alone_steps=20, budget_gib=64):
+    """The 64-case closed-channel ensemble (seed 3) served under its arrival
+    process, against the same cases run alone one after another."""
+    from paratc_amd import ensemble as E
+
+    wl = E.c3_workload()
+    t = time.time()
+    solvers = [E.make_case(c, 100 + q) for q, c in enumerate(wl.cases)]
+    progress(f"ensemble: built {len(solvers)} cases in {time.time() - t:.1f} s")
+    # alone: every case in a one-case ensemble (graph replays, no overlap)
+    alone = []
+    for q, s in enumerate(solvers):
+        with E.Ensemble(budget_gib << 30) as ens:
+            cid = ens.admit(s)
+            ens.step(1)
+            t = time.perf_counter()
+            ens.step(alone_steps)
+            alone.append((time.perf_counter() - t) / alone_steps)
+            solvers[q] = ens.release(cid, keep=True)
+    seq_s = steps * float(np.sum(alone))
+    progress(f"ensemble: one case at a time would take {seq_s:.1f} s for {steps} steps each")
+    r = E.serve(wl, steps, budget_gib << 30, solvers)
+    progress(f"ensemble: served in {r['wall_s']:.1f} s, p50 {r['p50_ms']:.3f} ms, p99 {r['p99_ms']:.3f} ms")
+    return {
+        "workload": "64 closed-channel cases (seed 3), nx 64-256, ny 48-128, nz 64-192, half with a settling "
+                    "scalar, Poisson arrivals 20/s, 500 steps each, dt 2e-3, FP64",
+        "wall_s": round(r["wall_s"], 3),
+        "case_steps_per_s": round(r["case_steps_per_s"], 1),
+        "mpts_steps_per_s": round(r["mpts_steps_per_s"], 1),
+        "step_latency_p50_ms": round(r["p50_ms"], 4),
+        "step_latency_p99_ms": round(r["p99_ms"], 4),
+        "one_at_a_time_s": round(seq_s, 3),
+        "speedup_vs_one_at_a_time": round(seq_s / r["wall_s"], 3),
+        "admission_wait_p99_s": round(r["admit_wait_p99_s"], 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +297,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rk", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -224,8 +308,10 @@ def main():
     ev = HipEvents()
     r = bench_poisson(args, ev)
     case = r["case"]
+    fp32 = bench_fp32_refine(r, ev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
+    ens = None if args.no_ensemble else bench_ensemble()
     n = case.npts
     kernel = stage_kernels(r["plan"])[r["dom"]]
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
@@ -260,8 +346,10 @@ def main():
                         ["xz_fwd", "-", "y_solve", "-", "zx_inv"] if r["plan"]["slab_planes"] else STAGES,
                         r["per_stage"]) if k != "-"},
                     "residual_max": r["residual"],
-                    "plan": r["plan"]},
+                    "plan": r["plan"],
+                    "fp32_refine": fp32},
         "rk_substep": rk,
+        "ensemble": ens,
     }
     print(json.dumps(line), flush=True)
 

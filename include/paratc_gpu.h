@@ -22,6 +22,8 @@
 #ifndef PARATC_GPU_H
 #define PARATC_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -222,6 +224,36 @@ int ptc_clc_mean_value(ptc_ctx *ctx, int ux, int scalar, double *res3);
  *      reference's div_limit / vel_limit guard (:142-145) would abort. */
 int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7);
 int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
+
+/* ---- Ensembles: many independent cases on one GPU.  No counterpart in the
+ *      reference, which runs one case per job; this is the serving layer
+ *      around ChannelIterate (src/m_ChannelSystem.f90:80-145).
+ *
+ *  A case is a context the caller has created, filled (ptc_upload, boundary
+ *  fills) and given a Poisson plan (ptc_init_poisson_solver or
+ *  ptc_poisson_config).  ptc_ensemble_admit takes ownership when the case's
+ *  device footprint fits the ensemble's HBM budget, else returns PTC_E_LIMIT
+ *  and the caller keeps the context (admission control: queue it and retry).
+ *  With a fixed time step (iCFL != 1) the case's step is captured once in a
+ *  HIP graph and each later step is one graph launch on the case's stream;
+ *  CFL-adaptive cases are stepped with ordinary launches.
+ *  ptc_ensemble_step advances every admitted case by nsteps steps (rounds of
+ *  one step per case, cases overlapping on their streams) and returns when
+ *  all are done.  Each case-step's latency is measured from the start of its
+ *  round; ptc_ensemble_latency gives the 50th and 99th percentiles (ms).
+ *  ptc_ensemble_release removes a case and hands its context back (ctx !=
+ *  NULL) or destroys it.  One host thread per ensemble. */
+typedef struct ptc_ensemble ptc_ensemble;
+int ptc_ensemble_create(size_t hbm_budget_bytes, ptc_ensemble **out);
+int ptc_ensemble_destroy(ptc_ensemble *e);
+/* device bytes a case holds once everything its step touches is allocated */
+int ptc_case_footprint(ptc_ctx *ctx, size_t *bytes);
+int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id);
+int ptc_ensemble_step(ptc_ensemble *e, int nsteps);
+int ptc_ensemble_case(ptc_ensemble *e, int case_id, ptc_ctx **ctx, long long *steps);
+int ptc_ensemble_release(ptc_ensemble *e, int case_id, ptc_ctx **ctx);
+int ptc_ensemble_usage(ptc_ensemble *e, size_t *used_bytes, size_t *budget_bytes, int *live_cases);
+int ptc_ensemble_latency(ptc_ensemble *e, int reset, double *p50_ms, double *p99_ms, long long *nsamples);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,9 @@ EXPORTS = [
     "ptc_set_bc_and_update_halo_pr", "ptc_set_bc_and_update_halo_scalar", "ptc_update_uy_ym",
     "ptc_calc_max_cfl", "ptc_check_divergence", "ptc_calc_vmax", "ptc_calc_ux_aver",
     "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar",
+    "ptc_ensemble_create", "ptc_ensemble_destroy", "ptc_case_footprint", "ptc_ensemble_admit",
+    "ptc_ensemble_step", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
+    "ptc_ensemble_latency",
 ]
 
 
@@ -124,6 +127,16 @@ def load(path: str = LIB_PATH):
     L.ptc_clc_mean_value.argtypes = [vp, ip, ip, dp]
     L.ptc_channel_iterate.argtypes = [vp, ip, dp]
     L.ptc_get_scalar.argtypes = [vp, ip, dp]
+    sz, ll = C.c_size_t, C.c_longlong
+    L.ptc_ensemble_create.argtypes = [sz, C.POINTER(vp)]
+    L.ptc_ensemble_destroy.argtypes = [vp]
+    L.ptc_case_footprint.argtypes = [vp, C.POINTER(sz)]
+    L.ptc_ensemble_admit.argtypes = [vp, vp, C.POINTER(ip)]
+    L.ptc_ensemble_step.argtypes = [vp, ip]
+    L.ptc_ensemble_case.argtypes = [vp, ip, C.POINTER(vp), C.POINTER(ll)]
+    L.ptc_ensemble_release.argtypes = [vp, ip, C.POINTER(vp)]
+    L.ptc_ensemble_usage.argtypes = [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(ip)]
+    L.ptc_ensemble_latency.argtypes = [vp, ip, dp, dp, C.POINTER(ll)]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype if name == "ptc_last_error" else C.c_int
     L.ptc_last_error.restype = C.c_char_p

@@ -132,7 +132,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
 }
 
 // ---- one time step (ChannelIterate) ---------------------------------------
-static int channel_iterate(ptc_ctx *c, double *mon) {
+int channel_step(ptc_ctx *c, double *mon) {
   double *ux, *uy, *uz, *p, *hx, *hy, *hz, *a1, *a2, *hl, *uym, *duym;
   double *sc = nullptr, *hc = nullptr, *ac = nullptr;
   PTC_TRY(field(c, PTC_F_UX, &ux));
@@ -189,6 +189,19 @@ static int channel_iterate(ptc_ctx *c, double *mon) {
     mon[5] = cflmp;
     mon[6] = dt;
   }
+  return PTC_OK;
+}
+
+int prepare_step(ptc_ctx *c) {
+  double *d = nullptr;
+  for (int f : {PTC_F_UX, PTC_F_UY, PTC_F_UZ, PTC_F_PR, PTC_F_HISTX, PTC_F_HISTY, PTC_F_HISTZ, PTC_F_ARR1, PTC_F_ARR2,
+                PTC_F_HALO, PTC_F_UY_YM, PTC_F_DUY_YM})
+    PTC_TRY(field(c, f, &d));
+  if (c->prm.scalar)
+    for (int f : {PTC_F_SCALAR, PTC_F_HISTC, PTC_F_ARRC}) PTC_TRY(field(c, f, &d));
+  PTC_TRY(stencil_prepare(c));
+  PTC_TRY(poisson_prepare(c));
+  PTC_HIP(hipStreamSynchronize(c->stream));
   return PTC_OK;
 }
 
@@ -310,7 +323,6 @@ int ptc_clc_ppe(ptc_ctx *ctx, int src, int dst) {
   PTC_TRY(field(ctx, src, &s));
   PTC_TRY(field(ctx, dst, &d));
   PTC_CHECK(src != dst, PTC_E_ARG, "clcPPE: src and dst must differ");
-  PTC_CHECK(ctx->pp.prec == PTC_PREC_FP64, PTC_E_UNSUPPORTED, "FP32+refinement Poisson is not built yet");
   return poisson_solve(ctx, s, d);
 }
 
@@ -589,7 +601,7 @@ int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7) {
   PTC_CTX(ctx);
   PTC_CHECK(nsteps >= 1, PTC_E_ARG, "nsteps must be >= 1");
   double mon[7];
-  for (int s = 0; s < nsteps; ++s) PTC_TRY(channel_iterate(ctx, (s == nsteps - 1) ? mon : nullptr));
+  for (int s = 0; s < nsteps; ++s) PTC_TRY(channel_step(ctx, (s == nsteps - 1) ? mon : nullptr));
   if (mon7) std::copy(mon, mon + 7, mon7);
   // src/m_ChannelSystem.f90:142-145
   if (ctx->prm.div_limit > 0 && mon[1] > ctx->prm.div_limit) {

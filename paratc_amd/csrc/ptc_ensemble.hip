@@ -1,0 +1,261 @@
+// ptc_ensemble.hip -- serving many independent cases on one GPU.
+//
+// The reference advances one case per job (src/m_ChannelSystem.f90:80-145).
+// An ensemble here holds any number of admitted cases, each a full context
+// (its own fields, Poisson plan and HIP stream), and advances them together:
+//
+//  * admission by HBM footprint: a case is admitted when the device bytes its
+//    context holds fit the ensemble's budget next to the cases already in it;
+//    otherwise PTC_E_LIMIT tells the caller to queue it and try again later;
+//  * one launch per case per step: with a fixed time step (iCFL != 1) the
+//    step's kernel sequence is captured once into a HIP graph and replayed;
+//    cases with CFL-adaptive dt (host-side dt from a device reduction) are
+//    stepped with ordinary launches;
+//  * cases run on their own streams, so small grids that cannot fill the
+//    chip on their own overlap each other;
+//  * every case-step records a completion event; its latency is measured
+//    from the start of the round that issued it (p50 / p99 on request).
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "ptc_internal.hpp"
+
+struct ptc_ensemble {
+  struct Case {
+    ptc_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t bytes = 0;
+    long long steps = 0;
+    bool live = false;
+  };
+  size_t budget = 0, used = 0;
+  std::vector<Case> cases;
+  hipStream_t side = nullptr;   // round-start events
+  std::vector<hipEvent_t> pool;  // reusable events
+  std::vector<double> lat_ms;    // per case-step latencies since the last reset
+};
+
+namespace ptc {
+
+// device bytes a context holds (fields, reduction partials, Poisson plan)
+static size_t ctx_bytes(const ptc_ctx *c) {
+  size_t b = 0;
+  for (int f = 0; f < PTC_NFIELDS; ++f)
+    if (c->d_field[f]) b += sizeof(double) * ((f == PTC_F_UY_YM || f == PTC_F_DUY_YM) ? (size_t)c->nx * c->nz : c->n3);
+  b += sizeof(double) * (size_t)c->red_cap;
+  if (c->pp.d_spec) b += sizeof(double2) * c->pp.spec_elems;
+  if (c->pp.d_tri_scratch) b += sizeof(double) * c->pp.spec_elems;
+  return b;
+}
+
+static int take_event(ptc_ensemble *e, hipEvent_t *ev) {
+  if (!e->pool.empty()) {
+    *ev = e->pool.back();
+    e->pool.pop_back();
+    return PTC_OK;
+  }
+  PTC_HIP(hipEventCreate(ev));
+  return PTC_OK;
+}
+
+// Capture one step.  Host-side coefficient state that the capture advances
+// (substep coefficients, SimTime) is put back afterwards; each replay then
+// stands for exactly one step and SimTime is advanced by dt per replay.
+static int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec) {
+  const double save[9] = {c->pmGamma, c->pmTheta, c->pmAlpha, c->pmAlphaC, c->pmBeta,
+                          c->pmBetaSc, c->dt, c->simtime, c->prgradaver};
+  PTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = channel_step(c, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  c->pmGamma = save[0];
+  c->pmTheta = save[1];
+  c->pmAlpha = save[2];
+  c->pmAlphaC = save[3];
+  c->pmBeta = save[4];
+  c->pmBetaSc = save[5];
+  c->dt = save[6];
+  c->simtime = save[7];
+  c->prgradaver = save[8];
+  if (rc != PTC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  PTC_HIP(ec);
+  const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    PTC_HIP(ei);
+  }
+  *graph = g;
+  return PTC_OK;
+}
+
+static int step_case(ptc_ensemble::Case &k) {
+  if (k.exec) {
+    PTC_HIP(hipGraphLaunch(k.exec, k.ctx->stream));
+    k.ctx->simtime += k.ctx->prm.dtmax;
+  } else {
+    PTC_TRY(channel_step(k.ctx, nullptr));
+  }
+  ++k.steps;
+  return PTC_OK;
+}
+
+}  // namespace ptc
+
+using namespace ptc;
+
+extern "C" {
+
+int ptc_ensemble_create(size_t hbm_budget, ptc_ensemble **out) {
+  PTC_CHECK(out != nullptr, PTC_E_ARG, "null output");
+  *out = nullptr;
+  auto *e = new ptc_ensemble();
+  e->budget = hbm_budget;
+  if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    set_error("hipStreamCreate failed");
+    return PTC_E_HIP;
+  }
+  *out = e;
+  return PTC_OK;
+}
+
+int ptc_ensemble_destroy(ptc_ensemble *e) {
+  if (!e) return PTC_OK;
+  for (auto &k : e->cases) {
+    if (k.ctx) (void)hipStreamSynchronize(k.ctx->stream);
+    if (k.exec) (void)hipGraphExecDestroy(k.exec);
+    if (k.graph) (void)hipGraphDestroy(k.graph);
+    if (k.ctx) ptc_destroy(k.ctx);
+  }
+  for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
+  if (e->side) (void)hipStreamDestroy(e->side);
+  delete e;
+  return PTC_OK;
+}
+
+int ptc_case_footprint(ptc_ctx *ctx, size_t *bytes) {
+  PTC_CHECK(ctx && bytes, PTC_E_ARG, "null argument");
+  PTC_TRY(prepare_step(ctx));
+  *bytes = ctx_bytes(ctx);
+  return PTC_OK;
+}
+
+int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id) {
+  PTC_CHECK(e && ctx && case_id, PTC_E_ARG, "null argument");
+  for (const auto &k : e->cases)
+    PTC_CHECK(!(k.live && k.ctx == ctx), PTC_E_ARG, "context already admitted");
+  PTC_TRY(prepare_step(ctx));
+  const size_t b = ctx_bytes(ctx);
+  if (e->used + b > e->budget) {
+    set_error("HBM budget: case needs " + std::to_string(b) + " B, " + std::to_string(e->budget - e->used) +
+              " B free");
+    return PTC_E_LIMIT;
+  }
+  ptc_ensemble::Case k;
+  k.ctx = ctx;
+  k.bytes = b;
+  k.live = true;
+  if (ctx->prm.icfl != 1) PTC_TRY(capture_step(ctx, &k.graph, &k.exec));
+  e->used += b;
+  int id = -1;
+  for (size_t q = 0; q < e->cases.size(); ++q)
+    if (!e->cases[q].live && !e->cases[q].ctx) {
+      id = (int)q;
+      break;
+    }
+  if (id < 0) {
+    id = (int)e->cases.size();
+    e->cases.push_back(k);
+  } else {
+    e->cases[id] = k;
+  }
+  *case_id = id;
+  return PTC_OK;
+}
+
+int ptc_ensemble_step(ptc_ensemble *e, int nsteps) {
+  PTC_CHECK(e, PTC_E_ARG, "null ensemble");
+  PTC_CHECK(nsteps >= 1, PTC_E_ARG, "nsteps must be >= 1");
+  for (int s = 0; s < nsteps; ++s) {
+    hipEvent_t start;
+    PTC_TRY(take_event(e, &start));
+    PTC_HIP(hipEventRecord(start, e->side));
+    std::vector<hipEvent_t> done;
+    done.reserve(e->cases.size());
+    for (auto &k : e->cases) {
+      if (!k.live) continue;
+      PTC_TRY(step_case(k));
+      hipEvent_t ev;
+      PTC_TRY(take_event(e, &ev));
+      PTC_HIP(hipEventRecord(ev, k.ctx->stream));
+      done.push_back(ev);
+    }
+    for (hipEvent_t ev : done) {
+      PTC_HIP(hipEventSynchronize(ev));
+      float ms = 0.f;
+      PTC_HIP(hipEventElapsedTime(&ms, start, ev));
+      e->lat_ms.push_back(ms);
+      e->pool.push_back(ev);
+    }
+    e->pool.push_back(start);
+  }
+  return PTC_OK;
+}
+
+int ptc_ensemble_case(ptc_ensemble *e, int case_id, ptc_ctx **ctx, long long *steps) {
+  PTC_CHECK(e && case_id >= 0 && case_id < (int)e->cases.size() && e->cases[case_id].live, PTC_E_ARG,
+            "unknown case");
+  if (ctx) *ctx = e->cases[case_id].ctx;
+  if (steps) *steps = e->cases[case_id].steps;
+  return PTC_OK;
+}
+
+int ptc_ensemble_release(ptc_ensemble *e, int case_id, ptc_ctx **ctx) {
+  PTC_CHECK(e && case_id >= 0 && case_id < (int)e->cases.size() && e->cases[case_id].live, PTC_E_ARG,
+            "unknown case");
+  auto &k = e->cases[case_id];
+  PTC_HIP(hipStreamSynchronize(k.ctx->stream));
+  if (k.exec) (void)hipGraphExecDestroy(k.exec);
+  if (k.graph) (void)hipGraphDestroy(k.graph);
+  e->used -= k.bytes;
+  if (ctx) {
+    *ctx = k.ctx;
+  } else {
+    ptc_destroy(k.ctx);
+  }
+  k = ptc_ensemble::Case();
+  return PTC_OK;
+}
+
+int ptc_ensemble_usage(ptc_ensemble *e, size_t *used, size_t *budget, int *live) {
+  PTC_CHECK(e, PTC_E_ARG, "null ensemble");
+  if (used) *used = e->used;
+  if (budget) *budget = e->budget;
+  if (live) *live = (int)std::count_if(e->cases.begin(), e->cases.end(), [](const ptc_ensemble::Case &k) {
+    return k.live;
+  });
+  return PTC_OK;
+}
+
+int ptc_ensemble_latency(ptc_ensemble *e, int reset, double *p50, double *p99, long long *nsamples) {
+  PTC_CHECK(e, PTC_E_ARG, "null ensemble");
+  std::vector<double> v = e->lat_ms;
+  if (nsamples) *nsamples = (long long)v.size();
+  auto pct = [&](double q) -> double {
+    if (v.empty()) return 0.0;
+    const size_t i = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + (long)i, v.end());
+    return v[i];
+  };
+  if (p50) *p50 = pct(0.50);
+  if (p99) *p99 = pct(0.99);
+  if (reset) e->lat_ms.clear();
+  return PTC_OK;
+}
+
+}  // extern "C"

@@ -66,6 +66,13 @@ constexpr int plan_span(int n, int s) {  // product of the radices before stage 
   return v;
 }
 
+// spectra are stored as double2 (FP64 form) or float2 (FP32-storage form);
+// arithmetic is FP64 either way
+__device__ __forceinline__ double2 ld2(const double2 &v) { return v; }
+__device__ __forceinline__ double2 ld2(const float2 &v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void st2(double2 &d, double2 v) { d = v; }
+__device__ __forceinline__ void st2(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
 // LDS padding of one line: one slot per 8 elements
 __device__ __forceinline__ int pad8(int m) { return m + (m >> 3); }
 constexpr int padded(int n) { return n + n / 8 + 1; }
@@ -281,8 +288,8 @@ constexpr int f2_xblock() {
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
-template <int N>
-__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, double2 *__restrict__ spec,
+template <int N, typename S>
+__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
   constexpr int G = f2_threads<N>();
@@ -305,7 +312,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
   __syncthreads();
   if (!live) return;
-  double2 *out = spec + sq * pitch;
+  S *out = spec + sq * pitch;
   for (int kx = g; kx < pitch; kx += G) {
     double2 X = make_double2(0.0, 0.0);
     if (kx <= N) {
@@ -316,13 +323,14 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
       const double2 wd = f2::cmul(wx[kx], d);
       X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
     }
-    out[kx] = X;
+    f2::st2(out[kx], X);
   }
 }
 
 // ---- x inverse: half spectra -> real rows times scale ----------------------
-template <int N>
-__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const double2 *__restrict__ spec, double *__restrict__ dst,
+// ACC: add the result to dst (a correction) instead of storing it
+template <int N, typename S, bool ACC>
+__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
                                                     double scale) {
@@ -336,11 +344,11 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const double2 *__
   const bool live = l < L && q < nlines;
   const long long qq = live ? q : 0;
   const long long jl = j0 + qq % nj, k = qq / nj;
-  const double2 *in = spec + (jl + (long long)ny * k) * pitch;
+  const S *in = spec + (jl + (long long)ny * k) * pitch;
   double2 *ls = sh + (l < L ? l : 0) * PN;
   // The re-tangle of element m needs X_m and X_{N-m}: the line's N+1 values
   // go through LDS once (coalesced) instead of being fetched twice.
-  for (int m = g; m <= N; m += G) ls[f2::pad8(m)] = live ? in[m] : make_double2(0.0, 0.0);
+  for (int m = g; m <= N; m += G) ls[f2::pad8(m)] = live ? f2::ld2(in[m]) : make_double2(0.0, 0.0);
   __syncthreads();
   auto load0 = [&](int m) {
     const double2 xk = ls[f2::pad8(m)];
@@ -355,61 +363,100 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const double2 *__
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   auto store_out = [&](int m, double2 v) {
-    if (live) row[m] = make_double2(v.x * scale, v.y * scale);
+    if (!live) return;
+    if (ACC) {
+      const double2 o = row[m];
+      row[m] = make_double2(o.x + v.x * scale, o.y + v.y * scale);
+    } else {
+      row[m] = make_double2(v.x * scale, v.y * scale);
+    }
   };
   f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
 }
 
 // ---- z pass: T columns x N rows, thread id = g * T + col ----------------
-template <int N, int T, bool INV>
-__global__ void __launch_bounds__(512) k2_fft_z(double2 *__restrict__ spec, const double2 *__restrict__ tw, int ny,
+template <int N, int T, bool INV, typename S>
+__global__ void __launch_bounds__(512) k2_fft_z(S *__restrict__ spec, const double2 *__restrict__ tw, int ny,
                                                  int pitch, int j0) {
   constexpr int G = f2_threads<N>();
   __shared__ double2 sh[N * T];
   const int col = threadIdx.x % T, g = threadIdx.x / T;
   const long long zs = (long long)pitch * ny;
-  double2 *base = spec + (long long)(j0 + blockIdx.y) * pitch + (long long)blockIdx.x * T + col;
-  auto load0 = [&](int m) { return base[m * zs]; };
+  S *base = spec + (long long)(j0 + blockIdx.y) * pitch + (long long)blockIdx.x * T + col;
+  auto load0 = [&](int m) { return f2::ld2(base[m * zs]); };
   auto lds = [&](int m) -> double2 & { return sh[m * T + col]; };
-  auto store_out = [&](int m, double2 v) { base[m * zs] = v; };
+  auto store_out = [&](int m, double2 v) { f2::st2(base[m * zs], v); };
   f2::run_fft<N, G, INV>(g, tw, load0, lds, store_out);
 }
 
 // ---- dispatch ----------------------------------------------------------------
 template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj) {
+static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
-  hipLaunchKernelGGL(k2_fft_x_fwd<N>, dim3(nblk), dim3(L * G), 0, c->stream, src, spec, c->pp.fx.tw, c->pp.d_wx,
-                     c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
+  if (c->pp.prec == PTC_PREC_FP32_REFINE)
+    hipLaunchKernelGGL((k2_fft_x_fwd<N, float2>), dim3(nblk), dim3(L * G), 0, c->stream, src, (float2 *)spec,
+                       c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
+  else
+    hipLaunchKernelGGL((k2_fft_x_fwd<N, double2>), dim3(nblk), dim3(L * G), 0, c->stream, src, (double2 *)spec,
+                       c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
 template <int N>
-static int x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj) {
+static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
-  hipLaunchKernelGGL(k2_fft_x_inv<N>, dim3(nblk), dim3(L * G), 0, c->stream, spec, dst, c->pp.fx.tw, c->pp.d_wx,
-                     c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, scale);
+  const dim3 grid(nblk), blk(L * G);
+  const auto *tw = c->pp.fx.tw;
+  const auto *wx = c->pp.d_wx;
+  const int ny = c->ny, nz = c->nz, pitch = c->pp.pitch;
+  const long long rl = c->nx;
+  if (c->pp.prec == PTC_PREC_FP32_REFINE) {
+    if (acc)
+      hipLaunchKernelGGL((k2_fft_x_inv<N, float2, true>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw, wx,
+                         ny, nz, j0, nj, pitch, rl, scale);
+    else
+      hipLaunchKernelGGL((k2_fft_x_inv<N, float2, false>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw,
+                         wx, ny, nz, j0, nj, pitch, rl, scale);
+  } else {
+    if (acc)
+      hipLaunchKernelGGL((k2_fft_x_inv<N, double2, true>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
+                         wx, ny, nz, j0, nj, pitch, rl, scale);
+    else
+      hipLaunchKernelGGL((k2_fft_x_inv<N, double2, false>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
+                         wx, ny, nz, j0, nj, pitch, rl, scale);
+  }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
 template <int N, int T>
-static int z_t(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj) {
+static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj) {
   constexpr int G = f2_threads<N>();
   dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)nj);
-  if (inverse)
-    hipLaunchKernelGGL((k2_fft_z<N, T, true>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny, c->pp.pitch,
-                       j0);
-  else
-    hipLaunchKernelGGL((k2_fft_z<N, T, false>), grid, dim3(G * T), 0, c->stream, spec, c->pp.fz.tw, c->ny,
-                       c->pp.pitch, j0);
+  const auto *tw = c->pp.fz.tw;
+  const int ny = c->ny, pitch = c->pp.pitch;
+  if (c->pp.prec == PTC_PREC_FP32_REFINE) {
+    if (inverse)
+      hipLaunchKernelGGL((k2_fft_z<N, T, true, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny, pitch,
+                         j0);
+    else
+      hipLaunchKernelGGL((k2_fft_z<N, T, false, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny,
+                         pitch, j0);
+  } else {
+    if (inverse)
+      hipLaunchKernelGGL((k2_fft_z<N, T, true, double2>), grid, dim3(G * T), 0, c->stream, (double2 *)spec, tw, ny,
+                         pitch, j0);
+    else
+      hipLaunchKernelGGL((k2_fft_z<N, T, false, double2>), grid, dim3(G * T), 0, c->stream, (double2 *)spec, tw, ny,
+                         pitch, j0);
+  }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -426,7 +473,7 @@ bool fft2_has_x(int h) {
   return false;
 }
 
-int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj) {
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
   switch (c->pp.h) {
 #define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj);
     PTC_X_SIZES(PTC_CASE)
@@ -435,9 +482,9 @@ int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj) {
   }
 }
 
-int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj) {
+int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale, j0, nj);
+#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale, j0, nj, acc);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;
@@ -456,7 +503,7 @@ bool fft2_has_z(int nz, int *tile) {
   return false;
 }
 
-int fft2_z(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj) {
+int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj) {
   switch (c->nz) {
 #define PTC_CASE(N, T) case N: return z_t<N, T>(c, spec, inverse, j0, nj);
     PTC_Z_SIZES(PTC_CASE)

@@ -102,7 +102,8 @@ struct PoissonPlan {
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
-  double2 *d_spec = nullptr;  // spectral buffer pitch * ny * nz complex
+  void *d_spec = nullptr;     // spectral buffer pitch * ny * nz complex (double2, or float2 for FP32 storage)
+  double *d_res = nullptr;    // FP32-storage form: FP64 residual field of the refinement sweeps
   double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
   size_t spec_elems = 0;
   bool profile = false;
@@ -152,23 +153,31 @@ int check_field(ptc_ctx *c, int id);
 int poisson_build_plan(ptc_ctx *c);
 int poisson_free_plan(ptc_ctx *c);
 int poisson_solve(ptc_ctx *c, const double *src, double *dst);
+int poisson_prepare(ptc_ctx *c);
+int stencil_prepare(ptc_ctx *c);
+// one ChannelIterate step (monitors into mon[7] when mon != NULL)
+int channel_step(ptc_ctx *c, double *mon);
+// allocate everything a step touches, so the step can be captured in a graph
+int prepare_step(ptc_ctx *c);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 
 // FFT launchers (ptc_fft.hip)
-int launch_fft_x_fwd(ptc_ctx *c, const double *src, double2 *spec);
-int launch_fft_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale);
-int launch_fft_z(ptc_ctx *c, double2 *spec, bool inverse);
+int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec);
+int launch_fft_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, bool acc);
+int launch_fft_z(ptc_ctx *c, void *spec, bool inverse);
 int fft_make_len(int n, FftLen *L);
 bool fft2_has_x(int h);
 bool fft2_has_z(int nz, int *tile);
-// specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum
-int fft2_x_fwd(ptc_ctx *c, const double *src, double2 *spec, int j0, int nj);
-int fft2_x_inv(ptc_ctx *c, const double2 *spec, double *dst, double scale, int j0, int nj);
-int fft2_z(ptc_ctx *c, double2 *spec, bool inverse, int j0, int nj);
+// specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum;
+// the spectrum holds double2 or, in the FP32-storage form, float2 values;
+// acc: the x-inverse adds its result to dst
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj);
+int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc);
+int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj);
 
 // tridiagonal launchers (ptc_tridiag.hip)
-int launch_tri_poisson(ptc_ctx *c, double2 *spec);
+int launch_tri_poisson(ptc_ctx *c, void *spec);
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym,
                   double beta, const double *shift);
 

@@ -44,6 +44,7 @@ int poisson_free_plan(ptc_ctx *c) {
   PoissonPlan &P = c->pp;
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
+  if (P.d_res) (void)hipFree(P.d_res);
   if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
   for (auto &e : P.prof_ev)
     if (e) (void)hipEventDestroy(e);
@@ -105,13 +106,19 @@ int poisson_build_plan(ptc_ctx *c) {
   // the device's last-level cache, and likewise on the way back
   P.slab = (P.x_fast && P.z_fast && P.slab_req > 0 && P.slab_req < c->ny) ? P.slab_req : 0;
 
+  if (P.prec == PTC_PREC_FP32_REFINE) {
+    PTC_CHECK(P.x_fast && P.z_fast, PTC_E_UNSUPPORTED,
+              "the FP32-storage Poisson form needs specialised FFT kernels for nx/2 and nz");
+    PTC_CHECK(P.tri_req != PTC_TRI_THOMAS, PTC_E_UNSUPPORTED, "the Thomas y-solve has no FP32-storage form");
+  }
+
   // y pass: partitioned unless Thomas is asked for, or (automatic choice) no
   // partitioned shape fits this ny
   P.tri = P.tri_req == PTC_TRI_THOMAS ? PTC_TRI_THOMAS : PTC_TRI_PARTITIONED;
   if (P.tri == PTC_TRI_PARTITIONED) {
     const int rc = tri_plan(c->ny, c->lds_limit, true, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb);
     if (rc != PTC_OK) {
-      if (P.tri_req == PTC_TRI_PARTITIONED) return rc;
+      if (P.tri_req == PTC_TRI_PARTITIONED || P.prec == PTC_PREC_FP32_REFINE) return rc;
       P.tri = PTC_TRI_THOMAS;
     }
   }
@@ -133,12 +140,21 @@ int poisson_build_plan(ptc_ctx *c) {
   P.d_wx = P.d_tw + tx.size() + tz.size();
 
   P.spec_elems = (size_t)P.pitch * c->ny * c->nz;
-  if (hipMalloc(&P.d_spec, sizeof(double2) * P.spec_elems) != hipSuccess) {
+  const size_t spec_bytes = (P.prec == PTC_PREC_FP32_REFINE ? sizeof(float2) : sizeof(double2)) * P.spec_elems;
+  if (hipMalloc(&P.d_spec, spec_bytes) != hipSuccess) {
     P.d_spec = nullptr;
     set_error("spectral buffer allocation failed");
     return PTC_E_ALLOC;
   }
-  PTC_HIP(hipMemsetAsync(P.d_spec, 0, sizeof(double2) * P.spec_elems, c->stream));
+  PTC_HIP(hipMemsetAsync(P.d_spec, 0, spec_bytes, c->stream));
+  if (P.prec == PTC_PREC_FP32_REFINE && P.sweeps > 0) {
+    if (hipMalloc(&P.d_res, sizeof(double) * c->n3) != hipSuccess) {
+      P.d_res = nullptr;
+      set_error("residual field allocation failed");
+      return PTC_E_ALLOC;
+    }
+    PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
+  }
   P.ready = true;
   return PTC_OK;
 }
@@ -154,21 +170,16 @@ static int ensure_thomas_scratch(ptc_ctx *c) {
   return PTC_OK;
 }
 
-int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
+// plan and scratch of the solve, built ahead of a captured step
+int poisson_prepare(ptc_ctx *c) {
   PTC_TRY(poisson_build_plan(c));
+  return ensure_thomas_scratch(c);
+}
+
+// One pass sequence src -> spectrum -> dst (acc: dst += result); ev (may be
+// NULL) receives six events around the five passes.
+static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, hipEvent_t *ev) {
   PoissonPlan &P = c->pp;
-  PTC_TRY(ensure_thomas_scratch(c));
-  hipEvent_t *ev = nullptr;
-  if (P.profile) {
-    const size_t need = (size_t)(P.prof_used + 1) * 6;
-    while (P.prof_ev.size() < need) {
-      hipEvent_t e;
-      PTC_HIP(hipEventCreate(&e));
-      P.prof_ev.push_back(e);
-    }
-    ev = P.prof_ev.data() + (size_t)P.prof_used * 6;
-    ++P.prof_used;
-  }
   if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
   if (P.slab == 0) {
     PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
@@ -189,16 +200,45 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
   if (P.slab == 0) {
     PTC_TRY(launch_fft_z(c, P.d_spec, true));
     if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
-    PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft));
+    PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft, acc));
   } else {
     if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
     for (int j0 = 0; j0 < c->ny; j0 += P.slab) {
       const int nj = std::min(P.slab, c->ny - j0);
       PTC_TRY(fft2_z(c, P.d_spec, true, j0, nj));
-      PTC_TRY(fft2_x_inv(c, P.d_spec, dst, c->normfft, j0, nj));
+      PTC_TRY(fft2_x_inv(c, P.d_spec, dst, c->normfft, j0, nj, acc));
     }
   }
   if (ev) PTC_HIP(hipEventRecord(ev[5], c->stream));
+  return PTC_OK;
+}
+
+int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r);
+
+int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
+  PTC_TRY(poisson_build_plan(c));
+  PoissonPlan &P = c->pp;
+  PTC_TRY(ensure_thomas_scratch(c));
+  hipEvent_t *ev = nullptr;
+  if (P.profile) {
+    const size_t need = (size_t)(P.prof_used + 1) * 6;
+    while (P.prof_ev.size() < need) {
+      hipEvent_t e;
+      PTC_HIP(hipEventCreate(&e));
+      P.prof_ev.push_back(e);
+    }
+    ev = P.prof_ev.data() + (size_t)P.prof_used * 6;
+    ++P.prof_used;
+  }
+  PTC_TRY(solve_sequence(c, src, dst, false, ev));
+  if (P.prec == PTC_PREC_FP32_REFINE) {
+    // defect correction: phi += L^-1 (f - L phi), the residual in FP64 and
+    // the correction through the same FP32-storage passes
+    for (int s = 0; s < P.sweeps; ++s) {
+      PTC_TRY(launch_residual(c, src, dst, P.d_res));
+      PTC_TRY(solve_sequence(c, P.d_res, dst, true, nullptr));
+    }
+  }
   return PTC_OK;
 }
 
@@ -224,8 +264,15 @@ int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves) {
 // L in x and z is the composed stencil D o G of the 4th-order staggered
 // operators: (1, -54, 783, -1460, 783, -54, 1) / 576 / dx^2, whose symbol is
 // the modified wavenumber of src/m_Poisson.f90:42,46.
-__global__ void __launch_bounds__(256) k_residual_max(const double *__restrict__ f, const double *__restrict__ phi,
-                                                     Geo g, double rdx2, double rdz2, double *__restrict__ part) {
+// r = f - L phi over the interior with the FP64 staggered operator: 4th-order
+// x/z (the symbol of the modified wavenumbers, (1, -54, 783, -1460, 783, -54,
+// 1) / 576 rdx^2) and am2ph / ac2ph / ap2ph in y with the Neumann ghosts
+// phi_0 = phi_1, phi_{ny+1} = phi_ny.  WRITE: store r (field layout); part
+// (may be NULL): per-block max |r|.
+template <bool WRITE>
+__global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, const double *__restrict__ phi,
+                                                 Geo g, double rdx2, double rdz2, double *__restrict__ rout,
+                                                 double *__restrict__ part) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long long total = (long long)nx * ny * nz;
   double mx = 0.0;
@@ -252,9 +299,12 @@ __global__ void __launch_bounds__(256) k_residual_max(const double *__restrict__
     const double ym = j > 1 ? at(i, j - 1, k) : c0;
     const double yp = j < ny ? at(i, j + 1, k) : c0;
     const double lap = rdx2 * lx + rdz2 * lz + (am[j] * ym + ac[j] * c0 + ap[j] * yp);
-    const double res = f[(long long)i + (long long)nx * ((long long)j + (long long)(ny + 2) * k)] - lap;
+    const long long o = (long long)i + (long long)nx * ((long long)j + (long long)(ny + 2) * k);
+    const double res = f[o] - lap;
+    if (WRITE) rout[o] = res;
     mx = fmax(mx, fabs(res));
   }
+  if (!part) return;
   __shared__ double sh[256];
   sh[threadIdx.x] = mx;
   __syncthreads();
@@ -265,6 +315,14 @@ __global__ void __launch_bounds__(256) k_residual_max(const double *__restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
+  const int nblk = std::min(8192, c->n_cu * 16);
+  const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
+  hipLaunchKernelGGL(k_residual<true>, dim3(nblk), dim3(256), 0, c->stream, f, phi, c->geo, rdx2, rdz2, r, nullptr);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax) {
   const int nblk = std::min(2048, c->n_cu * 8);
   if (c->red_cap < nblk) {
@@ -273,7 +331,8 @@ int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *r
     c->red_cap = nblk;
   }
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual_max, dim3(nblk), dim3(256), 0, c->stream, src, dst, c->geo, rdx2, rdz2, c->d_red);
+  hipLaunchKernelGGL(k_residual<false>, dim3(nblk), dim3(256), 0, c->stream, src, dst, c->geo, rdx2, rdz2, nullptr,
+                     c->d_red);
   PTC_HIP(hipGetLastError());
   std::vector<double> h((size_t)nblk);
   PTC_HIP(hipMemcpyAsync(h.data(), c->d_red, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));

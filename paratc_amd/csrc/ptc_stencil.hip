@@ -523,6 +523,10 @@ int launch_add_shift(ptc_ctx *c, double *rhsx) {
 
 const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
 
+// reduction partials of the RHS and PrSrc kernels, allocated ahead of a
+// captured step
+int stencil_prepare(ptc_ctx *c) { return ensure_red(c, pt_nblocks(c)); }
+
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host) {
   const int nb = pt_nblocks(c);

@@ -224,8 +224,10 @@ constexpr int tri_cn_block_cap() {
 // lane 2m+1 its imaginary part: a row is still one contiguous 8-byte-per-lane
 // access, the right-hand side takes half the registers, and with LPB = 64 a
 // chunk is one wavefront so the row coefficients are wave-uniform.
-template <int MM, int LPB>
-__global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict__ spec, const double *__restrict__ met,
+// S: the spectrum's storage (double2, or float2 in the FP32-storage form);
+// the elimination itself is FP64 in both.
+template <int MM, int LPB, typename S>
+__global__ void __launch_bounds__(1024) k_tri_poisson_split(S *__restrict__ spec, const double *__restrict__ met,
                                                            int nyh, const double *__restrict__ wnx,
                                                            const double *__restrict__ wnz, int ny, int pitch, int h,
                                                            int P) {
@@ -241,11 +243,12 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict_
   const double lam = valid ? wnx[kx] + wnz[kz] : 1.0;
   const bool pin = (kx == 0 && kz == 0);
   // row base addresses are wave-uniform (scalar), the lane adds a 32-bit offset
-  double *base = reinterpret_cast<double *>(spec + (long long)kz * ny * pitch) + (long long)blockIdx.x * LPB;
+  using R = decltype(spec->x);
+  R *base = reinterpret_cast<R *>(spec + (long long)kz * ny * pitch) + (long long)blockIdx.x * LPB;
   const long long rs = 2LL * pitch;
   double f[MM];
 #pragma unroll
-  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? base[(s + r) * rs + lane] : 0.0;
+  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? (double)base[(s + r) * rs + lane] : 0.0;
   if (pin && s == 0) f[0] = 0.0;  // phi(1) = 0 on the (0,0) line
   PoissonRowsRe getc{met + M_AM2PH * nyh, met + M_AC2PH * nyh, met + M_AP2PH * nyh, lam, ny, pin};
   double A[MM], C[MM];
@@ -256,7 +259,7 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(double2 *__restrict_
   if (valid) {
 #pragma unroll
     for (int r = 0; r < MM; ++r)
-      if (s + r < ny) base[(s + r) * rs + lane] = f[r];
+      if (s + r < ny) base[(s + r) * rs + lane] = (R)f[r];
   }
 }
 
@@ -381,13 +384,15 @@ int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int
                  : tri_choose(ny, lds_limit, mm_req, kCnShapes, 4, P_out, mm_out, lpb_out);
 }
 
-int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
+int launch_tri_poisson(ptc_ctx *c, void *spec) {
   PoissonPlan &P = c->pp;
+  const bool f32 = P.prec == PTC_PREC_FP32_REFINE;
   if (P.tri == PTC_TRI_THOMAS) {
+    PTC_CHECK(!f32, PTC_E_UNSUPPORTED, "the Thomas y-solve has no FP32-storage form");
     dim3 blk(64, 4);
     dim3 grid((unsigned)((P.h + 1 + 63) / 64), (unsigned)((c->nz + 3) / 4));
-    hipLaunchKernelGGL(k_tri_poisson_thomas, grid, blk, 0, c->stream, spec, P.d_tri_scratch, c->d_met, c->ny + 2,
-                       c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, c->nz);
+    hipLaunchKernelGGL(k_tri_poisson_thomas, grid, blk, 0, c->stream, (double2 *)spec, P.d_tri_scratch, c->d_met,
+                       c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, c->nz);
     PTC_HIP(hipGetLastError());
     return PTC_OK;
   }
@@ -395,12 +400,16 @@ int launch_tri_poisson(ptc_ctx *c, double2 *spec) {
   // lpb lanes per block = lpb/2 complex lines
   dim3 grid((unsigned)((2 * (P.h + 1) + lpb - 1) / lpb), (unsigned)c->nz);
   const size_t lds = (size_t)4 * nP * lpb * 24;
-#define PTC_CASE(MM, LPB)                                                                                    \
-  if (P.tri_mmax == MM && lpb == LPB) {                                                                      \
-    hipLaunchKernelGGL((k_tri_poisson_split<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, spec, c->d_met,  \
-                       c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);                              \
-    PTC_HIP(hipGetLastError());                                                                              \
-    return PTC_OK;                                                                                           \
+#define PTC_CASE(MM, LPB)                                                                                     \
+  if (P.tri_mmax == MM && lpb == LPB) {                                                                       \
+    if (f32)                                                                                                  \
+      hipLaunchKernelGGL((k_tri_poisson_split<MM, LPB, float2>), grid, dim3(LPB * nP), lds, c->stream,        \
+                         (float2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);   \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_tri_poisson_split<MM, LPB, double2>), grid, dim3(LPB * nP), lds, c->stream,       \
+                         (double2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);  \
+    PTC_HIP(hipGetLastError());                                                                               \
+    return PTC_OK;                                                                                            \
   }
   PTC_CASE(4, 64)
   PTC_CASE(8, 64)

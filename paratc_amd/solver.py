@@ -29,6 +29,22 @@ class Solver:
         check(self.L.ptc_create(C.byref(g), C.byref(p), _dp(self.yp), C.byref(h)), "ptc_create")
         self.h = h
 
+    @classmethod
+    def from_handle(cls, case, handle) -> "Solver":
+        """Wrap an existing context (one handed back by an ensemble)."""
+        self = cls.__new__(cls)
+        self.L = capi.load()
+        self.case = case
+        self.nx, self.ny, self.nz = case.nx, case.ny, case.nz
+        self.yp = np.ascontiguousarray(case.yp(), dtype=np.float64)
+        self.h = handle if isinstance(handle, C.c_void_p) else C.c_void_p(handle)
+        return self
+
+    def detach(self) -> C.c_void_p:
+        """Give up ownership of the context (an ensemble takes it over)."""
+        h, self.h = self.h, None
+        return h
+
     def close(self):
         if getattr(self, "h", None):
             self.L.ptc_destroy(self.h)

@@ -117,3 +117,17 @@ def test_restart_roundtrip_and_size_check(tmp_path):
         fh.write(b"\0" * 8)
     with pytest.raises(ValueError, match="file byte wrong2"):
         restart.read_restart(path, 6, 4, 8, scalar=True)
+
+
+def test_ensemble_workload_draw():
+    from paratc_amd import ensemble as E
+
+    wl = E.c3_workload()
+    assert len(wl.cases) == 64 and sum(c.scalar for c in wl.cases) == 32
+    assert all(c.nx in E.NX_SET and c.ny in E.NY_SET and c.nz in E.NZ_SET for c in wl.cases)
+    assert all(1 / 5000.0 <= c.xnu <= 1 / 2000.0 for c in wl.cases)
+    assert all(0.005 <= c.falling_vel <= 0.03 for c in wl.cases if c.scalar)
+    gaps = np.diff(wl.arrivals)
+    assert np.all(gaps >= 0) and 0.5 / 20 < gaps.mean() < 2.0 / 20  # Poisson process, rate 20/s
+    again = E.c3_workload()
+    assert [(c.nx, c.ny, c.nz, c.xnu) for c in again.cases] == [(c.nx, c.ny, c.nz, c.xnu) for c in wl.cases]

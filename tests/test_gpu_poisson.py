@@ -198,3 +198,50 @@ def test_specialised_fft_kernels_match_generic_and_oracle(grid):
     ref = orc.halo()
     orc.clcPPE(f.copy(order="F"), ref, 1)
     assert rel_err(fast, inner(ref)) <= TOL
+
+
+# ---- FP32 storage + FP64 defect correction ---------------------------------
+@pytest.mark.parametrize("grid,slab", [((64, 48, 32, 2.0), 0), ((128, 96, 128, 1.8), 0),
+                                       ((128, 96, 128, 1.8), 16), ((96, 140, 64, 2.2), 0)])
+def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab):
+    """Spectra stored in FP32 between passes (FP64 arithmetic); each sweep
+    adds the solve of the FP64 residual f - L phi."""
+    nx, ny, nz, gam = grid
+    case = CS.case_poisson(nx, ny, nz, gam)
+    f = ic.poisson_rhs(case, seed=12)
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    ref = inner(ref)
+    errs, res = [], []
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        for sweeps in (0, 1, 2, 3):
+            s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, sweeps)
+            if slab:
+                s.poisson_slab_planes(slab)
+            cfg = s.poisson_get_config()
+            assert cfg["prec"] == "fp32+refine" and cfg["sweeps"] == sweeps, cfg
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+            errs.append(rel_err(s.download(capi.F_HALO, capi.L_INTERIOR), ref))
+            res.append(s.poisson_residual(capi.F_ARR1, capi.F_HALO))
+        src_after = s.download(capi.F_ARR1, capi.L_INTERIOR)
+    print(grid, slab, "errors", [f"{e:.2e}" for e in errs], "residuals", [f"{r:.2e}" for r in res], flush=True)
+    assert np.array_equal(src_after, f)
+    assert errs[0] < 1e-4
+    assert errs[1] < 1e-2 * errs[0] and errs[2] <= max(1e-2 * errs[1], 2e-12)
+    assert errs[3] <= 1e-11
+    assert res[3] < res[0]
+
+
+def test_fp32_form_refuses_what_it_cannot_run():
+    case = CS.case_poisson(80, 16, 32, 2.0)  # nx/2 = 40 has no specialised kernel
+    with Solver(case) as s:
+        with pytest.raises(capi.PtcError, match="specialised FFT"):
+            s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 2)
+    case = CS.case_poisson(64, 16, 32, 2.0)
+    with Solver(case) as s:
+        with pytest.raises(capi.PtcError, match="Thomas"):
+            s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_THOMAS, 0, 2)
+        s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 2)
+        assert s.poisson_get_config()["tri"] == "partitioned"
