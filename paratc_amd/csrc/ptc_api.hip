@@ -192,6 +192,81 @@ int channel_step(ptc_ctx *c, double *mon) {
   return PTC_OK;
 }
 
+// Capture one step.  Host-side coefficient state that the capture advances
+// (substep coefficients, SimTime) is put back afterwards; each replay then
+// stands for exactly one step and the caller advances SimTime by dt.
+int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec) {
+  PTC_TRY(prepare_step(c));
+  const double save[9] = {c->pmGamma, c->pmTheta, c->pmAlpha, c->pmAlphaC, c->pmBeta,
+                          c->pmBetaSc, c->dt, c->simtime, c->prgradaver};
+  const bool prof = c->pp.profile;
+  c->pp.profile = false;
+  PTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = channel_step(c, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  c->pp.profile = prof;
+  c->pmGamma = save[0];
+  c->pmTheta = save[1];
+  c->pmAlpha = save[2];
+  c->pmAlphaC = save[3];
+  c->pmBeta = save[4];
+  c->pmBetaSc = save[5];
+  c->dt = save[6];
+  c->simtime = save[7];
+  c->prgradaver = save[8];
+  if (rc != PTC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  PTC_HIP(ec);
+  const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    PTC_HIP(ei);
+  }
+  *graph = g;
+  return PTC_OK;
+}
+
+void drop_step_graph(ptc_ctx *c) {
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
+  c->step_exec = nullptr;
+  c->step_graph = nullptr;
+}
+
+// One fixed-dt step as one graph launch (captured on first use).  With mon,
+// the CFL of the fields before the step and the monitors after it are
+// computed around the launch, as ChannelIterate reports them.
+static int graph_step(ptc_ctx *c, double *mon) {
+  if (!c->step_exec) PTC_TRY(capture_step(c, &c->step_graph, &c->step_exec));
+  double cflmp = 0.0;
+  double *ux, *uy, *uz;
+  PTC_TRY(field(c, PTC_F_UX, &ux));
+  PTC_TRY(field(c, PTC_F_UY, &uy));
+  PTC_TRY(field(c, PTC_F_UZ, &uz));
+  if (mon) {
+    double uddxmax = 0.0;
+    PTC_TRY(launch_monitor(c, 0, ux, uy, uz, nullptr, &uddxmax));
+    cflmp = 0.5 * uddxmax * c->dt;
+  }
+  PTC_HIP(hipGraphLaunch(c->step_exec, c->stream));
+  c->dt = c->prm.dtmax;
+  c->simtime += c->prm.dtmax;  // the RK3 alphas sum to one
+  if (mon) {
+    PTC_TRY(get_device_scalar(c, S_DIVMAX, &mon[0]));
+    c->divmax_prsrc = mon[0];
+    PTC_TRY(launch_monitor(c, 1, ux, uy, uz, nullptr, &mon[1]));
+    PTC_TRY(launch_monitor(c, 2, ux, uy, uz, nullptr, &mon[2]));
+    PTC_TRY(launch_monitor(c, 3, ux, uy, uz, nullptr, &mon[3]));
+    PTC_TRY(launch_monitor(c, 4, ux, uy, uz, nullptr, &mon[4]));
+    mon[5] = cflmp;
+    mon[6] = c->prm.dtmax;
+  }
+  return PTC_OK;
+}
+
 int prepare_step(ptc_ctx *c) {
   double *d = nullptr;
   for (int f : {PTC_F_UX, PTC_F_UY, PTC_F_UZ, PTC_F_PR, PTC_F_HISTX, PTC_F_HISTY, PTC_F_HISTZ, PTC_F_ARR1, PTC_F_ARR2,
@@ -601,7 +676,13 @@ int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7) {
   PTC_CTX(ctx);
   PTC_CHECK(nsteps >= 1, PTC_E_ARG, "nsteps must be >= 1");
   double mon[7];
-  for (int s = 0; s < nsteps; ++s) PTC_TRY(channel_step(ctx, (s == nsteps - 1) ? mon : nullptr));
+  // fixed dt: each step is one launch of a captured graph; CFL-adaptive dt
+  // needs the host between steps
+  for (int s = 0; s < nsteps; ++s) {
+    double *m = (s == nsteps - 1) ? mon : nullptr;
+    if (ctx->prm.icfl != 1) PTC_TRY(graph_step(ctx, m));
+    else PTC_TRY(channel_step(ctx, m));
+  }
   if (mon7) std::copy(mon, mon + 7, mon7);
   // src/m_ChannelSystem.f90:142-145
   if (ctx->prm.div_limit > 0 && mon[1] > ctx->prm.div_limit) {

@@ -16,7 +16,6 @@
 //  * every case-step records a completion event; its latency is measured
 //    from the start of the round that issued it (p50 / p99 on request).
 #include <algorithm>
-#include <chrono>
 #include <vector>
 
 #include "ptc_internal.hpp"
@@ -45,8 +44,9 @@ static size_t ctx_bytes(const ptc_ctx *c) {
   for (int f = 0; f < PTC_NFIELDS; ++f)
     if (c->d_field[f]) b += sizeof(double) * ((f == PTC_F_UY_YM || f == PTC_F_DUY_YM) ? (size_t)c->nx * c->nz : c->n3);
   b += sizeof(double) * (size_t)c->red_cap;
-  if (c->pp.d_spec) b += sizeof(double2) * c->pp.spec_elems;
+  if (c->pp.d_spec) b += c->pp.spec_bytes;
   if (c->pp.d_tri_scratch) b += sizeof(double) * c->pp.spec_elems;
+  if (c->pp.d_res) b += sizeof(double) * c->n3;
   return b;
 }
 
@@ -60,43 +60,10 @@ static int take_event(ptc_ensemble *e, hipEvent_t *ev) {
   return PTC_OK;
 }
 
-// Capture one step.  Host-side coefficient state that the capture advances
-// (substep coefficients, SimTime) is put back afterwards; each replay then
-// stands for exactly one step and SimTime is advanced by dt per replay.
-static int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec) {
-  const double save[9] = {c->pmGamma, c->pmTheta, c->pmAlpha, c->pmAlphaC, c->pmBeta,
-                          c->pmBetaSc, c->dt, c->simtime, c->prgradaver};
-  PTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = channel_step(c, nullptr);
-  hipGraph_t g = nullptr;
-  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
-  c->pmGamma = save[0];
-  c->pmTheta = save[1];
-  c->pmAlpha = save[2];
-  c->pmAlphaC = save[3];
-  c->pmBeta = save[4];
-  c->pmBetaSc = save[5];
-  c->dt = save[6];
-  c->simtime = save[7];
-  c->prgradaver = save[8];
-  if (rc != PTC_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
-  }
-  PTC_HIP(ec);
-  const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
-  if (ei != hipSuccess) {
-    (void)hipGraphDestroy(g);
-    PTC_HIP(ei);
-  }
-  *graph = g;
-  return PTC_OK;
-}
-
 static int step_case(ptc_ensemble::Case &k) {
   if (k.exec) {
     PTC_HIP(hipGraphLaunch(k.exec, k.ctx->stream));
-    k.ctx->simtime += k.ctx->prm.dtmax;
+    k.ctx->simtime += k.ctx->prm.dtmax;  // the RK3 alphas sum to one
   } else {
     PTC_TRY(channel_step(k.ctx, nullptr));
   }

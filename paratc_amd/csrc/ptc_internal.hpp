@@ -106,6 +106,7 @@ struct PoissonPlan {
   double *d_res = nullptr;    // FP32-storage form: FP64 residual field of the refinement sweeps
   double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
   size_t spec_elems = 0;
+  size_t spec_bytes = 0;
   bool profile = false;
   std::vector<hipEvent_t> prof_ev;  // 6 events per profiled solve
   int prof_used = 0;                // profiled solves since the last read
@@ -141,6 +142,8 @@ struct ptc_ctx {
   double dt = 0, simtime = 0, prgradaver = 0;
   double divmax_prsrc = 0;
   ptc::PoissonPlan pp;
+  hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
+  hipGraphExec_t step_exec = nullptr;
 };
 
 namespace ptc {
@@ -159,6 +162,9 @@ int stencil_prepare(ptc_ctx *c);
 int channel_step(ptc_ctx *c, double *mon);
 // allocate everything a step touches, so the step can be captured in a graph
 int prepare_step(ptc_ctx *c);
+int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec);
+// forget the captured step (its buffers are about to change)
+void drop_step_graph(ptc_ctx *c);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 

@@ -42,6 +42,7 @@ static void make_twiddles(int n, std::vector<double2> &out) {
 
 int poisson_free_plan(ptc_ctx *c) {
   PoissonPlan &P = c->pp;
+  drop_step_graph(c);  // the captured step holds the plan's buffers
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
   if (P.d_res) (void)hipFree(P.d_res);
@@ -141,6 +142,7 @@ int poisson_build_plan(ptc_ctx *c) {
 
   P.spec_elems = (size_t)P.pitch * c->ny * c->nz;
   const size_t spec_bytes = (P.prec == PTC_PREC_FP32_REFINE ? sizeof(float2) : sizeof(double2)) * P.spec_elems;
+  P.spec_bytes = spec_bytes;
   if (hipMalloc(&P.d_spec, spec_bytes) != hipSuccess) {
     P.d_spec = nullptr;
     set_error("spectral buffer allocation failed");

@@ -44,7 +44,15 @@ module m_paratc_gpu
   public :: ptc_set_bc_and_update_halo, ptc_set_bc_and_update_halo_pr
   public :: ptc_set_bc_and_update_halo_scalar, ptc_channel_iterate, ptc_calc_ux_aver
   public :: ptc_check_divergence, ptc_synchronize
+  public :: ptc_pm_coe_update, ptc_init_time_scheme, ptc_clc_rhs_xyz, ptc_clc_u1hat, ptc_clc_u2hat
+  public :: ptc_clc_u3hat, ptc_clc_scalar, ptc_clc_prsrc, ptc_fluid_vel_update, ptc_pressure_update
+  public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
+  public :: ptc_clc_mean_value, ptc_poisson_config, ptc_poisson_set, ptc_poisson_get_config
   public :: gpu_check, gpu_put, gpu_get, gpu_clcPPE
+
+  integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1
+  integer(c_int), parameter, public :: PTC_TRI_AUTO = 0, PTC_TRI_THOMAS = 1, PTC_TRI_PARTITIONED = 2
+  integer(c_int), parameter, public :: PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2
 
   interface
     integer(c_int) function ptc_create(grid, prm, yp, ctx) bind(C, name="ptc_create")
@@ -125,6 +133,106 @@ module m_paratc_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: ux, uy, uz
       real(c_double), intent(out) :: divmax
+    end function
+    ! m_Parameters / m_TScheme (src/m_Parameters.f90:215-230, src/m_TScheme.f90:25-532)
+    integer(c_int) function ptc_pm_coe_update(ctx, ns, dt) bind(C, name="ptc_pm_coe_update")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ns
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function ptc_init_time_scheme(ctx) bind(C, name="ptc_init_time_scheme")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function ptc_clc_rhs_xyz(ctx, ux, uy, uz, sc, rx, ry, rz, rc, hx, hy, hz, hc, p) &
+        bind(C, name="ptc_clc_rhs_xyz")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz, sc, rx, ry, rz, rc, hx, hy, hz, hc, p
+    end function
+    integer(c_int) function ptc_clc_u1hat(ctx, ux, rhsx) bind(C, name="ptc_clc_u1hat")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, rhsx
+    end function
+    integer(c_int) function ptc_clc_u2hat(ctx, uy, rhsy, duy_ym) bind(C, name="ptc_clc_u2hat")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: uy, rhsy, duy_ym
+    end function
+    integer(c_int) function ptc_clc_u3hat(ctx, uz, rhsz) bind(C, name="ptc_clc_u3hat")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: uz, rhsz
+    end function
+    integer(c_int) function ptc_clc_scalar(ctx, sc, rhsc) bind(C, name="ptc_clc_scalar")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: sc, rhsc
+    end function
+    integer(c_int) function ptc_clc_prsrc(ctx, ux, uy, uz, prsrc, divmax) bind(C, name="ptc_clc_prsrc")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz, prsrc
+      real(c_double), intent(out) :: divmax
+    end function
+    integer(c_int) function ptc_fluid_vel_update(ctx, phi, ux, uy, uz) bind(C, name="ptc_fluid_vel_update")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: phi, ux, uy, uz
+    end function
+    integer(c_int) function ptc_pressure_update(ctx, p, phi) bind(C, name="ptc_pressure_update")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p, phi
+    end function
+    ! m_BC_and_Halo / m_FlowCase (src/m_BC_and_Halo.f90:69-103, src/m_FlowCase.f90:112-122)
+    integer(c_int) function ptc_set_bc_and_update_halo_for_prsrc(ctx, ux, uy, uz, uy_ym) &
+        bind(C, name="ptc_set_bc_and_update_halo_for_prsrc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz, uy_ym
+    end function
+    integer(c_int) function ptc_update_uy_ym(ctx, uy_ym, duy_ym) bind(C, name="ptc_update_uy_ym")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: uy_ym, duy_ym
+    end function
+    ! m_Tools monitors (src/m_Tools.f90:24-110,256-301)
+    integer(c_int) function ptc_calc_max_cfl(ctx, ux, uy, uz, uddxmax) bind(C, name="ptc_calc_max_cfl")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz
+      real(c_double), intent(out) :: uddxmax
+    end function
+    integer(c_int) function ptc_calc_vmax(ctx, ux, uy, uz, vmax) bind(C, name="ptc_calc_vmax")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz
+      real(c_double), intent(out) :: vmax(3)
+    end function
+    integer(c_int) function ptc_clc_mean_value(ctx, ux, sc, res) bind(C, name="ptc_clc_mean_value")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, sc
+      real(c_double), intent(out) :: res(3)
+    end function
+    ! Poisson plan (the scheme choice of InitPoissonSolver, src/m_Poisson.f90:50-171)
+    integer(c_int) function ptc_poisson_config(ctx, prec, tri, tile, sweeps) bind(C, name="ptc_poisson_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: prec, tri, tile, sweeps
+    end function
+    integer(c_int) function ptc_poisson_set(ctx, knob, val) bind(C, name="ptc_poisson_set")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: knob, val
+    end function
+    integer(c_int) function ptc_poisson_get_config(ctx, out9) bind(C, name="ptc_poisson_get_config")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: out9(9)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr

@@ -131,3 +131,10 @@ def test_ensemble_workload_draw():
     assert np.all(gaps >= 0) and 0.5 / 20 < gaps.mean() < 2.0 / 20  # Poisson process, rate 20/s
     again = E.c3_workload()
     assert [(c.nx, c.ny, c.nz, c.xnu) for c in again.cases] == [(c.nx, c.ny, c.nz, c.xnu) for c in wl.cases]
+
+
+def test_fortran_binding_names_declared_symbols():
+    text = open(os.path.join(ROOT, "paratc_amd", "fortran", "m_paratc_gpu.f90")).read()
+    names = set(re.findall(r'bind\(C,\s*name="(ptc_[a-z0-9_]+)"\)', text))
+    assert len(names) >= 30
+    assert names <= set(header_symbols()), names - set(header_symbols())
