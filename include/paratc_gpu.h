@@ -225,6 +225,17 @@ int ptc_clc_mean_value(ptc_ctx *ctx, int ux, int scalar, double *res3);
 int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7);
 int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
 
+/* ---- clcStat (src/m_FlowCase.f90:257-949), the statistics side.
+ *  ptc_clc_stat adds the x-z means of the NCHASTAT statistics (40 with a
+ *  scalar, else 35) of planes j = 1..ny+1 to a device SumStat (:283-413),
+ *  PrGradAver to PrGradsum (:416) and counts the sample (nfstime, :428);
+ *  scalar = -1 without a scalar.  ptc_get_stat copies SumStat(NCHASTAT, ny+1)
+ *  (column-major, as the reference's array; may be NULL), the statistic count,
+ *  nfstime and PrGradsum; reset = 1 zeroes them as after the reference writes
+ *  its stats file (:944).  The spectra of clcStat are not part of this. */
+int ptc_clc_stat(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar);
+int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double *prgradsum, int reset);
+
 /* ---- Ensembles: many independent cases on one GPU.  No counterpart in the
  *      reference, which runs one case per job; this is the serving layer
  *      around ChannelIterate (src/m_ChannelSystem.f90:80-145).

@@ -87,6 +87,7 @@ def lib():
         L.orc_calc_ux_aver.restype = C.c_double
         L.orc_calc_ux_aver.argtypes = [vp, dp]
         L.orc_clc_mean_value.argtypes = [vp, dp, dp, dp]
+        L.orc_clc_stat.argtypes = [vp] + [dp] * 6
         L.orc_channel_iterate.argtypes = [vp, C.c_int] + [dp] * 15 + [dp]
         _lib = L
     return _lib
@@ -258,6 +259,10 @@ class Oracle:
         v = np.zeros(3)
         self.L.orc_clc_mean_value(self.h, _p(ux), _p(s), _p(v))
         return v
+
+    def clcStat(self, ux, uy, uz, p, scalar, sumstat):
+        """Adds this sample's plane means to sumstat (NCHASTAT, ny+1), Fortran order."""
+        self.L.orc_clc_stat(self.h, _p(ux), _p(uy), _p(uz), _p(p), _p(scalar), _p(sumstat))
 
     def ChannelIterate(self, st: "State", choice: int = 1):
         mon = np.zeros(7)

@@ -1181,6 +1181,135 @@ void orc_clc_mean_value(const orc_ctx *c, const double *ux, const double *s, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* clcStat: src/m_FlowCase.f90:283-413                                        */
+/* ------------------------------------------------------------------------ */
+void orc_clc_stat(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                  const double *sc, double *sumstat) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nyp = ny + 1;
+  const int ns = c->cfg.scalar ? 40 : 35;
+  const double uCRF = c->cfg.ucrf, half = 0.5;
+  const double IC1 = c->ic1, IC2 = c->ic2, IC3 = c->ic3, IC4 = c->ic4;
+  const double rdxt = c->rdx / 12.0, rdzt = c->rdz / 12.0;
+  const double inxz = 1.0 / ((double)nx * (double)nz);
+  double sv[40];
+  for (int jc = 1; jc <= ny; jc++) {
+    const int jp = jc + 1, jm = jc - 1;
+    const double InterpY1 = c->yint[jc], InterpY2 = 1.0 - InterpY1;
+    const double cac = c->rdyc[jc], cacU = c->rdyc[jp], caj = c->rdyp[jc];
+    for (int q = 0; q < 40; q++) sv[q] = 0.0;
+    for (int kc = 1; kc <= nz; kc++) {
+      const int ks = kc - 2, km = kc - 1, kp = kc + 1, ku = kc + 2;
+      for (int ic = 1; ic <= nx; ic++) {
+        const int is = ic - 2, im = ic - 1, ip = ic + 1, iu = ic + 2;
+        const double uxloc = H(c, ux, ic, jc, kc) + uCRF;
+        const double uyloc = H(c, uy, ic, jc, kc);
+        const double uzloc = H(c, uz, ic, jc, kc);
+        const double prloc = H(c, pr, ic, jc, kc);
+        const double uxCell = IC1 * H(c, ux, im, jc, kc) + IC2 * H(c, ux, ic, jc, kc) + IC3 * H(c, ux, ip, jc, kc) + IC4 * H(c, ux, iu, jc, kc) + uCRF;
+        const double uyCell = (uyloc + H(c, uy, ic, jp, kc)) * half;
+        const double uy_xc = IC1 * H(c, uy, is, jc, kc) + IC2 * H(c, uy, im, jc, kc) + IC3 * H(c, uy, ic, jc, kc) + IC4 * H(c, uy, ip, jc, kc);
+        const double uy_xp = IC1 * H(c, uy, is, jp, kc) + IC2 * H(c, uy, im, jp, kc) + IC3 * H(c, uy, ic, jp, kc) + IC4 * H(c, uy, ip, jp, kc);
+        const double uzCell = IC1 * H(c, uz, ic, jc, km) + IC2 * H(c, uz, ic, jc, kc) + IC3 * H(c, uz, ic, jc, kp) + IC4 * H(c, uz, ic, jc, ku);
+        const double prloc2 = InterpY1 * (IC1 * H(c, pr, is, jm, kc) + IC2 * H(c, pr, im, jm, kc) + IC3 * H(c, pr, ic, jm, kc) + IC4 * H(c, pr, ip, jm, kc))
+                            + InterpY2 * (IC1 * H(c, pr, is, jc, kc) + IC2 * H(c, pr, im, jc, kc) + IC3 * H(c, pr, ic, jc, kc) + IC4 * H(c, pr, ip, jc, kc));
+        const double dudx = c->dxc1 * H(c, ux, im, jc, kc) + c->dxc2 * H(c, ux, ic, jc, kc) + c->dxc3 * H(c, ux, ip, jc, kc) + c->dxc4 * H(c, ux, iu, jc, kc);
+        const double dudy = (H(c, ux, ic, jc, kc) - H(c, ux, ic, jm, kc)) * cac;
+        const double dudz = c->dzc1 * H(c, ux, ic, jc, ks) + c->dzc2 * H(c, ux, ic, jc, km) + c->dzc3 * H(c, ux, ic, jc, kc) + c->dzc4 * H(c, ux, ic, jc, kp);
+        const double dvdx = c->dxc1 * H(c, uy, is, jc, kc) + c->dxc2 * H(c, uy, im, jc, kc) + c->dxc3 * H(c, uy, ic, jc, kc) + c->dxc4 * H(c, uy, ip, jc, kc);
+        const double dvdy = (H(c, uy, ic, jp, kc) - uyloc) * caj;
+        const double dvdz = c->dzc1 * H(c, uy, ic, jc, ks) + c->dzc2 * H(c, uy, ic, jc, km) + c->dzc3 * H(c, uy, ic, jc, kc) + c->dzc4 * H(c, uy, ic, jc, kp);
+        const double dwdx = c->dxc1 * H(c, uz, is, jc, kc) + c->dxc2 * H(c, uz, im, jc, kc) + c->dxc3 * H(c, uz, ic, jc, kc) + c->dxc4 * H(c, uz, ip, jc, kc);
+        const double dwdy = (uzloc - H(c, uz, ic, jm, kc)) * cac;
+        const double dwdz = c->dzc1 * H(c, uz, ic, jc, km) + c->dzc2 * H(c, uz, ic, jc, kc) + c->dzc3 * H(c, uz, ic, jc, kp) + c->dzc4 * H(c, uz, ic, jc, ku);
+        const double dudxx = c->dxxc1 * H(c, ux, is, jc, kc) + c->dxxc2 * H(c, ux, im, jc, kc) + c->dxxc3 * H(c, ux, ic, jc, kc) + c->dxxc4 * H(c, ux, ip, jc, kc) + c->dxxc5 * H(c, ux, iu, jc, kc);
+        const double dudyy = c->ap2c[jc] * H(c, ux, ic, jp, kc) + c->ac2c[jc] * H(c, ux, ic, jc, kc) + c->am2c[jc] * H(c, ux, ic, jm, kc);
+        const double dudzz = c->dzzc1 * H(c, ux, ic, jc, ks) + c->dzzc2 * H(c, ux, ic, jc, km) + c->dzzc3 * H(c, ux, ic, jc, kc) + c->dzzc4 * H(c, ux, ic, jc, kp) + c->dzzc5 * H(c, ux, ic, jc, ku);
+        const double dvdxx = c->dxxc1 * H(c, uy, is, jc, kc) + c->dxxc2 * H(c, uy, im, jc, kc) + c->dxxc3 * H(c, uy, ic, jc, kc) + c->dxxc4 * H(c, uy, ip, jc, kc) + c->dxxc5 * H(c, uy, iu, jc, kc);
+        const double dvdyy = c->ap2p[jc] * H(c, uy, ic, jp, kc) + c->ac2p[jc] * H(c, uy, ic, jc, kc) + c->am2p[jc] * H(c, uy, ic, jm, kc);
+        const double dvdzz = c->dzzc1 * H(c, uy, ic, jc, ks) + c->dzzc2 * H(c, uy, ic, jc, km) + c->dzzc3 * H(c, uy, ic, jc, kc) + c->dzzc4 * H(c, uy, ic, jc, kp) + c->dzzc5 * H(c, uy, ic, jc, ku);
+        const double dwdxx = c->dxxc1 * H(c, uz, is, jc, kc) + c->dxxc2 * H(c, uz, im, jc, kc) + c->dxxc3 * H(c, uz, ic, jc, kc) + c->dxxc4 * H(c, uz, ip, jc, kc) + c->dxxc5 * H(c, uz, iu, jc, kc);
+        const double dwdyy = c->ap2c[jc] * H(c, uz, ic, jp, kc) + c->ac2c[jc] * H(c, uz, ic, jc, kc) + c->am2c[jc] * H(c, uz, ic, jm, kc);
+        const double dwdzz = c->dzzc1 * H(c, uz, ic, jc, ks) + c->dzzc2 * H(c, uz, ic, jc, km) + c->dzzc3 * H(c, uz, ic, jc, kc) + c->dzzc4 * H(c, uz, ic, jc, kp) + c->dzzc5 * H(c, uz, ic, jc, ku);
+        const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+        const double dudxC = (-H(c, ux, iu, jc, kc) + 8.0 * H(c, ux, ip, jc, kc) - 8.0 * H(c, ux, im, jc, kc) + H(c, ux, is, jc, kc)) * rdxt;
+        const double dvdxU = c->dxc1 * H(c, uy, is, jp, kc) + c->dxc2 * H(c, uy, im, jp, kc) + c->dxc3 * H(c, uy, ic, jp, kc) + c->dxc4 * H(c, uy, ip, jp, kc);
+        const double dudyU = (H(c, ux, ic, jp, kc) - H(c, ux, ic, jc, kc)) * cacU;
+        const double dudzC = (InterpY1 * (-H(c, ux, ic, jm, ku) + 8.0 * H(c, ux, ic, jm, kp) - 8.0 * H(c, ux, ic, jm, km) + H(c, ux, ic, jm, ks))
+                            + InterpY2 * (-H(c, ux, ic, jc, ku) + 8.0 * H(c, ux, ic, jc, kp) - 8.0 * H(c, ux, ic, jc, km) + H(c, ux, ic, jc, ks))) * rdzt;
+        const double dvdzC = ((-H(c, uy, is, jc, ku) + 8.0 * H(c, uy, is, jc, kp) - 8.0 * H(c, uy, is, jc, km) + H(c, uy, is, jc, ks)) * IC1
+                            + (-H(c, uy, im, jc, ku) + 8.0 * H(c, uy, im, jc, kp) - 8.0 * H(c, uy, im, jc, km) + H(c, uy, im, jc, ks)) * IC2
+                            + (-H(c, uy, ic, jc, ku) + 8.0 * H(c, uy, ic, jc, kp) - 8.0 * H(c, uy, ic, jc, km) + H(c, uy, ic, jc, ks)) * IC3
+                            + (-H(c, uy, ip, jc, ku) + 8.0 * H(c, uy, ip, jc, kp) - 8.0 * H(c, uy, ip, jc, km) + H(c, uy, ip, jc, ks)) * IC4) * rdzt;
+        sv[0] += uxloc;
+        sv[1] += uyloc;
+        sv[2] += uzloc;
+        sv[3] += prloc;
+        sv[4] += uxloc * uxloc;
+        sv[5] += uyloc * uyloc;
+        sv[6] += uzloc * uzloc;
+        sv[7] += prloc * prloc;
+        sv[8] += uxCell * uyCell;
+        sv[9] += uyCell * uzCell;
+        sv[10] += uxCell * uzCell;
+        sv[11] += uxCell * prloc;
+        sv[12] += uyCell * prloc;
+        sv[13] += uzCell * prloc;
+        sv[14] += uxCell * uxCell * uyCell;
+        sv[15] += uyloc * uyloc * uyloc;
+        sv[16] += uzCell * uzCell * uyCell;
+        sv[17] += uxCell * uyCell * uyCell;
+        sv[18] += uxloc * uxloc * uxloc;
+        sv[19] += uzloc * uzloc * uzloc;
+        sv[20] += uxloc * uxloc * uxloc * uxloc;
+        sv[21] += uyloc * uyloc * uyloc * uyloc;
+        sv[22] += uzloc * uzloc * uzloc * uzloc;
+        sv[23] += prloc * dudx;
+        sv[24] += prloc * dvdy;
+        sv[25] += prloc * dwdz;
+        sv[26] += prloc2 * (dudy + dvdx);
+        sv[27] += uxloc * (dudxx + dudyy + dudzz);
+        sv[28] += uyloc * (dvdxx + dvdyy + dvdzz);
+        sv[29] += uzloc * (dwdxx + dwdyy + dwdzz);
+        sv[30] += (dudxC * (dvdxU + dvdx) + (dudyU + dudy) * (uy_xp - uy_xc) * caj) * half;
+        sv[31] += dudzC * dvdzC;
+        sv[32] += vor_x * vor_x;
+        sv[33] += vor_y * vor_y;
+        sv[34] += vor_z * vor_z;
+        if (ns == 40) {
+          const double scloc = H(c, sc, ic, jc, kc);
+          sv[35] += scloc;
+          sv[36] += scloc * scloc;
+          sv[37] += scloc * uxCell;
+          sv[38] += scloc * uyCell;
+          sv[39] += scloc * uzCell;
+        }
+      }
+    }
+    for (int q = 0; q < ns; q++) sumstat[q + (size_t)ns * (jc - 1)] += sv[q] * inxz;
+  }
+  /* nyp only (:393-413) */
+  {
+    const int jc = nyp, jm = jc - 1;
+    const double cac = c->rdyc[jc];
+    const double InterpY1 = c->yint[jc], InterpY2 = 1.0 - InterpY1;
+    for (int q = 0; q < 40; q++) sv[q] = 0.0;
+    for (int kc = 1; kc <= nz; kc++)
+      for (int ic = 1; ic <= nx; ic++) {
+        const int is = ic - 2, im = ic - 1, ip = ic + 1;
+        const double prloc2 = InterpY1 * (IC1 * H(c, pr, is, jm, kc) + IC2 * H(c, pr, im, jm, kc) + IC3 * H(c, pr, ic, jm, kc) + IC4 * H(c, pr, ip, jm, kc))
+                            + InterpY2 * (IC1 * H(c, pr, is, jc, kc) + IC2 * H(c, pr, im, jc, kc) + IC3 * H(c, pr, ic, jc, kc) + IC4 * H(c, pr, ip, jc, kc));
+        const double dudy = (H(c, ux, ic, jc, kc) - H(c, ux, ic, jm, kc)) * cac;
+        const double dwdy = (H(c, uz, ic, jc, kc) - H(c, uz, ic, jm, kc)) * cac;
+        const double vor_x = dwdy, vor_z = -dudy;
+        sv[26] += prloc2 * (dudy + 0.0);
+        sv[32] += vor_x * vor_x;
+        sv[34] += vor_z * vor_z;
+      }
+    for (int q = 0; q < ns; q++) sumstat[q + (size_t)ns * (jc - 1)] += sv[q] * inxz;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* ChannelIterate: src/m_ChannelSystem.f90:80-145                            */
 /* ------------------------------------------------------------------------ */
 void orc_channel_iterate(orc_ctx *c, int poisson_choice,

@@ -101,6 +101,12 @@ void orc_calc_vmax(const orc_ctx *c, const double *ux, const double *uy, const d
 double orc_calc_ux_aver(const orc_ctx *c, const double *ux);
 void orc_clc_mean_value(const orc_ctx *c, const double *ux, const double *scalar, double *res3);
 
+/* clcStat accumulation (m_FlowCase.f90:283-413): adds the x-z means of the
+ * NCHASTAT = 40 (scalar build) or 35 statistics of planes j = 1..ny+1 to
+ * sumstat(NCHASTAT, ny+1), column-major as the reference's SumStat. */
+void orc_clc_stat(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pressure,
+                  const double *scalar, double *sumstat);
+
 /* One ChannelIterate time step (m_ChannelSystem.f90:80-145).  Buffers follow
  * the reference's aliasing: arr1 = RealArr1, arr2 = RealArr2, halo = RealHalo,
  * arrc = RealArrC.  mon receives [divmax1 (last substep), divmax2, vmax x3,

@@ -17,7 +17,6 @@ void destroy(ptc_ctx *c);
 const char *last_error();
 int upload(ptc_ctx *c, int id, const double *host, int layout);
 int download(ptc_ctx *c, int id, double *host, int layout);
-int fill_device(ptc_ctx *c, double *a, long long n, double v);
 int copy_device(ptc_ctx *c, double *dst, const double *src, long long n);
 size_t field_count(const ptc_ctx *c, int id);
 int launch_add_shift(ptc_ctx *c, double *rhsx);
@@ -695,6 +694,27 @@ int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7) {
     return PTC_E_LIMIT;
   }
   return PTC_OK;
+}
+
+int ptc_clc_stat(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar) {
+  PTC_CTX(ctx);
+  double *a, *b, *d, *p, *s = nullptr;
+  PTC_TRY(field(ctx, ux, &a));
+  PTC_TRY(field(ctx, uy, &b));
+  PTC_TRY(field(ctx, uz, &d));
+  PTC_TRY(field(ctx, pressure, &p));
+  if (ctx->prm.scalar) {
+    PTC_CHECK(scalar >= 0, PTC_E_ARG, "clcStat: the scalar build needs the scalar field");
+    PTC_TRY(field(ctx, scalar, &s));
+  }
+  return launch_stat(ctx, a, b, d, p, s);
+}
+
+int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double *prgradsum, int reset) {
+  PTC_CTX(ctx);
+  if (nstat) *nstat = ctx->prm.scalar ? 40 : 35;
+  if (nfstime) *nfstime = ctx->nfstime;
+  return read_stat(ctx, sumstat, prgradsum, reset);
 }
 
 int ptc_get_scalar(ptc_ctx *ctx, int which, double *value) {

@@ -142,6 +142,8 @@ struct ptc_ctx {
   double dt = 0, simtime = 0, prgradaver = 0;
   double divmax_prsrc = 0;
   ptc::PoissonPlan pp;
+  double *d_stat = nullptr;  // clcStat accumulator SumStat(NCHASTAT, ny+1)
+  int nfstime = 0;           // samples in it
   hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
   hipGraphExec_t step_exec = nullptr;
 };
@@ -160,6 +162,12 @@ int poisson_prepare(ptc_ctx *c);
 int stencil_prepare(ptc_ctx *c);
 // one ChannelIterate step (monitors into mon[7] when mon != NULL)
 int channel_step(ptc_ctx *c, double *mon);
+// fill n doubles of device memory with v on the context's stream
+int fill_device(ptc_ctx *c, double *a, long long n, double v);
+// clcStat accumulation and read-back (ptc_stencil.hip)
+int launch_stat(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                const double *sc);
+int read_stat(ptc_ctx *c, double *out, double *prgradsum, int reset);
 // allocate everything a step touches, so the step can be captured in a graph
 int prepare_step(ptc_ctx *c);
 int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec);

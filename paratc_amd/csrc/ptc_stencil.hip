@@ -23,7 +23,7 @@
 namespace ptc {
 
 // device scalar slots in ctx->d_scal
-enum { S_DP1NS = 0, S_PRGRAD = 1, S_DIVMAX = 2, S_MON0 = 8 };
+enum { S_DP1NS = 0, S_PRGRAD = 1, S_DIVMAX = 2, S_PRGSUM = 3, S_MON0 = 8 };
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
@@ -651,6 +651,218 @@ int get_device_scalar(ptc_ctx *c, int slot, double *out) {
   PTC_HIP(hipMemcpyAsync(c->h_pinned + 2, c->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PTC_HIP(hipStreamSynchronize(c->stream));
   *out = c->h_pinned[2];
+  return PTC_OK;
+}
+
+}  // namespace ptc
+
+namespace ptc {
+
+// ---- clcStat (src/m_FlowCase.f90:283-413) ----------------------------------
+// Block (chunk, j): plane j = 1..ny+1, planes k in [chunk*kc, chunk*kc + kc).
+// Each thread sums the reference's SumVec terms over its points; the block
+// reduces them in a fixed order into part[(j-1)][chunk][q].  Plane ny+1 has
+// only the wall terms 27, 33 and 35 (1-based) of the reference.
+template <bool SC>
+__global__ void __launch_bounds__(256) k_stat_partial(Geo g, double ucrf, const double *__restrict__ ux,
+                                                      const double *__restrict__ uy, const double *__restrict__ uz,
+                                                      const double *__restrict__ pr, const double *__restrict__ sc,
+                                                      int kc, double *__restrict__ part) {
+  constexpr int NS = SC ? 40 : 35;
+  __shared__ double sh[256];
+  const int j = blockIdx.y + 1, ny = g.ny;
+  const int k0 = blockIdx.x * kc, k1 = min(k0 + kc, g.nz);
+  const double *met = g.met;
+  const int nyh = g.nyh;
+  const double IY1 = met[M_YINT * nyh + j], IY2 = 1.0 - IY1;
+  const double cac = met[M_RDYC * nyh + j];
+  const double I1 = g.ic1, I2 = g.ic2, I3 = g.ic3, I4 = g.ic4;
+  const double rdxt = g.rdx / 12.0, rdzt = g.rdz / 12.0;
+  double sv[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) sv[q] = 0.0;
+  if (j <= ny) {
+    const int jm = j - 1, jp = j + 1;
+    const double cacU = met[M_RDYC * nyh + jp], caj = met[M_RDYP * nyh + j];
+    const double ap2c = met[M_AP2C * nyh + j], ac2c = met[M_AC2C * nyh + j], am2c = met[M_AM2C * nyh + j];
+    const double ap2p = met[M_AP2P * nyh + j], ac2p = met[M_AC2P * nyh + j], am2p = met[M_AM2P * nyh + j];
+    for (int k = k0; k < k1; ++k)
+      for (int i = threadIdx.x; i < g.nx; i += 256) {
+        const Ngh n(g, i, k);
+        auto U = [&](int di, int jj, int dk) { return n(ux, di, jj, dk); };
+        auto V = [&](int di, int jj, int dk) { return n(uy, di, jj, dk); };
+        auto W = [&](int di, int jj, int dk) { return n(uz, di, jj, dk); };
+        auto Pp = [&](int di, int jj, int dk) { return n(pr, di, jj, dk); };
+        const double uxloc = U(0, j, 0) + ucrf, uyloc = V(0, j, 0), uzloc = W(0, j, 0), prloc = Pp(0, j, 0);
+        const double uxCell = I1 * U(-1, j, 0) + I2 * U(0, j, 0) + I3 * U(1, j, 0) + I4 * U(2, j, 0) + ucrf;
+        const double uyCell = (uyloc + V(0, jp, 0)) * 0.5;
+        const double uy_xc = I1 * V(-2, j, 0) + I2 * V(-1, j, 0) + I3 * V(0, j, 0) + I4 * V(1, j, 0);
+        const double uy_xp = I1 * V(-2, jp, 0) + I2 * V(-1, jp, 0) + I3 * V(0, jp, 0) + I4 * V(1, jp, 0);
+        const double uzCell = I1 * W(0, j, -1) + I2 * W(0, j, 0) + I3 * W(0, j, 1) + I4 * W(0, j, 2);
+        const double prloc2 = IY1 * (I1 * Pp(-2, jm, 0) + I2 * Pp(-1, jm, 0) + I3 * Pp(0, jm, 0) + I4 * Pp(1, jm, 0)) +
+                              IY2 * (I1 * Pp(-2, j, 0) + I2 * Pp(-1, j, 0) + I3 * Pp(0, j, 0) + I4 * Pp(1, j, 0));
+        const double dudx = g.dxc1 * U(-1, j, 0) + g.dxc2 * U(0, j, 0) + g.dxc3 * U(1, j, 0) + g.dxc4 * U(2, j, 0);
+        const double dudy = (U(0, j, 0) - U(0, jm, 0)) * cac;
+        const double dudz = g.dzc1 * U(0, j, -2) + g.dzc2 * U(0, j, -1) + g.dzc3 * U(0, j, 0) + g.dzc4 * U(0, j, 1);
+        const double dvdx = g.dxc1 * V(-2, j, 0) + g.dxc2 * V(-1, j, 0) + g.dxc3 * V(0, j, 0) + g.dxc4 * V(1, j, 0);
+        const double dvdy = (V(0, jp, 0) - uyloc) * caj;
+        const double dvdz = g.dzc1 * V(0, j, -2) + g.dzc2 * V(0, j, -1) + g.dzc3 * V(0, j, 0) + g.dzc4 * V(0, j, 1);
+        const double dwdx = g.dxc1 * W(-2, j, 0) + g.dxc2 * W(-1, j, 0) + g.dxc3 * W(0, j, 0) + g.dxc4 * W(1, j, 0);
+        const double dwdy = (uzloc - W(0, jm, 0)) * cac;
+        const double dwdz = g.dzc1 * W(0, j, -1) + g.dzc2 * W(0, j, 0) + g.dzc3 * W(0, j, 1) + g.dzc4 * W(0, j, 2);
+        auto dxx = [&](auto F) {
+          return g.dxxc1 * F(-2, j, 0) + g.dxxc2 * F(-1, j, 0) + g.dxxc3 * F(0, j, 0) + g.dxxc4 * F(1, j, 0) +
+                 g.dxxc5 * F(2, j, 0);
+        };
+        auto dzz = [&](auto F) {
+          return g.dzzc1 * F(0, j, -2) + g.dzzc2 * F(0, j, -1) + g.dzzc3 * F(0, j, 0) + g.dzzc4 * F(0, j, 1) +
+                 g.dzzc5 * F(0, j, 2);
+        };
+        const double dudxx = dxx(U), dudzz = dzz(U);
+        const double dudyy = ap2c * U(0, jp, 0) + ac2c * U(0, j, 0) + am2c * U(0, jm, 0);
+        const double dvdxx = dxx(V), dvdzz = dzz(V);
+        const double dvdyy = ap2p * V(0, jp, 0) + ac2p * V(0, j, 0) + am2p * V(0, jm, 0);
+        const double dwdxx = dxx(W), dwdzz = dzz(W);
+        const double dwdyy = ap2c * W(0, jp, 0) + ac2c * W(0, j, 0) + am2c * W(0, jm, 0);
+        const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+        const double dudxC = (-U(2, j, 0) + 8.0 * U(1, j, 0) - 8.0 * U(-1, j, 0) + U(-2, j, 0)) * rdxt;
+        const double dvdxU = g.dxc1 * V(-2, jp, 0) + g.dxc2 * V(-1, jp, 0) + g.dxc3 * V(0, jp, 0) + g.dxc4 * V(1, jp, 0);
+        const double dudyU = (U(0, jp, 0) - U(0, j, 0)) * cacU;
+        const double dudzC = (IY1 * (-U(0, jm, 2) + 8.0 * U(0, jm, 1) - 8.0 * U(0, jm, -1) + U(0, jm, -2)) +
+                              IY2 * (-U(0, j, 2) + 8.0 * U(0, j, 1) - 8.0 * U(0, j, -1) + U(0, j, -2))) *
+                             rdzt;
+        auto c4z = [&](int di) { return -V(di, j, 2) + 8.0 * V(di, j, 1) - 8.0 * V(di, j, -1) + V(di, j, -2); };
+        const double dvdzC = (c4z(-2) * I1 + c4z(-1) * I2 + c4z(0) * I3 + c4z(1) * I4) * rdzt;
+        sv[0] += uxloc;
+        sv[1] += uyloc;
+        sv[2] += uzloc;
+        sv[3] += prloc;
+        sv[4] += uxloc * uxloc;
+        sv[5] += uyloc * uyloc;
+        sv[6] += uzloc * uzloc;
+        sv[7] += prloc * prloc;
+        sv[8] += uxCell * uyCell;
+        sv[9] += uyCell * uzCell;
+        sv[10] += uxCell * uzCell;
+        sv[11] += uxCell * prloc;
+        sv[12] += uyCell * prloc;
+        sv[13] += uzCell * prloc;
+        sv[14] += uxCell * uxCell * uyCell;
+        sv[15] += uyloc * uyloc * uyloc;
+        sv[16] += uzCell * uzCell * uyCell;
+        sv[17] += uxCell * uyCell * uyCell;
+        sv[18] += uxloc * uxloc * uxloc;
+        sv[19] += uzloc * uzloc * uzloc;
+        sv[20] += uxloc * uxloc * uxloc * uxloc;
+        sv[21] += uyloc * uyloc * uyloc * uyloc;
+        sv[22] += uzloc * uzloc * uzloc * uzloc;
+        sv[23] += prloc * dudx;
+        sv[24] += prloc * dvdy;
+        sv[25] += prloc * dwdz;
+        sv[26] += prloc2 * (dudy + dvdx);
+        sv[27] += uxloc * (dudxx + dudyy + dudzz);
+        sv[28] += uyloc * (dvdxx + dvdyy + dvdzz);
+        sv[29] += uzloc * (dwdxx + dwdyy + dwdzz);
+        sv[30] += (dudxC * (dvdxU + dvdx) + (dudyU + dudy) * (uy_xp - uy_xc) * caj) * 0.5;
+        sv[31] += dudzC * dvdzC;
+        sv[32] += vor_x * vor_x;
+        sv[33] += vor_y * vor_y;
+        sv[34] += vor_z * vor_z;
+        if constexpr (SC) {
+          const double scloc = n(sc, 0, j, 0);
+          sv[35] += scloc;
+          sv[36] += scloc * scloc;
+          sv[37] += scloc * uxCell;
+          sv[38] += scloc * uyCell;
+          sv[39] += scloc * uzCell;
+        }
+      }
+  } else {
+    const int jm = j - 1;
+    for (int k = k0; k < k1; ++k)
+      for (int i = threadIdx.x; i < g.nx; i += 256) {
+        const Ngh n(g, i, k);
+        auto Pp = [&](int di, int jj) { return n(pr, di, jj, 0); };
+        const double prloc2 = IY1 * (I1 * Pp(-2, jm) + I2 * Pp(-1, jm) + I3 * Pp(0, jm) + I4 * Pp(1, jm)) +
+                              IY2 * (I1 * Pp(-2, j) + I2 * Pp(-1, j) + I3 * Pp(0, j) + I4 * Pp(1, j));
+        const double dudy = (n(ux, 0, j, 0) - n(ux, 0, jm, 0)) * cac;
+        const double dwdy = (n(uz, 0, j, 0) - n(uz, 0, jm, 0)) * cac;
+        sv[26] += prloc2 * (dudy + 0.0);
+        sv[32] += dwdy * dwdy;
+        sv[34] += dudy * dudy;
+      }
+  }
+  double *out = part + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * NS;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const double s = block_sum(sv[q], sh);
+    if (threadIdx.x == 0) out[q] = s;
+    __syncthreads();
+  }
+}
+
+// SumStat(q, j) += inxz * (sum over chunks in order); one thread per (q, j)
+__global__ void __launch_bounds__(256) k_stat_final(const double *__restrict__ part, int nchunk, int ns, int nyp,
+                                                    double inxz, double *__restrict__ sumstat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ns * nyp) return;
+  const int q = t % ns, j = t / ns;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[((long long)j * nchunk + c) * ns + q];
+  sumstat[t] += s * inxz;
+}
+
+// PrGradsum += PrGradAver (:416), on the device
+__global__ void k_stat_prgrad(double *scal) { scal[S_PRGSUM] += scal[S_PRGRAD]; }
+
+int launch_stat(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                const double *sc) {
+  const int ns = c->prm.scalar ? 40 : 35, nyp = c->ny + 1;
+  if (!c->d_stat) {
+    PTC_HIP(hipMalloc(&c->d_stat, sizeof(double) * (size_t)ns * nyp));
+    PTC_TRY(fill_device(c, c->d_stat, (long long)ns * nyp, 0.0));
+  }
+  // chunks of k planes: ~8 blocks per CU over all planes
+  const int want = std::max(1, (c->n_cu * 8) / nyp);
+  const int nchunk = std::min(c->nz, want);
+  const int kc = (c->nz + nchunk - 1) / nchunk;
+  const int nblk = (c->nz + kc - 1) / kc;
+  PTC_TRY(ensure_red(c, nblk * nyp * ns));
+  if (c->prm.scalar)
+    hipLaunchKernelGGL(k_stat_partial<true>, dim3(nblk, nyp), dim3(256), 0, c->stream, c->geo, c->prm.ucrf, ux, uy,
+                       uz, pr, sc, kc, c->d_red);
+  else
+    hipLaunchKernelGGL(k_stat_partial<false>, dim3(nblk, nyp), dim3(256), 0, c->stream, c->geo, c->prm.ucrf, ux, uy,
+                       uz, pr, sc, kc, c->d_red);
+  PTC_HIP(hipGetLastError());
+  const double inxz = 1.0 / ((double)c->nx * (double)c->nz);
+  hipLaunchKernelGGL(k_stat_final, dim3((ns * nyp + 255) / 256), dim3(256), 0, c->stream, c->d_red, nblk, ns, nyp,
+                     inxz, c->d_stat);
+  PTC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_stat_prgrad, dim3(1), dim3(1), 0, c->stream, c->d_scal);
+  PTC_HIP(hipGetLastError());
+  ++c->nfstime;
+  return PTC_OK;
+}
+
+int read_stat(ptc_ctx *c, double *out, double *prgradsum, int reset) {
+  const int ns = c->prm.scalar ? 40 : 35, nyp = c->ny + 1;
+  if (out) {
+    if (c->d_stat) {
+      PTC_HIP(hipMemcpyAsync(out, c->d_stat, sizeof(double) * (size_t)ns * nyp, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      std::fill(out, out + (size_t)ns * nyp, 0.0);
+    }
+  }
+  PTC_HIP(hipMemcpyAsync(c->h_pinned + 3, c->d_scal + S_PRGSUM, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PTC_HIP(hipStreamSynchronize(c->stream));
+  if (prgradsum) *prgradsum = c->h_pinned[3];
+  if (reset) {
+    // src/m_FlowCase.f90:944: nfstime = 0; SumStat = 0; PrGradsum = 0
+    if (c->d_stat) PTC_TRY(fill_device(c, c->d_stat, (long long)ns * nyp, 0.0));
+    PTC_TRY(fill_device(c, c->d_scal + S_PRGSUM, 1, 0.0));
+    c->nfstime = 0;
+  }
   return PTC_OK;
 }
 

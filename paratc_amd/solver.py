@@ -231,6 +231,22 @@ class Solver:
         check(self.L.ptc_channel_iterate(self.h, nsteps, _dp(mon)), "ChannelIterate")
         return mon
 
+    # ---- m_FlowCase statistics ------------------------------------------------
+    def clcStat(self, ux=capi.F_UX, uy=capi.F_UY, uz=capi.F_UZ, pressure=capi.F_PR, scalar=None):
+        """Accumulate one statistics sample (clcStat, src/m_FlowCase.f90:283-428)."""
+        if scalar is None:
+            scalar = capi.F_SCALAR if self.case.scalar else -1
+        check(self.L.ptc_clc_stat(self.h, ux, uy, uz, pressure, scalar), "clcStat")
+
+    def get_stat(self, reset: bool = False):
+        """(SumStat as (NCHASTAT, ny+1), nfstime, PrGradsum)."""
+        ns = 40 if self.case.scalar else 35
+        out = np.zeros((ns, self.ny + 1), order="F")
+        n, nf, pg = C.c_int(), C.c_int(), C.c_double()
+        check(self.L.ptc_get_stat(self.h, _dp(out), C.byref(n), C.byref(nf), C.byref(pg), 1 if reset else 0),
+              "ptc_get_stat")
+        return out, nf.value, pg.value
+
     def scalar(self, which: int) -> float:
         d = C.c_double()
         check(self.L.ptc_get_scalar(self.h, which, C.byref(d)), "ptc_get_scalar")

@@ -138,3 +138,16 @@ def test_fortran_binding_names_declared_symbols():
     names = set(re.findall(r'bind\(C,\s*name="(ptc_[a-z0-9_]+)"\)', text))
     assert len(names) >= 30
     assert names <= set(header_symbols()), names - set(header_symbols())
+
+
+def test_stats_file_format(tmp_path):
+    from paratc_amd import stats
+
+    st = np.arange(35 * 3, dtype=float).reshape((35, 3), order="F") - 7.5
+    path = str(tmp_path / "s")
+    stats.write_stats(path, 400, 4, 100, True, -0.0123, st)
+    lines = open(path).read().splitlines()
+    assert lines[0] == "  The time step range for this fluid statistics is     100:    100:    400"
+    assert lines[3] == "    time averaged pressure gradient is:    3.075000000000000E-03"
+    assert len(lines) == 5 + 3 and all(len(ln) == 24 * 35 for ln in lines[5:])
+    assert float(lines[5][:24]) == -7.5 / 4 and stats.stats_name("r/", 7) == "r/stats0000000007"
