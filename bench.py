@@ -298,6 +298,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rk", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage + refinement leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -308,7 +309,7 @@ def main():
     ev = HipEvents()
     r = bench_poisson(args, ev)
     case = r["case"]
-    fp32 = bench_fp32_refine(r, ev)
+    fp32 = None if args.no_fp32 else bench_fp32_refine(r, ev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
     ens = None if args.no_ensemble else bench_ensemble()
