@@ -171,13 +171,18 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *   PTC_POISSON_SLAB_PLANES run the x and z passes slab by slab of this many
  *                           y planes (0 = whole field; specialised FFT
  *                           kernels only), so each z pass finds its input in
- *                           the last-level cache. */
-enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2 };
+ *                           the last-level cache.
+ *   PTC_POISSON_TRI_LINES   0: the partitioned y-solve loads its rows straight
+ *                           into registers (64 real lines per workgroup);
+ *                           16 or 32: it stages that many lines through LDS
+ *                           in smaller workgroups. */
+enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2, PTC_POISSON_TRI_LINES = 3 };
 int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
-/* the plan in use: out9 = [prec, tri, z tile, sweeps, specialised-kernel
+/* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per
- * workgroup, slab planes] (after autotuning; chunk fields are 0 for Thomas) */
-int ptc_poisson_get_config(ptc_ctx *ctx, int *out9);
+ * workgroup, slab planes, LDS-staged lines (0 = register kernel)] (after
+ * autotuning; chunk fields are 0 for Thomas) */
+int ptc_poisson_get_config(ptc_ctx *ctx, int *out10);
 /* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
 /* Per-stage kernel times: with enable = 1 every later solve records HIP events

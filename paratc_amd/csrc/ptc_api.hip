@@ -87,7 +87,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   // FFT kernels (specialised or a generic z tile) with the default y-solve,
   // then the y-solve (partitioned chunk length, or Thomas) with that FFT.
   struct Cand {
-    int tri, mm, tile, slab;
+    int tri, mm, tile, slab, ll;
   };
   auto run = [&](const Cand &cd, double *ms) -> int {
     poisson_free_plan(c);
@@ -95,13 +95,14 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     c->pp.tri_mm_req = cd.mm;
     c->pp.ztile_req = cd.tile;
     c->pp.slab_req = cd.slab;
+    c->pp.tri_ll_req = cd.ll;
     if (poisson_build_plan(c) != PTC_OK) return PTC_E_UNSUPPORTED;
     if (cd.tile > 0 && c->pp.ztile != cd.tile) return PTC_E_UNSUPPORTED;  // clipped: covered elsewhere
     if (cd.slab > 0 && c->pp.slab != cd.slab) return PTC_E_UNSUPPORTED;
     return time_solve(c, s, d, 3, ms);
   };
   double best = 1e300;
-  Cand pick{PTC_TRI_AUTO, 0, 0, 0};
+  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0};
   auto consider = [&](const Cand &cd) {
     double ms = 0;
     if (run(cd, &ms) == PTC_OK && ms < best) {
@@ -109,19 +110,21 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
       pick = cd;
     }
   };
-  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0});
+  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0});
   const int tile = pick.tile;
-  for (int mm : {4, 8, 12, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0});
-  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0});
+  for (int mm : {4, 8, 12, 16})
+    for (int ll : {0, 16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll});
+  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0});
   if (tile == 0) {
     const Cand base = pick;
-    for (int slab : {64, 32, 16, 8}) consider(Cand{base.tri, base.mm, 0, slab});
+    for (int slab : {64, 32, 16, 8}) consider(Cand{base.tri, base.mm, 0, slab, base.ll});
   }
   poisson_free_plan(c);
   c->pp.tri_req = pick.tri;
   c->pp.tri_mm_req = pick.mm;
   c->pp.ztile_req = pick.tile;
   c->pp.slab_req = pick.slab;
+  c->pp.tri_ll_req = pick.ll;
   PTC_TRY(poisson_build_plan(c));
   // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
   PTC_TRY(fill_device(c, s, (long long)c->n3, 0.0));
@@ -429,13 +432,16 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
               "chunk rows must be 0, 4, 8, 12 or 16");
   } else if (knob == PTC_POISSON_SLAB_PLANES) {
     PTC_CHECK(value >= 0, PTC_E_ARG, "slab planes must be >= 0");
+  } else if (knob == PTC_POISSON_TRI_LINES) {
+    PTC_CHECK(value == 0 || value == 16 || value == 32, PTC_E_ARG, "tridiagonal lines per workgroup must be 0, 16 or 32");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
   PTC_HIP(hipStreamSynchronize(ctx->stream));
   poisson_free_plan(ctx);
   if (knob == PTC_POISSON_CHUNK_ROWS) ctx->pp.tri_mm_req = value;
-  else ctx->pp.slab_req = value;
+  else if (knob == PTC_POISSON_SLAB_PLANES) ctx->pp.slab_req = value;
+  else ctx->pp.tri_ll_req = value;
   return poisson_build_plan(ctx);
 }
 
@@ -452,6 +458,7 @@ int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
   o[6] = part ? ctx->pp.tri_chunks : 0;
   o[7] = part ? ctx->pp.tri_lpb : 0;
   o[8] = ctx->pp.slab;
+  o[9] = part ? ctx->pp.tri_ll : 0;
   return PTC_OK;
 }
 

@@ -30,6 +30,7 @@ namespace ptc {
 
 bool fft_config_ok(const FftLen &L, int ncol, int nthr);
 int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out);
+int tri_plan_lds(int ny, size_t lds_limit, int ll, int mm_req, int *P_out, int *mm_out);
 
 static void make_twiddles(int n, std::vector<double2> &out) {
   const double twopi = 6.283185307179586476925286766559005768;
@@ -54,6 +55,7 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.tri_req = P.tri_req;
   keep.tri_mm_req = P.tri_mm_req;
   keep.slab_req = P.slab_req;
+  keep.tri_ll_req = P.tri_ll_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.profile = P.profile;
@@ -116,7 +118,12 @@ int poisson_build_plan(ptc_ctx *c) {
   // y pass: partitioned unless Thomas is asked for, or (automatic choice) no
   // partitioned shape fits this ny
   P.tri = P.tri_req == PTC_TRI_THOMAS ? PTC_TRI_THOMAS : PTC_TRI_PARTITIONED;
-  if (P.tri == PTC_TRI_PARTITIONED) {
+  P.tri_ll = 0;
+  if (P.tri == PTC_TRI_PARTITIONED && P.tri_ll_req > 0) {
+    PTC_TRY(tri_plan_lds(c->ny, c->lds_limit, P.tri_ll_req, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax));
+    P.tri_ll = P.tri_ll_req;
+    P.tri_lpb = P.tri_ll;
+  } else if (P.tri == PTC_TRI_PARTITIONED) {
     const int rc = tri_plan(c->ny, c->lds_limit, true, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb);
     if (rc != PTC_OK) {
       if (P.tri_req == PTC_TRI_PARTITIONED || P.prec == PTC_PREC_FP32_REFINE) return rc;

@@ -23,6 +23,8 @@
 // The (0,0) line has row j=1 replaced by phi=0 (src/m_Poisson.f90:399-404).
 // Crank-Nicolson lines (i, k) of clcU1Hat/U2Hat/U3Hat/clcScalar
 // (src/m_TScheme.f90:365-506) are real with j-only coefficients.
+#include <algorithm>
+
 #include "ptc_internal.hpp"
 
 namespace ptc {
@@ -263,6 +265,67 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(S *__restrict__ spec
   }
 }
 
+// ---- Poisson y-solve, partitioned, staged through LDS ----------------------
+// The same elimination for LL real lines per workgroup: the lines' rows are
+// loaded once into an LDS tile with coalesced LL-wide row reads, each thread
+// (chunk p, line l) takes its chunk from the tile, and the results go back
+// out the same way.  With LL = 16 or 32 a workgroup is small (P * LL
+// threads) and several fit on a CU, so one workgroup's loads overlap another's
+// elimination.  The row coefficients are copied to LDS as well; the cyclic
+// reduction's buffers reuse the tile.
+template <int MM, int LL, typename S>
+__global__ void __launch_bounds__(512) k_tri_poisson_lds(S *__restrict__ spec, const double *__restrict__ met, int nyh,
+                                                         const double *__restrict__ wnx,
+                                                         const double *__restrict__ wnz, int ny, int pitch, int h,
+                                                         int P) {
+  extern __shared__ double smem[];
+  constexpr int LP = LL + 1;  // padded tile row
+  double *mA = smem, *mC = mA + nyh, *mP = mC + nyh;
+  double *tile = mP + nyh;
+  const int t = threadIdx.x, nthr = blockDim.x;
+  for (int j = t; j < nyh; j += nthr) {
+    mA[j] = met[M_AM2PH * nyh + j];
+    mC[j] = met[M_AC2PH * nyh + j];
+    mP[j] = met[M_AP2PH * nyh + j];
+  }
+  using R = decltype(spec->x);
+  const long long rs = 2LL * pitch;
+  const int col0 = blockIdx.x * LL, ncol = 2 * (h + 1), kz = blockIdx.y;
+  R *base = reinterpret_cast<R *>(spec + (long long)kz * ny * pitch) + col0;
+  for (int e = t; e < ny * LL; e += nthr) {
+    const int r = e / LL, q = e - r * LL;
+    tile[r * LP + q] = (col0 + q < ncol) ? (double)base[r * rs + q] : 0.0;
+  }
+  __syncthreads();
+  const int line = t % LL, p = t / LL;
+  const int s = p * MM;
+  double f[MM];
+#pragma unroll
+  for (int r = 0; r < MM; ++r) f[r] = (s + r < ny) ? tile[(s + r) * LP + line] : 0.0;
+  const int kx = (col0 + line) >> 1;
+  const bool valid = col0 + line < ncol;
+  const double lam = valid ? wnx[kx] + wnz[kz] : 1.0;
+  const bool pin = (kx == 0 && kz == 0);
+  if (pin && s == 0) f[0] = 0.0;  // phi(1) = 0 on the (0,0) line
+  PoissonRowsRe getc{mA, mC, mP, lam, ny, pin};
+  double A[MM], C[MM];
+  double aS, cS, aE, cE, fS, fE, xs, xe;
+  chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
+  __syncthreads();  // every chunk has left the tile: it becomes the reduction's buffers
+  double *RA = tile, *RC = RA + 4 * P * LL, *RF = RC + 4 * P * LL;
+  solve_reduced<LL>(aS, cS, fS, aE, cE, fE, P, p, line, RA, RC, RF, xs, xe);
+  chunk_backsub<MM>(f, A, C, xs, xe);
+  __syncthreads();  // the reduction's buffers are read: back to the tile
+#pragma unroll
+  for (int r = 0; r < MM; ++r)
+    if (s + r < ny) tile[(s + r) * LP + line] = f[r];
+  __syncthreads();
+  for (int e = t; e < ny * LL; e += nthr) {
+    const int r = e / LL, q = e - r * LL;
+    if (col0 + q < ncol) base[r * rs + q] = (R)tile[r * LP + q];
+  }
+}
+
 // ---- Poisson y-solve, Thomas (reference InverseTridiagonal) --------------
 __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict__ spec, double *__restrict__ cs,
                                                            const double *__restrict__ met, int nyh,
@@ -379,6 +442,30 @@ static int tri_choose(int ny, size_t lds_limit, int mm_req, const TriShape *shap
   return PTC_E_UNSUPPORTED;
 }
 
+// LDS-staged Poisson kernel: LL lines per workgroup, P * LL <= 512 threads,
+// LDS = row coefficients + max(tile, reduction buffers)
+size_t tri_lds_bytes(int ny, int P, int ll) {
+  const size_t tile = (size_t)ny * (ll + 1), red = (size_t)12 * P * ll;
+  return sizeof(double) * (3 * (size_t)(ny + 2) + std::max(tile, red));
+}
+
+int tri_plan_lds(int ny, size_t lds_limit, int ll, int mm_req, int *P_out, int *mm_out) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int mm : {4, 8, 12, 16}) {
+      if (mm_req > 0 && mm != mm_req) continue;
+      const int P = (ny + mm - 1) / mm;
+      if (P * ll > 512) continue;
+      if (pass == 0 && (P * mm - ny) * 8 > ny) continue;
+      if (tri_lds_bytes(ny, P, ll) > lds_limit) continue;
+      *P_out = P;
+      *mm_out = mm;
+      return PTC_OK;
+    }
+  set_error("ny = " + std::to_string(ny) + " does not fit the LDS-staged tridiagonal kernel with " +
+            std::to_string(ll) + " lines per workgroup");
+  return PTC_E_UNSUPPORTED;
+}
+
 int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out) {
   return poisson ? tri_choose(ny, lds_limit, mm_req, kPoissonShapes, 4, P_out, mm_out, lpb_out)
                  : tri_choose(ny, lds_limit, mm_req, kCnShapes, 4, P_out, mm_out, lpb_out);
@@ -395,6 +482,33 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
                        c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, c->nz);
     PTC_HIP(hipGetLastError());
     return PTC_OK;
+  }
+  if (P.tri_ll > 0) {
+    const int nP = P.tri_chunks, ll = P.tri_ll;
+    dim3 grid((unsigned)((2 * (P.h + 1) + ll - 1) / ll), (unsigned)c->nz);
+    const size_t lds = tri_lds_bytes(c->ny, nP, ll);
+#define PTC_LDS(MM, LL)                                                                                        \
+  if (P.tri_mmax == MM && ll == LL) {                                                                          \
+    if (f32)                                                                                                   \
+      hipLaunchKernelGGL((k_tri_poisson_lds<MM, LL, float2>), grid, dim3(LL * nP), lds, c->stream,             \
+                         (float2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);    \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_tri_poisson_lds<MM, LL, double2>), grid, dim3(LL * nP), lds, c->stream,            \
+                         (double2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);   \
+    PTC_HIP(hipGetLastError());                                                                                \
+    return PTC_OK;                                                                                             \
+  }
+    PTC_LDS(4, 16)
+    PTC_LDS(8, 16)
+    PTC_LDS(12, 16)
+    PTC_LDS(16, 16)
+    PTC_LDS(4, 32)
+    PTC_LDS(8, 32)
+    PTC_LDS(12, 32)
+    PTC_LDS(16, 32)
+#undef PTC_LDS
+    set_error("bad LDS tridiagonal launch shape");
+    return PTC_E_STATE;
   }
   const int nP = P.tri_chunks, lpb = P.tri_lpb;
   // lpb lanes per block = lpb/2 complex lines

@@ -53,6 +53,7 @@ module m_paratc_gpu
   integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1
   integer(c_int), parameter, public :: PTC_TRI_AUTO = 0, PTC_TRI_THOMAS = 1, PTC_TRI_PARTITIONED = 2
   integer(c_int), parameter, public :: PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2
+  integer(c_int), parameter, public :: PTC_POISSON_TRI_LINES = 3
 
   interface
     integer(c_int) function ptc_create(grid, prm, yp, ctx) bind(C, name="ptc_create")
@@ -229,10 +230,10 @@ module m_paratc_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: knob, val
     end function
-    integer(c_int) function ptc_poisson_get_config(ctx, out9) bind(C, name="ptc_poisson_get_config")
+    integer(c_int) function ptc_poisson_get_config(ctx, out10) bind(C, name="ptc_poisson_get_config")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), intent(out) :: out9(9)
+      integer(c_int), intent(out) :: out10(10)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr

@@ -126,13 +126,17 @@ class Solver:
     def poisson_slab_planes(self, planes: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_SLAB_PLANES, planes), "ptc_poisson_set")
 
+    def poisson_tri_lines(self, lines: int = 0):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_LINES, lines), "ptc_poisson_set")
+
     def poisson_get_config(self) -> dict:
-        o = (C.c_int * 9)()
+        o = (C.c_int * 10)()
         check(self.L.ptc_poisson_get_config(self.h, o), "ptc_poisson_get_config")
         tri = {capi.TRI_THOMAS: "thomas", capi.TRI_PARTITIONED: "partitioned"}.get(o[1], str(o[1]))
         return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
                 "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2),
-                "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7], "slab_planes": o[8]}
+                "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7], "slab_planes": o[8],
+                "tri_lds_lines": o[9]}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()

@@ -30,7 +30,7 @@ GRIDS = [
 ]
 
 
-def solve_both(grid, tri, seed=1, compat=True, rows=0):
+def solve_both(grid, tri, seed=1, compat=True, rows=0, lines=0):
     nx, ny, nz, gam = grid
     case = CS.case_poisson(nx, ny, nz, gam)
     f = ic.poisson_rhs(case, seed=seed) if compat else np.asfortranarray(
@@ -43,10 +43,13 @@ def solve_both(grid, tri, seed=1, compat=True, rows=0):
     lnorm = float((np.abs(am) + np.abs(ac) + np.abs(ap)).max() + (1460 + 2 * 838) / 576 * (rdx2 + rdz2))
     with Solver(case) as s:
         s.poisson_config(capi.PREC_FP64, tri, 0, 0)
+        if lines:
+            s.poisson_tri_lines(lines)
         if rows:
             s.poisson_chunk_rows(rows)
             cfg = s.poisson_get_config()
             assert cfg["tri"] == "partitioned" and cfg["chunk_rows"] == rows, cfg
+            assert cfg["tri_lds_lines"] == lines, cfg
         s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         out = s.download(capi.F_HALO, capi.L_INTERIOR)
@@ -100,6 +103,16 @@ def test_slabbed_xz_passes_match_oracle(slab):
     ref = orc.halo()
     orc.clcPPE(f.copy(order="F"), ref, 1)
     assert rel_err(out, inner(ref)) <= TOL
+
+
+@pytest.mark.parametrize("lines", [16, 32])
+@pytest.mark.parametrize("ny,rows", [(64, 4), (60, 8), (50, 12), (256, 16), (96, 12)])
+def test_lds_staged_y_solve_matches_oracle(lines, ny, rows):
+    """The LDS-staged partitioned kernel (16 or 32 real lines per workgroup)."""
+    grid = (32, ny, 24, 1.9)
+    f, ref, out, _, res = solve_both(grid, capi.TRI_PARTITIONED, seed=2, rows=rows, lines=lines)
+    assert rel_err(out, ref) <= TOL
+    assert res <= 1e-13 * np.abs(out).max()
 
 
 def test_automatic_variant_falls_back_to_thomas():
@@ -201,9 +214,10 @@ def test_specialised_fft_kernels_match_generic_and_oracle(grid):
 
 
 # ---- FP32 storage + FP64 defect correction ---------------------------------
-@pytest.mark.parametrize("grid,slab", [((64, 48, 32, 2.0), 0), ((128, 96, 128, 1.8), 0),
-                                       ((128, 96, 128, 1.8), 16), ((96, 140, 64, 2.2), 0)])
-def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab):
+@pytest.mark.parametrize("grid,slab,lines", [((64, 48, 32, 2.0), 0, 0), ((128, 96, 128, 1.8), 0, 0),
+                                             ((128, 96, 128, 1.8), 16, 0), ((96, 140, 64, 2.2), 0, 0),
+                                             ((128, 96, 128, 1.8), 0, 16)])
+def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab, lines):
     """Spectra stored in FP32 between passes (FP64 arithmetic); each sweep
     adds the solve of the FP64 residual f - L phi."""
     nx, ny, nz, gam = grid
@@ -220,6 +234,8 @@ def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab):
             s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, sweeps)
             if slab:
                 s.poisson_slab_planes(slab)
+            if lines:
+                s.poisson_tri_lines(lines)
             cfg = s.poisson_get_config()
             assert cfg["prec"] == "fp32+refine" and cfg["sweeps"] == sweeps, cfg
             s.clcPPE(capi.F_ARR1, capi.F_HALO)
