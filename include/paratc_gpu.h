@@ -175,8 +175,12 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *   PTC_POISSON_TRI_LINES   0: the partitioned y-solve loads its rows straight
  *                           into registers (64 real lines per workgroup);
  *                           16 or 32: it stages that many lines through LDS
- *                           in smaller workgroups. */
-enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2, PTC_POISSON_TRI_LINES = 3 };
+ *                           in smaller workgroups.
+ *   PTC_POISSON_FAST_ZTILE  column tile of the specialised z kernel (0 = its
+ *                           default; 16 for nz = 512 and 32 for nz = 256 are
+ *                           the wider compiled alternatives).  A tile with no
+ *                           compiled kernel falls back to the generic z pass. */
+enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2, PTC_POISSON_TRI_LINES = 3, PTC_POISSON_FAST_ZTILE = 4 };
 int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
 /* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per

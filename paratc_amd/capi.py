@@ -27,7 +27,7 @@ PTC_E_LIMIT = -6
 L_HALO, L_INTERIOR, L_PLANE = 0, 1, 2
 PREC_FP64, PREC_FP32_REFINE = 0, 1
 TRI_AUTO, TRI_THOMAS, TRI_PARTITIONED = 0, 1, 2
-POISSON_CHUNK_ROWS, POISSON_SLAB_PLANES, POISSON_TRI_LINES = 1, 2, 3
+POISSON_CHUNK_ROWS, POISSON_SLAB_PLANES, POISSON_TRI_LINES, POISSON_FAST_ZTILE = 1, 2, 3, 4
 S_DIVMAX_PRSRC, S_PRGRADAVER, S_SIMTIME, S_DT = 0, 1, 2, 3
 
 # every symbol declared in include/paratc_gpu.h

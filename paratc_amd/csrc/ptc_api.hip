@@ -87,7 +87,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   // FFT kernels (specialised or a generic z tile) with the default y-solve,
   // then the y-solve (partitioned chunk length, or Thomas) with that FFT.
   struct Cand {
-    int tri, mm, tile, slab, ll;
+    int tri, mm, tile, slab, ll, zf;
   };
   auto run = [&](const Cand &cd, double *ms) -> int {
     poisson_free_plan(c);
@@ -96,13 +96,15 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     c->pp.ztile_req = cd.tile;
     c->pp.slab_req = cd.slab;
     c->pp.tri_ll_req = cd.ll;
+    c->pp.zfast_req = cd.zf;
     if (poisson_build_plan(c) != PTC_OK) return PTC_E_UNSUPPORTED;
+    if (cd.zf > 0 && !(c->pp.z_fast && c->pp.ztile == cd.zf)) return PTC_E_UNSUPPORTED;
     if (cd.tile > 0 && c->pp.ztile != cd.tile) return PTC_E_UNSUPPORTED;  // clipped: covered elsewhere
     if (cd.slab > 0 && c->pp.slab != cd.slab) return PTC_E_UNSUPPORTED;
     return time_solve(c, s, d, 3, ms);
   };
   double best = 1e300;
-  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0};
+  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0};
   auto consider = [&](const Cand &cd) {
     double ms = 0;
     if (run(cd, &ms) == PTC_OK && ms < best) {
@@ -110,14 +112,15 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
       pick = cd;
     }
   };
-  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0});
-  const int tile = pick.tile;
+  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0});
+  for (int zf : {16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf});
+  const int tile = pick.tile, zf = pick.zf;
   for (int mm : {4, 8, 12, 16})
-    for (int ll : {0, 16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll});
-  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0});
+    for (int ll : {0, 16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll, zf});
+  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0, zf});
   if (tile == 0) {
     const Cand base = pick;
-    for (int slab : {64, 32, 16, 8}) consider(Cand{base.tri, base.mm, 0, slab, base.ll});
+    for (int slab : {64, 32, 16, 8}) consider(Cand{base.tri, base.mm, 0, slab, base.ll, base.zf});
   }
   poisson_free_plan(c);
   c->pp.tri_req = pick.tri;
@@ -125,6 +128,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   c->pp.ztile_req = pick.tile;
   c->pp.slab_req = pick.slab;
   c->pp.tri_ll_req = pick.ll;
+  c->pp.zfast_req = pick.zf;
   PTC_TRY(poisson_build_plan(c));
   // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
   PTC_TRY(fill_device(c, s, (long long)c->n3, 0.0));
@@ -434,6 +438,8 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
     PTC_CHECK(value >= 0, PTC_E_ARG, "slab planes must be >= 0");
   } else if (knob == PTC_POISSON_TRI_LINES) {
     PTC_CHECK(value == 0 || value == 16 || value == 32, PTC_E_ARG, "tridiagonal lines per workgroup must be 0, 16 or 32");
+  } else if (knob == PTC_POISSON_FAST_ZTILE) {
+    PTC_CHECK(value >= 0 && value <= 32, PTC_E_ARG, "z tile must be in 0..32");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
@@ -441,7 +447,8 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   poisson_free_plan(ctx);
   if (knob == PTC_POISSON_CHUNK_ROWS) ctx->pp.tri_mm_req = value;
   else if (knob == PTC_POISSON_SLAB_PLANES) ctx->pp.slab_req = value;
-  else ctx->pp.tri_ll_req = value;
+  else if (knob == PTC_POISSON_TRI_LINES) ctx->pp.tri_ll_req = value;
+  else ctx->pp.zfast_req = value;
   return poisson_build_plan(ctx);
 }
 

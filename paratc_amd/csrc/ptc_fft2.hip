@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
 
 // ---- z pass: T columns x N rows, thread id = g * T + col ----------------
 template <int N, int T, bool INV, typename S>
-__global__ void __launch_bounds__(512) k2_fft_z(S *__restrict__ spec, const double2 *__restrict__ tw, int ny,
+__global__ void __launch_bounds__(f2_threads<N>() * T) k2_fft_z(S *__restrict__ spec, const double2 *__restrict__ tw, int ny,
                                                  int pitch, int j0) {
   constexpr int G = f2_threads<N>();
   __shared__ double2 sh[N * T];
@@ -492,24 +492,32 @@ int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, 
 }
 
 // z lengths with a specialised kernel and their column tile (<= 64 KB LDS,
-// <= 512 threads)
+// <= 512 threads): the default tile of each length
 #define PTC_Z_SIZES(Z) \
   Z(16, 16) Z(32, 16) Z(64, 16) Z(96, 16) Z(128, 16) Z(192, 16) Z(256, 16) Z(512, 8) Z(700, 4) Z(1400, 2)
+// wider alternatives for the autotuner: 256-byte row segments, one 1024-thread
+// workgroup of 128 KB LDS per CU
+#define PTC_Z_WIDE(Z) Z(256, 32) Z(512, 16)
 
-bool fft2_has_z(int nz, int *tile) {
-#define PTC_CASE(N, T) if (nz == N) { *tile = T; return true; }
+// req = 0: the default tile; else that tile if compiled for nz
+bool fft2_has_z(int nz, int req, int *tile) {
+#define PTC_CASE(N, T) \
+  if (nz == N && (req == 0 || req == T)) { *tile = T; return true; }
   PTC_Z_SIZES(PTC_CASE)
+  PTC_Z_WIDE(PTC_CASE)
 #undef PTC_CASE
   return false;
 }
 
 int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj) {
-  switch (c->nz) {
-#define PTC_CASE(N, T) case N: return z_t<N, T>(c, spec, inverse, j0, nj);
-    PTC_Z_SIZES(PTC_CASE)
+  const int T = c->pp.ztile;
+#define PTC_CASE(N, TT) \
+  if (c->nz == N && T == TT) return z_t<N, TT>(c, spec, inverse, j0, nj);
+  PTC_Z_SIZES(PTC_CASE)
+  PTC_Z_WIDE(PTC_CASE)
 #undef PTC_CASE
-    default: return PTC_E_UNSUPPORTED;
-  }
+  set_error("no specialised z kernel for this length and tile");
+  return PTC_E_UNSUPPORTED;
 }
 
 }  // namespace ptc

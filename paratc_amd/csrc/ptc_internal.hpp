@@ -92,6 +92,7 @@ struct PoissonPlan {
   int pitch = 0;      // complex row pitch of the spectral buffer (>= h + 1)
   int ztile_req = 0;  // requested z tile: 0 = automatic (specialised kernels where built)
   int ztile = 4;      // columns per z-FFT workgroup in use
+  int zfast_req = 0;  // column tile of the specialised z kernel (0 = its default)
   bool x_fast = false, z_fast = false;  // compile-time specialised FFT kernels in use
   int z_threads = 256;
   int x_lines = 4;    // lines per x-FFT workgroup
@@ -184,7 +185,7 @@ int launch_fft_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, bo
 int launch_fft_z(ptc_ctx *c, void *spec, bool inverse);
 int fft_make_len(int n, FftLen *L);
 bool fft2_has_x(int h);
-bool fft2_has_z(int nz, int *tile);
+bool fft2_has_z(int nz, int req, int *tile);
 // specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum;
 // the spectrum holds double2 or, in the FP32-storage form, float2 values;
 // acc: the x-inverse adds its result to dst

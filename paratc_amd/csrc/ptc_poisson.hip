@@ -58,6 +58,7 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.tri_ll_req = P.tri_ll_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
+  keep.zfast_req = P.zfast_req;
   keep.profile = P.profile;
   P = keep;
   return PTC_OK;
@@ -88,7 +89,7 @@ int poisson_build_plan(ptc_ctx *c) {
   }
 
   int zt = 0;
-  P.z_fast = P.ztile_req == 0 && fft2_has_z(c->nz, &zt) && P.pitch % zt == 0;
+  P.z_fast = P.ztile_req == 0 && fft2_has_z(c->nz, P.zfast_req, &zt) && P.pitch % zt == 0;
   if (P.z_fast) {
     P.ztile = zt;
   } else {

@@ -53,7 +53,7 @@ module m_paratc_gpu
   integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1
   integer(c_int), parameter, public :: PTC_TRI_AUTO = 0, PTC_TRI_THOMAS = 1, PTC_TRI_PARTITIONED = 2
   integer(c_int), parameter, public :: PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2
-  integer(c_int), parameter, public :: PTC_POISSON_TRI_LINES = 3
+  integer(c_int), parameter, public :: PTC_POISSON_TRI_LINES = 3, PTC_POISSON_FAST_ZTILE = 4
 
   interface
     integer(c_int) function ptc_create(grid, prm, yp, ctx) bind(C, name="ptc_create")

@@ -126,6 +126,9 @@ class Solver:
     def poisson_slab_planes(self, planes: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_SLAB_PLANES, planes), "ptc_poisson_set")
 
+    def poisson_fast_ztile(self, tile: int = 0):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_FAST_ZTILE, tile), "ptc_poisson_set")
+
     def poisson_tri_lines(self, lines: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_LINES, lines), "ptc_poisson_set")
 

@@ -115,6 +115,25 @@ def test_lds_staged_y_solve_matches_oracle(lines, ny, rows):
     assert res <= 1e-13 * np.abs(out).max()
 
 
+@pytest.mark.parametrize("grid,ztile", [((64, 8, 512), 16), ((128, 12, 256), 32)])
+def test_wide_z_tiles_match_oracle(grid, ztile):
+    nx, ny, nz = grid
+    case = CS.case_poisson(nx, ny, nz, 2.0)
+    f = ic.poisson_rhs(case, seed=13)
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
+        s.poisson_fast_ztile(ztile)
+        cfg = s.poisson_get_config()
+        assert cfg["z_fast"] and cfg["z_tile"] == ztile, cfg
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        out = s.download(capi.F_HALO, capi.L_INTERIOR)
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    assert rel_err(out, inner(ref)) <= TOL
+
+
 def test_automatic_variant_falls_back_to_thomas():
     """ny beyond every partitioned shape: AUTO runs Thomas, PARTITIONED refuses."""
     case = CS.case_poisson(16, 272, 8, 2.0)
