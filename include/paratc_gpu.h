@@ -232,6 +232,10 @@ int ptc_clc_mean_value(ptc_ctx *ctx, int ux, int scalar, double *res3);
  *      [divmax1, divmax2, vmax x3, cfl*dt, dt]; returns PTC_E_LIMIT when the
  *      reference's div_limit / vel_limit guard (:142-145) would abort. */
 int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7);
+/* With a fixed dt (iCFL != 1) ptc_channel_iterate captures one step into a
+ * HIP graph on first use and replays it (enable = 1, the default); enable = 0
+ * launches every kernel of every step instead.  No reference counterpart. */
+int ptc_step_graphs(ptc_ctx *ctx, int enable);
 int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
 
 /* ---- clcStat (src/m_FlowCase.f90:257-949), the statistics side.

@@ -693,7 +693,7 @@ int ptc_channel_iterate(ptc_ctx *ctx, int nsteps, double *mon7) {
   // needs the host between steps
   for (int s = 0; s < nsteps; ++s) {
     double *m = (s == nsteps - 1) ? mon : nullptr;
-    if (ctx->prm.icfl != 1) PTC_TRY(graph_step(ctx, m));
+    if (ctx->prm.icfl != 1 && ctx->step_graphs) PTC_TRY(graph_step(ctx, m));
     else PTC_TRY(channel_step(ctx, m));
   }
   if (mon7) std::copy(mon, mon + 7, mon7);
@@ -729,6 +729,14 @@ int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double
   if (nstat) *nstat = ctx->prm.scalar ? 40 : 35;
   if (nfstime) *nfstime = ctx->nfstime;
   return read_stat(ctx, sumstat, prgradsum, reset);
+}
+
+int ptc_step_graphs(ptc_ctx *ctx, int enable) {
+  PTC_CTX(ctx);
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->step_graphs = enable != 0;
+  if (!ctx->step_graphs) drop_step_graph(ctx);
+  return PTC_OK;
 }
 
 int ptc_get_scalar(ptc_ctx *ctx, int which, double *value) {

@@ -147,6 +147,7 @@ struct ptc_ctx {
   ptc::PoissonPlan pp;
   double *d_stat = nullptr;  // clcStat accumulator SumStat(NCHASTAT, ny+1)
   int nfstime = 0;           // samples in it
+  bool step_graphs = true;               // ptc_channel_iterate replays a captured step when dt is fixed
   hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
   hipGraphExec_t step_exec = nullptr;
 };

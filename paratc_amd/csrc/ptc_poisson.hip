@@ -69,7 +69,9 @@ int poisson_build_plan(ptc_ctx *c) {
   if (P.ready) return PTC_OK;
   PTC_CHECK(c->nx % 2 == 0 && c->nz % 2 == 0, PTC_E_ARG, "mod(nxc,2)/=0 || mod(nzc,2)/=0 WRONG");
   P.h = c->nx / 2;
-  P.pitch = ((P.h + 1 + 15) / 16) * 16;
+  // row pitch: a multiple of 16 complex values, or of a wider requested z tile
+  const int pq = std::max(16, P.zfast_req);
+  P.pitch = ((P.h + 1 + pq - 1) / pq) * pq;
   PTC_TRY(fft_make_len(P.h, &P.fx));
   PTC_TRY(fft_make_len(c->nz, &P.fz));
   const size_t lds_cap = std::min<size_t>(c->lds_limit, 65536);

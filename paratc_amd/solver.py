@@ -233,6 +233,10 @@ class Solver:
         return v
 
     # ---- m_ChannelSystem ----------------------------------------------------
+    def step_graphs(self, enable: bool = True):
+        """Replay a captured step graph in ChannelIterate (fixed dt) or launch every kernel."""
+        check(self.L.ptc_step_graphs(self.h, 1 if enable else 0), "ptc_step_graphs")
+
     def ChannelIterate(self, nsteps: int = 1) -> np.ndarray:
         mon = np.zeros(7)
         check(self.L.ptc_channel_iterate(self.h, nsteps, _dp(mon)), "ChannelIterate")

@@ -29,7 +29,8 @@ def test_graph_replays_match_eager_steps(kind):
         if kind == "adaptive":
             case.icfl = 1
     a, b = twins(case)
-    other = E.make_case(CS.case_channel(16, 8, 8, 1.5), 9)
+    b.step_graphs(False)  # the reference for every replay: ordinary launches
+    other = E.make_case(CS.case_channel(16, 16, 16, 1.5), 9)
     fields = FIELDS + ((capi.F_SCALAR,) if case.scalar else ())
     with E.Ensemble(1 << 34) as ens:
         cid = ens.admit(a)
@@ -46,8 +47,8 @@ def test_graph_replays_match_eager_steps(kind):
         p50, p99, n = ens.latency()
         assert n == 4 and 0.0 < p50 <= p99
         got = ens.release(cid, keep=True)
-        got.ChannelIterate(1)  # a released context steps on its own again
-        b.ChannelIterate(1)
+        got.ChannelIterate(2)  # a released context steps on its own again (its own step graph)
+        b.ChannelIterate(2)
         assert np.array_equal(got.download(capi.F_UX), b.download(capi.F_UX))
         got.close()
     other.close()

@@ -296,3 +296,27 @@ def test_channel_iterate(kind, nsteps, tol):
     # post-projection divergence at round-off
     assert mg[1] <= 1e-10
     assert abs(coe[7] - ocoe[7]) <= 1e-14  # SimTime
+
+
+def test_step_graph_equals_launches():
+    """ChannelIterate with a replayed step graph gives the fields and monitors
+    of ordinary launches, bit for bit, over several steps."""
+    case = small_channel()
+    outs = []
+    for graphs in (True, False):
+        orc = Oracle(case)
+        st = State(orc)
+        st.ux, st.uy, st.uz = ic.channel_fields(case, seed=3, amp=0.1)
+        orc.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
+        with Solver(case) as s:
+            upload_state(s, st.ux, st.uy, st.uz, st.p, None)
+            s.InitPoissonSolver(F.F_ARR1, F.F_HALO)
+            s.step_graphs(graphs)
+            mons = [s.ChannelIterate(2) for _ in range(3)]
+            outs.append(([s.download(f) for f in (F.F_UX, F.F_UY, F.F_UZ, F.F_PR)], mons, s.coe()))
+    (fa, ma, ca), (fb, mb, cb) = outs
+    for a, b in zip(fa, fb):
+        assert np.array_equal(a, b)
+    for a, b in zip(ma, mb):
+        assert np.array_equal(a, b)
+    assert ca[7] == pytest.approx(cb[7], abs=1e-14)
