@@ -274,7 +274,7 @@ def bench_ensemble(steps=500, alone_steps=20, budget_gib=64):
             solvers[q] = ens.release(cid, keep=True)
     seq_s = steps * float(np.sum(alone))
     progress(f"ensemble: one case at a time would take {seq_s:.1f} s for {steps} steps each")
-    r = E.serve(wl, steps, budget_gib << 30, solvers)
+    r = E.serve(wl, steps, budget_gib << 30, solvers, log=progress)
     progress(f"ensemble: served in {r['wall_s']:.1f} s, p50 {r['p50_ms']:.3f} ms, p99 {r['p99_ms']:.3f} ms")
     return {
         "workload": "64 closed-channel cases (seed 3), nx 64-256, ny 48-128, nz 64-192, half with a settling "

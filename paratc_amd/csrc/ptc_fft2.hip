@@ -328,12 +328,13 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
 }
 
 // ---- x inverse: half spectra -> real rows times scale ----------------------
-// ACC: add the result to dst (a correction) instead of storing it
+// ACC: add the result to dst (a correction) instead of storing it, minus the
+// device scalar *shift (the mean the correction removes)
 template <int N, typename S, bool ACC>
 __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
-                                                    double scale) {
+                                                    double scale, const double *__restrict__ shift) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = f2::padded(N);
@@ -362,11 +363,12 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
   };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
+  const double sh0 = ACC ? shift[0] : 0.0;
   auto store_out = [&](int m, double2 v) {
     if (!live) return;
     if (ACC) {
       const double2 o = row[m];
-      row[m] = make_double2(o.x + v.x * scale, o.y + v.y * scale);
+      row[m] = make_double2(o.x + v.x * scale - sh0, o.y + v.y * scale - sh0);
     } else {
       row[m] = make_double2(v.x * scale, v.y * scale);
     }
@@ -417,20 +419,21 @@ static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int 
   const auto *wx = c->pp.d_wx;
   const int ny = c->ny, nz = c->nz, pitch = c->pp.pitch;
   const long long rl = c->nx;
+  const double *shift = c->d_scal + kSlotPinShift;
   if (c->pp.prec == PTC_PREC_FP32_REFINE) {
     if (acc)
       hipLaunchKernelGGL((k2_fft_x_inv<N, float2, true>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw, wx,
-                         ny, nz, j0, nj, pitch, rl, scale);
+                         ny, nz, j0, nj, pitch, rl, scale, shift);
     else
       hipLaunchKernelGGL((k2_fft_x_inv<N, float2, false>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
   } else {
     if (acc)
       hipLaunchKernelGGL((k2_fft_x_inv<N, double2, true>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
     else
       hipLaunchKernelGGL((k2_fft_x_inv<N, double2, false>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
   }
   PTC_HIP(hipGetLastError());
   return PTC_OK;

@@ -107,6 +107,7 @@ struct PoissonPlan {
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
   void *d_spec = nullptr;     // spectral buffer pitch * ny * nz complex (double2, or float2 for FP32 storage)
   double *d_res = nullptr;    // FP32-storage form: FP64 residual field of the refinement sweeps
+  double *d_psum = nullptr;   // FP32-storage form: per-block sums of phi over the j = 1 plane
   double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
   size_t spec_elems = 0;
   size_t spec_bytes = 0;
@@ -166,6 +167,10 @@ int poisson_prepare(ptc_ctx *c);
 int stencil_prepare(ptc_ctx *c);
 // one ChannelIterate step (monitors into mon[7] when mon != NULL)
 int channel_step(ptc_ctx *c, double *mon);
+// device scalar holding the x-z mean of phi's first row before a correction
+// (the refinement's x-inverse subtracts it, keeping the (0,0) pin)
+constexpr int kSlotPinShift = 4;
+
 // fill n doubles of device memory with v on the context's stream
 int fill_device(ptc_ctx *c, double *a, long long n, double v);
 // clcStat accumulation and read-back (ptc_stencil.hip)

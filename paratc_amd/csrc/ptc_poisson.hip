@@ -47,6 +47,7 @@ int poisson_free_plan(ptc_ctx *c) {
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
   if (P.d_res) (void)hipFree(P.d_res);
+  if (P.d_psum) (void)hipFree(P.d_psum);
   if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
   for (auto &e : P.prof_ev)
     if (e) (void)hipEventDestroy(e);
@@ -166,6 +167,11 @@ int poisson_build_plan(ptc_ctx *c) {
       return PTC_E_ALLOC;
     }
     PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
+    if (hipMalloc(&P.d_psum, sizeof(double) * 8192) != hipSuccess) {
+      P.d_psum = nullptr;
+      set_error("residual partial-sum allocation failed");
+      return PTC_E_ALLOC;
+    }
   }
   P.ready = true;
   return PTC_OK;
@@ -245,7 +251,9 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
   PTC_TRY(solve_sequence(c, src, dst, false, ev));
   if (P.prec == PTC_PREC_FP32_REFINE) {
     // defect correction: phi += L^-1 (f - L phi), the residual in FP64 and
-    // the correction through the same FP32-storage passes
+    // the correction through the same FP32-storage passes; the correction's
+    // x-inverse also removes the mean of phi's first row, so the (0,0) mode
+    // keeps the reference's pin phi(j=1) = 0 (src/m_Poisson.f90:399-404)
     for (int s = 0; s < P.sweeps; ++s) {
       PTC_TRY(launch_residual(c, src, dst, P.d_res));
       PTC_TRY(solve_sequence(c, P.d_res, dst, true, nullptr));
@@ -281,13 +289,15 @@ int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves) {
 // 1) / 576 rdx^2) and am2ph / ac2ph / ap2ph in y with the Neumann ghosts
 // phi_0 = phi_1, phi_{ny+1} = phi_ny.  WRITE: store r (field layout); part
 // (may be NULL): per-block max |r|.
+// psum (may be NULL): per-block sum of phi over the plane j = 1, in a fixed
+// order.
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, const double *__restrict__ phi,
                                                  Geo g, double rdx2, double rdz2, double *__restrict__ rout,
-                                                 double *__restrict__ part) {
+                                                 double *__restrict__ part, double *__restrict__ psum) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long long total = (long long)nx * ny * nz;
-  double mx = 0.0;
+  double mx = 0.0, ps = 0.0;
   const double *am = g.met + M_AM2PH * g.nyh, *ac = g.met + M_AC2PH * g.nyh, *ap = g.met + M_AP2PH * g.nyh;
   constexpr double st[4] = {-1460.0 / 576.0, 783.0 / 576.0, -54.0 / 576.0, 1.0 / 576.0};
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
@@ -315,22 +325,55 @@ __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, 
     const double res = f[o] - lap;
     if (WRITE) rout[o] = res;
     mx = fmax(mx, fabs(res));
+    if (j == 1) ps += c0;
   }
-  if (!part) return;
   __shared__ double sh[256];
-  sh[threadIdx.x] = mx;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+  if (part) {
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  if (psum) {
+    sh[threadIdx.x] = ps;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) psum[blockIdx.x] = sh[0];
+  }
+}
+
+// out = (sum of the nblk partial sums, in a fixed order) * scale
+__global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ psum, int nblk, double scale,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) s += psum[b];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0] * scale;
 }
 
 int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
   const int nblk = std::min(8192, c->n_cu * 16);
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual<true>, dim3(nblk), dim3(256), 0, c->stream, f, phi, c->geo, rdx2, rdz2, r, nullptr);
+  hipLaunchKernelGGL(k_residual<true>, dim3(nblk), dim3(256), 0, c->stream, f, phi, c->geo, rdx2, rdz2, r, nullptr,
+                     c->pp.d_psum);
+  PTC_HIP(hipGetLastError());
+  // the x-z mean of phi's first row: the (0,0) mode's pinned value, which no
+  // residual sees (a constant is in the Neumann operator's null space)
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->pp.d_psum, nblk,
+                     1.0 / ((double)c->nx * (double)c->nz), c->d_scal + kSlotPinShift);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -344,7 +387,7 @@ int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *r
   }
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
   hipLaunchKernelGGL(k_residual<false>, dim3(nblk), dim3(256), 0, c->stream, src, dst, c->geo, rdx2, rdz2, nullptr,
-                     c->d_red);
+                     c->d_red, nullptr);
   PTC_HIP(hipGetLastError());
   std::vector<double> h((size_t)nblk);
   PTC_HIP(hipMemcpyAsync(h.data(), c->d_red, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
