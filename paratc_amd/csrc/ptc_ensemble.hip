@@ -154,22 +154,31 @@ int ptc_ensemble_step(ptc_ensemble *e, int nsteps) {
     PTC_HIP(hipEventRecord(start, e->side));
     std::vector<hipEvent_t> done;
     done.reserve(e->cases.size());
+    int rc = PTC_OK;
     for (auto &k : e->cases) {
       if (!k.live) continue;
-      PTC_TRY(step_case(k));
+      rc = step_case(k);
+      if (rc != PTC_OK) break;
       hipEvent_t ev;
-      PTC_TRY(take_event(e, &ev));
-      PTC_HIP(hipEventRecord(ev, k.ctx->stream));
+      rc = take_event(e, &ev);
+      if (rc != PTC_OK) break;
+      if (hipEventRecord(ev, k.ctx->stream) != hipSuccess) {
+        e->pool.push_back(ev);
+        set_error("hipEventRecord failed");
+        rc = PTC_E_HIP;
+        break;
+      }
       done.push_back(ev);
     }
+    // wait for everything this round launched, also after a failure
     for (hipEvent_t ev : done) {
-      PTC_HIP(hipEventSynchronize(ev));
       float ms = 0.f;
-      PTC_HIP(hipEventElapsedTime(&ms, start, ev));
-      e->lat_ms.push_back(ms);
+      if (hipEventSynchronize(ev) == hipSuccess && hipEventElapsedTime(&ms, start, ev) == hipSuccess)
+        e->lat_ms.push_back(ms);
       e->pool.push_back(ev);
     }
     e->pool.push_back(start);
+    if (rc != PTC_OK) return rc;
   }
   return PTC_OK;
 }

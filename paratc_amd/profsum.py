@@ -21,7 +21,8 @@ import sqlite3
 import sys
 from typing import Dict, List, Optional, Tuple
 
-TRAFFIC_HEADER = "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | read bytes | write bytes | traffic bytes |"
+TRAFFIC_HEADER = ("| kernel @ grid (threads) | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | read bytes | write bytes "
+                  "| traffic bytes |")
 
 
 def short_name(name: str) -> str:
@@ -41,31 +42,37 @@ def short_name(name: str) -> str:
 
 
 def trace_table(db: str) -> str:
+    """Per kernel and grid size (launches of one kernel over different extents,
+    such as slabs, are listed apart)."""
     con = sqlite3.connect(db)
-    rows = con.execute("select name, count(*), sum(end-start), avg(end-start), min(end-start), max(end-start) "
-                       "from kernels group by name order by sum(end-start) desc").fetchall()
-    tot = sum(r[2] for r in rows) or 1
-    out = ["| kernel | calls | total us | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|---:|"]
-    for name, n, s, a, lo, hi in rows:
-        out.append(f"| {short_name(name)} | {n} | {s / 1e3:.1f} | {a / 1e3:.2f} | {lo / 1e3:.2f} | {hi / 1e3:.2f} "
-                   f"| {100 * s / tot:.1f} |")
+    rows = con.execute("select name, grid_x * grid_y * grid_z, count(*), sum(end-start), avg(end-start), "
+                       "min(end-start), max(end-start) from kernels group by name, grid_x * grid_y * grid_z "
+                       "order by sum(end-start) desc").fetchall()
+    tot = sum(r[3] for r in rows) or 1
+    out = ["| kernel | grid (threads) | calls | total us | avg us | min us | max us | % |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for name, grid, n, s, a, lo, hi in rows:
+        out.append(f"| {short_name(name)} | {grid} | {n} | {s / 1e3:.1f} | {a / 1e3:.2f} | {lo / 1e3:.2f} "
+                   f"| {hi / 1e3:.2f} | {100 * s / tot:.1f} |")
     return "\n".join(out)
 
 
 def _counter_means(path: str) -> Dict[str, Tuple[int, float]]:
+    """Mean counter value per kernel and grid size ("name @ grid")."""
     acc: Dict[str, List[float]] = collections.defaultdict(list)
     with open(path) as fh:
         for r in csv.DictReader(fh):
-            acc[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+            acc[f"{short_name(r['Kernel_Name'])} @ {r['Grid_Size']}"].append(float(r["Counter_Value"]))
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
 def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
     fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
-    if "k_copy" not in fetch or "k_copy" not in write:
+    copy = [k for k in fetch if k.startswith("k_copy @")]
+    if not copy or copy[0] not in write:
         raise SystemExit("k_copy is not in the counter files: profile bench.py --calib-copy")
-    rf = nbytes / (fetch["k_copy"][1] * 1024.0)
-    wf = nbytes / (write["k_copy"][1] * 1024.0)
+    rf = nbytes / (fetch[copy[0]][1] * 1024.0)
+    wf = nbytes / (write[copy[0]][1] * 1024.0)
     out = [f"read_factor {rf:.4f} write_factor {wf:.4f} (k_copy of {nbytes:.0f} B each way)", "",
            TRAFFIC_HEADER, "|---|---:|---:|---:|---:|---:|---:|"]
     for k in sorted(fetch, key=lambda k: -fetch[k][1]):
@@ -78,16 +85,20 @@ def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
 
 
 def read_traffic(md_path: str, kernel: str) -> Optional[float]:
-    """Traffic bytes per launch of `kernel` (a short-name prefix) from a traffic table, or None."""
+    """Traffic bytes per launch of `kernel` (a short-name prefix) from a traffic
+    table, or None; with several grid sizes, the largest launch."""
     try:
         lines = open(md_path).read().splitlines()
     except OSError:
         return None
+    best = None
     for ln in lines:
         cells = [c.strip() for c in ln.strip().strip("|").split("|")]
         if len(cells) == 7 and cells[0].startswith(kernel) and cells[6].replace(".", "").isdigit():
-            return float(cells[6])
-    return None
+            grid = int(cells[0].rsplit("@", 1)[1]) if "@" in cells[0] else 0
+            if best is None or grid > best[0]:
+                best = (grid, float(cells[6]))
+    return None if best is None else best[1]
 
 
 def main(argv: List[str]) -> None:
