@@ -179,9 +179,27 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *   PTC_POISSON_FAST_ZTILE  column tile of the specialised z kernel (0 = its
  *                           default; 16 for nz = 512 and 32 for nz = 256 are
  *                           the wider compiled alternatives).  A tile with no
- *                           compiled kernel falls back to the generic z pass. */
-enum { PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2, PTC_POISSON_TRI_LINES = 3, PTC_POISSON_FAST_ZTILE = 4 };
+ *                           compiled kernel falls back to the generic z pass.
+ *   PTC_POISSON_TRI_LANES   real lines per workgroup of the register y-solve
+ *                           (0 = automatic, 64, 32 or 16): narrower
+ *                           workgroups are smaller, so several share a CU.
+ *   PTC_POISSON_TRI_COMPLEX 0: the partitioned y-solve treats the real and
+ *                           imaginary parts as separate lanes; 16 or 32: one
+ *                           lane per complex line (shared row coefficients and
+ *                           divisions), that many lines per workgroup. */
+enum {
+  PTC_POISSON_CHUNK_ROWS = 1,
+  PTC_POISSON_SLAB_PLANES = 2,
+  PTC_POISSON_TRI_LINES = 3,
+  PTC_POISSON_FAST_ZTILE = 4,
+  PTC_POISSON_TRI_LANES = 5,
+  PTC_POISSON_TRI_COMPLEX = 6
+};
 int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
+/* the value a plan knob has in the plan in use (chunk rows, slab planes,
+ * LDS-staged lines, z tile, lanes per workgroup, complex lines per
+ * workgroup; 0 where the knob's variant is not in use) */
+int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
 /* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per
  * workgroup, slab planes, LDS-staged lines (0 = register kernel)] (after

@@ -85,18 +85,24 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   // (src/m_Poisson.f90:50-171): every candidate plan is built and timed over
   // three solves of the (scratch) src array; the fastest is kept.  First the
   // FFT kernels (specialised or a generic z tile) with the default y-solve,
-  // then the y-solve (partitioned chunk length, or Thomas) with that FFT.
+  // then the y-solve (partitioned: chunk length and workgroup width, LDS
+  // staging or complex lines; or Thomas) with that FFT.
   struct Cand {
-    int tri, mm, tile, slab, ll, zf;
+    int tri, mm, tile, slab, ll, zf, lanes, cl;
   };
-  auto run = [&](const Cand &cd, double *ms) -> int {
-    poisson_free_plan(c);
+  auto apply = [&](const Cand &cd) {
     c->pp.tri_req = cd.tri;
     c->pp.tri_mm_req = cd.mm;
     c->pp.ztile_req = cd.tile;
     c->pp.slab_req = cd.slab;
     c->pp.tri_ll_req = cd.ll;
     c->pp.zfast_req = cd.zf;
+    c->pp.tri_lpb_req = cd.lanes;
+    c->pp.tri_cl_req = cd.cl;
+  };
+  auto run = [&](const Cand &cd, double *ms) -> int {
+    poisson_free_plan(c);
+    apply(cd);
     if (poisson_build_plan(c) != PTC_OK) return PTC_E_UNSUPPORTED;
     if (cd.zf > 0 && !(c->pp.z_fast && c->pp.ztile == cd.zf)) return PTC_E_UNSUPPORTED;
     if (cd.tile > 0 && c->pp.ztile != cd.tile) return PTC_E_UNSUPPORTED;  // clipped: covered elsewhere
@@ -104,7 +110,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     return time_solve(c, s, d, 3, ms);
   };
   double best = 1e300;
-  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0};
+  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0, 0, 0};
   auto consider = [&](const Cand &cd) {
     double ms = 0;
     if (run(cd, &ms) == PTC_OK && ms < best) {
@@ -112,23 +118,22 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
       pick = cd;
     }
   };
-  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0});
-  for (int zf : {16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf});
+  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0, 0, 0});
+  for (int zf : {16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0});
   const int tile = pick.tile, zf = pick.zf;
-  for (int mm : {4, 8, 12, 16})
-    for (int ll : {0, 16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll, zf});
-  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0, zf});
+  for (int mm : {4, 8, 12, 16}) {
+    for (int lanes : {64, 32, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, lanes, 0});
+    for (int ll : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll, zf, 0, 0});
+    for (int cl : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, 0, cl});
+  }
+  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0, zf, 0, 0});
   if (tile == 0) {
     const Cand base = pick;
-    for (int slab : {64, 32, 16, 8}) consider(Cand{base.tri, base.mm, 0, slab, base.ll, base.zf});
+    for (int slab : {64, 32, 16, 8})
+      consider(Cand{base.tri, base.mm, 0, slab, base.ll, base.zf, base.lanes, base.cl});
   }
   poisson_free_plan(c);
-  c->pp.tri_req = pick.tri;
-  c->pp.tri_mm_req = pick.mm;
-  c->pp.ztile_req = pick.tile;
-  c->pp.slab_req = pick.slab;
-  c->pp.tri_ll_req = pick.ll;
-  c->pp.zfast_req = pick.zf;
+  apply(pick);
   PTC_TRY(poisson_build_plan(c));
   // src/m_Poisson.f90:174: prsrc = zero; prphiHalo = zero
   PTC_TRY(fill_device(c, s, (long long)c->n3, 0.0));
@@ -440,6 +445,12 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
     PTC_CHECK(value == 0 || value == 16 || value == 32, PTC_E_ARG, "tridiagonal lines per workgroup must be 0, 16 or 32");
   } else if (knob == PTC_POISSON_FAST_ZTILE) {
     PTC_CHECK(value >= 0 && value <= 32, PTC_E_ARG, "z tile must be in 0..32");
+  } else if (knob == PTC_POISSON_TRI_LANES) {
+    PTC_CHECK(value == 0 || value == 16 || value == 32 || value == 64, PTC_E_ARG,
+              "y-solve lanes per workgroup must be 0, 16, 32 or 64");
+  } else if (knob == PTC_POISSON_TRI_COMPLEX) {
+    PTC_CHECK(value == 0 || value == 16 || value == 32, PTC_E_ARG,
+              "complex lines per workgroup must be 0, 16 or 32");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
@@ -448,8 +459,27 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   if (knob == PTC_POISSON_CHUNK_ROWS) ctx->pp.tri_mm_req = value;
   else if (knob == PTC_POISSON_SLAB_PLANES) ctx->pp.slab_req = value;
   else if (knob == PTC_POISSON_TRI_LINES) ctx->pp.tri_ll_req = value;
+  else if (knob == PTC_POISSON_TRI_LANES) ctx->pp.tri_lpb_req = value;
+  else if (knob == PTC_POISSON_TRI_COMPLEX) ctx->pp.tri_cl_req = value;
   else ctx->pp.zfast_req = value;
   return poisson_build_plan(ctx);
+}
+
+int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value) {
+  PTC_CTX(ctx);
+  PTC_CHECK(value != nullptr, PTC_E_ARG, "null output");
+  const PoissonPlan &P = ctx->pp;
+  const bool part = P.tri == PTC_TRI_PARTITIONED;
+  switch (knob) {
+    case PTC_POISSON_CHUNK_ROWS: *value = part ? P.tri_mmax : 0; break;
+    case PTC_POISSON_SLAB_PLANES: *value = P.slab; break;
+    case PTC_POISSON_TRI_LINES: *value = part ? P.tri_ll : 0; break;
+    case PTC_POISSON_FAST_ZTILE: *value = P.z_fast ? P.ztile : 0; break;
+    case PTC_POISSON_TRI_LANES: *value = part && !P.tri_ll && !P.tri_cl ? P.tri_lpb : 0; break;
+    case PTC_POISSON_TRI_COMPLEX: *value = part ? P.tri_cl : 0; break;
+    default: PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
+  }
+  return PTC_OK;
 }
 
 int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {

@@ -193,13 +193,39 @@ struct Dft<7, INV> : DftOdd<7, INV> {};
 // ---- one stage for the G threads of a line; Load(m) / Store(m, v) address the
 //      line's elements.  SYNC: a barrier separates this stage's loads from its
 //      stores (needed when both go to the same LDS buffer). ----
-template <int N, int S, int G, bool INV, bool SYNC, class Load, class Store>
-__device__ __forceinline__ void stage(int g, const double2 *__restrict__ tw, const Load &load, const Store &store) {
+// butterflies per thread of stage S (at most 4: G = N / rmax, radix >= 2)
+template <int N, int S, int G>
+constexpr int stage_bpt() {
+  return (N / make_plan(N).r[S] + G - 1) / G;
+}
+
+// The first twiddle w1 of each butterfly a thread runs in stage S, read from
+// the table ahead of the transform (it does not depend on the data), so no
+// stage waits on a global load.
+template <int N, int S, int G, bool INV>
+__device__ __forceinline__ void stage_twiddles(int g, const double2 *__restrict__ tw, double2 (&w)[4]) {
   constexpr PlanC P = make_plan(N);
   constexpr int R = P.r[S];
   constexpr int NS = plan_span(N, S);
   constexpr int NB = N / R;
-  constexpr int BPT = (NB + G - 1) / G;
+  constexpr int BPT = stage_bpt<N, S, G>();
+  static_assert(BPT <= 4, "at most four butterflies per thread");
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    const int b = g + i * G;
+    const int k = (BPT * G == NB || b < NB) ? b % NS : 0;
+    w[i] = NS > 1 ? tw[k * (N / (NS * R))] : make_double2(1.0, 0.0);
+    if (INV) w[i].y = -w[i].y;
+  }
+}
+
+template <int N, int S, int G, bool INV, bool SYNC, class Load, class Store>
+__device__ __forceinline__ void stage(int g, const double2 (&wpre)[4], const Load &load, const Store &store) {
+  constexpr PlanC P = make_plan(N);
+  constexpr int R = P.r[S];
+  constexpr int NS = plan_span(N, S);
+  constexpr int NB = N / R;
+  constexpr int BPT = stage_bpt<N, S, G>();
   double2 v[BPT][R];
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
@@ -216,8 +242,7 @@ __device__ __forceinline__ void stage(int g, const double2 *__restrict__ tw, con
     if (BPT * G == NB || b < NB) {
       const int k = b % NS;
       if (NS > 1) {
-        double2 w1 = tw[k * (N / (NS * R))];
-        if (INV) w1.y = -w1.y;
+        const double2 w1 = wpre[i];
         double2 w = w1;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -241,35 +266,43 @@ template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class Lds
 __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
                                         const StoreL &storeL) {
   constexpr PlanC P = make_plan(N);
+  static_assert(P.ns <= 7, "too many stages");
   auto ldsload = [&](int m) { return lds(m); };
   auto ldsstore = [&](int m, double2 v) { lds(m) = v; };
+  double2 W[7][4];
+  stage_twiddles<N, 0, G, INV>(g, tw, W[0]);
+  if constexpr (P.ns > 1) stage_twiddles<N, (P.ns > 1 ? 1 : 0), G, INV>(g, tw, W[1]);
+  if constexpr (P.ns > 2) stage_twiddles<N, (P.ns > 2 ? 2 : 0), G, INV>(g, tw, W[2]);
+  if constexpr (P.ns > 3) stage_twiddles<N, (P.ns > 3 ? 3 : 0), G, INV>(g, tw, W[3]);
+  if constexpr (P.ns > 4) stage_twiddles<N, (P.ns > 4 ? 4 : 0), G, INV>(g, tw, W[4]);
+  if constexpr (P.ns > 5) stage_twiddles<N, (P.ns > 5 ? 5 : 0), G, INV>(g, tw, W[5]);
+  if constexpr (P.ns > 6) stage_twiddles<N, (P.ns > 6 ? 6 : 0), G, INV>(g, tw, W[6]);
   if constexpr (P.ns == 1) {
-    stage<N, 0, G, INV, false>(g, tw, load0, storeL);
+    stage<N, 0, G, INV, false>(g, W[0], load0, storeL);
   } else {
-    stage<N, 0, G, INV, LOAD0_LDS>(g, tw, load0, ldsstore);
+    stage<N, 0, G, INV, LOAD0_LDS>(g, W[0], load0, ldsstore);
     __syncthreads();
     if constexpr (P.ns > 2) {
-      stage<N, 1, G, INV, true>(g, tw, ldsload, ldsstore);
+      stage<N, 1, G, INV, true>(g, W[1], ldsload, ldsstore);
       __syncthreads();
     }
     if constexpr (P.ns > 3) {
-      stage<N, 2, G, INV, true>(g, tw, ldsload, ldsstore);
+      stage<N, 2, G, INV, true>(g, W[2], ldsload, ldsstore);
       __syncthreads();
     }
     if constexpr (P.ns > 4) {
-      stage<N, 3, G, INV, true>(g, tw, ldsload, ldsstore);
+      stage<N, 3, G, INV, true>(g, W[3], ldsload, ldsstore);
       __syncthreads();
     }
     if constexpr (P.ns > 5) {
-      stage<N, 4, G, INV, true>(g, tw, ldsload, ldsstore);
+      stage<N, 4, G, INV, true>(g, W[4], ldsload, ldsstore);
       __syncthreads();
     }
     if constexpr (P.ns > 6) {
-      stage<N, 5, G, INV, true>(g, tw, ldsload, ldsstore);
+      stage<N, 5, G, INV, true>(g, W[5], ldsload, ldsstore);
       __syncthreads();
     }
-    static_assert(P.ns <= 7, "too many stages");
-    stage<N, P.ns - 1, G, INV, true>(g, tw, ldsload, storeL);
+    stage<N, P.ns - 1, G, INV, true>(g, W[P.ns - 1], ldsload, storeL);
   }
 }
 
@@ -348,8 +381,16 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
   const S *in = spec + (jl + (long long)ny * k) * pitch;
   double2 *ls = sh + (l < L ? l : 0) * PN;
   // The re-tangle of element m needs X_m and X_{N-m}: the line's N+1 values
-  // go through LDS once (coalesced) instead of being fetched twice.
-  for (int m = g; m <= N; m += G) ls[f2::pad8(m)] = live ? f2::ld2(in[m]) : make_double2(0.0, 0.0);
+  // go through LDS once (coalesced) instead of being fetched twice.  All of a
+  // thread's loads are issued before the first LDS write (lines past the end
+  // read line 0 and store nothing; indices past N repeat element N, so
+  // every thread that reaches it writes the same value).
+  constexpr int NT = N / G + 1;
+  double2 pre[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) pre[t] = f2::ld2(in[min(g + t * G, N)]);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) ls[f2::pad8(min(g + t * G, N))] = pre[t];
   __syncthreads();
   auto load0 = [&](int m) {
     const double2 xk = ls[f2::pad8(m)];

@@ -100,8 +100,11 @@ struct PoissonPlan {
   int tri_chunks = 1; // chunks (waves) per line in the partitioned solver
   int tri_mmax = 16;  // register rows per thread (chunk length)
   int tri_lpb = 64;   // lines per workgroup
+  int tri_lpb_req = 0; // register y-solve: real lines per workgroup requested (0 = automatic)
   int tri_ll_req = 0; // LDS-staged y-solve: real lines per workgroup requested (0 = register kernel)
   int tri_ll = 0;     // in use
+  int tri_cl_req = 0; // complex-line y-solve: complex lines per workgroup requested (0 = real-line kernels)
+  int tri_cl = 0;     // in use
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h

@@ -29,8 +29,10 @@
 namespace ptc {
 
 bool fft_config_ok(const FftLen &L, int ncol, int nthr);
-int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out);
+int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int lpb_req, int *P_out, int *mm_out, int *lpb_out);
 int tri_plan_lds(int ny, size_t lds_limit, int ll, int mm_req, int *P_out, int *mm_out);
+int tri_plan_cplx(int ny, size_t lds_limit, int cl, int mm_req, int *P_out, int *mm_out);
+int residual_blocks(const ptc_ctx *c);
 
 static void make_twiddles(int n, std::vector<double2> &out) {
   const double twopi = 6.283185307179586476925286766559005768;
@@ -57,6 +59,8 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.tri_mm_req = P.tri_mm_req;
   keep.slab_req = P.slab_req;
   keep.tri_ll_req = P.tri_ll_req;
+  keep.tri_lpb_req = P.tri_lpb_req;
+  keep.tri_cl_req = P.tri_cl_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.zfast_req = P.zfast_req;
@@ -123,12 +127,18 @@ int poisson_build_plan(ptc_ctx *c) {
   // partitioned shape fits this ny
   P.tri = P.tri_req == PTC_TRI_THOMAS ? PTC_TRI_THOMAS : PTC_TRI_PARTITIONED;
   P.tri_ll = 0;
-  if (P.tri == PTC_TRI_PARTITIONED && P.tri_ll_req > 0) {
+  P.tri_cl = 0;
+  if (P.tri == PTC_TRI_PARTITIONED && P.tri_cl_req > 0) {
+    PTC_TRY(tri_plan_cplx(c->ny, c->lds_limit, P.tri_cl_req, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax));
+    P.tri_cl = P.tri_cl_req;
+    P.tri_lpb = P.tri_cl;
+  } else if (P.tri == PTC_TRI_PARTITIONED && P.tri_ll_req > 0) {
     PTC_TRY(tri_plan_lds(c->ny, c->lds_limit, P.tri_ll_req, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax));
     P.tri_ll = P.tri_ll_req;
     P.tri_lpb = P.tri_ll;
   } else if (P.tri == PTC_TRI_PARTITIONED) {
-    const int rc = tri_plan(c->ny, c->lds_limit, true, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb);
+    const int rc =
+        tri_plan(c->ny, c->lds_limit, true, P.tri_mm_req, P.tri_lpb_req, &P.tri_chunks, &P.tri_mmax, &P.tri_lpb);
     if (rc != PTC_OK) {
       if (P.tri_req == PTC_TRI_PARTITIONED || P.prec == PTC_PREC_FP32_REFINE) return rc;
       P.tri = PTC_TRI_THOMAS;
@@ -167,7 +177,7 @@ int poisson_build_plan(ptc_ctx *c) {
       return PTC_E_ALLOC;
     }
     PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
-    if (hipMalloc(&P.d_psum, sizeof(double) * 8192) != hipSuccess) {
+    if (hipMalloc(&P.d_psum, sizeof(double) * (size_t)residual_blocks(c)) != hipSuccess) {
       P.d_psum = nullptr;
       set_error("residual partial-sum allocation failed");
       return PTC_E_ALLOC;
@@ -284,69 +294,145 @@ int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves) {
 // L in x and z is the composed stencil D o G of the 4th-order staggered
 // operators: (1, -54, 783, -1460, 783, -54, 1) / 576 / dx^2, whose symbol is
 // the modified wavenumber of src/m_Poisson.f90:42,46.
+// Order the LDS writes of a wavefront before its lanes read each other's
+// values (the lanes of one wavefront run in lockstep; this keeps the compiler
+// from moving the accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // r = f - L phi over the interior with the FP64 staggered operator: 4th-order
 // x/z (the symbol of the modified wavenumbers, (1, -54, 783, -1460, 783, -54,
 // 1) / 576 rdx^2) and am2ph / ac2ph / ap2ph in y with the Neumann ghosts
-// phi_0 = phi_1, phi_{ny+1} = phi_ny.  WRITE: store r (field layout); part
-// (may be NULL): per-block max |r|.
-// psum (may be NULL): per-block sum of phi over the plane j = 1, in a fixed
-// order.
+// phi_0 = phi_1, phi_{ny+1} = phi_ny.
+//
+// A wavefront owns one y row of 64 x columns and marches along k over planes
+// [k0, k1).  Every global load is issued kResAhead planes before its use: the
+// column's z stencil sits in a register window that runs kResAhead - 1 planes
+// past the stencil's reach, and f, the two y neighbours and the six x-halo
+// columns (lanes 0..5) come through register queues of that depth.  The x
+// neighbours are the adjacent lanes' centre values, exchanged through a
+// per-wavefront LDS row.  WRITE: store r (field layout).  part (may be NULL):
+// per-block max |r|; psum (may be NULL): per-block sum of phi over j = 1.
+constexpr int kResAhead = 4;
+
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, const double *__restrict__ phi,
-                                                 Geo g, double rdx2, double rdz2, double *__restrict__ rout,
-                                                 double *__restrict__ part, double *__restrict__ psum) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long total = (long long)nx * ny * nz;
-  double mx = 0.0, ps = 0.0;
-  const double *am = g.met + M_AM2PH * g.nyh, *ac = g.met + M_AC2PH * g.nyh, *ap = g.met + M_AP2PH * g.nyh;
-  constexpr double st[4] = {-1460.0 / 576.0, 783.0 / 576.0, -54.0 / 576.0, 1.0 / 576.0};
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(q % nx);
-    const long long r = q / nx;
-    const int j = (int)(r % ny) + 1;
-    const int k = (int)(r / ny);
-    auto at = [&](int ii, int jj, int kk) -> double {
-      ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
-      kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
-      return phi[(long long)ii + (long long)nx * ((long long)jj + (long long)(ny + 2) * kk)];
-    };
-    const double c0 = at(i, j, k);
-    double lx = st[0] * c0, lz = st[0] * c0;
-#pragma unroll
-    for (int s = 1; s <= 3; ++s) {
-      lx += st[s] * (at(i - s, j, k) + at(i + s, j, k));
-      lz += st[s] * (at(i, j, k - s) + at(i, j, k + s));
-    }
-    const double ym = j > 1 ? at(i, j - 1, k) : c0;
-    const double yp = j < ny ? at(i, j + 1, k) : c0;
-    const double lap = rdx2 * lx + rdz2 * lz + (am[j] * ym + ac[j] * c0 + ap[j] * yp);
-    const long long o = (long long)i + (long long)nx * ((long long)j + (long long)(ny + 2) * k);
-    const double res = f[o] - lap;
-    if (WRITE) rout[o] = res;
-    mx = fmax(mx, fabs(res));
-    if (j == 1) ps += c0;
-  }
+                                                 Geo g, double rdx2, double rdz2, int kc,
+                                                 double *__restrict__ rout, double *__restrict__ part,
+                                                 double *__restrict__ psum) {
+  constexpr int D = kResAhead, NW = 7 + D - 1;
+  __shared__ double xrow[4][2][72];  // per wavefront: one plane row, 3 halo columns each side
   __shared__ double sh[256];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int lane = threadIdx.x, wv = threadIdx.y;
+  const int i0 = blockIdx.x * 64, i = i0 + lane;
+  const int j = blockIdx.y * 4 + wv + 1;
+  const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, nz);
+  const bool act = i < nx && j <= ny;
+  const int jc = min(j, ny);
+  // the column this lane loads: its own (wrapped for lanes past nx), or for
+  // lanes 0..5 also an x-halo column (slot 0..2 and 67..69 of the LDS row)
+  const int col = (i0 + lane) % nx;
+  const int hslot = lane < 3 ? lane : 64 + lane;
+  const int hcol = ((i0 - 3 + hslot) % nx + nx) % nx;
+  const long long pl = (long long)nx * (ny + 2);
+  const double am = g.met[M_AM2PH * g.nyh + jc], ac = g.met[M_AC2PH * g.nyh + jc],
+               ap = g.met[M_AP2PH * g.nyh + jc];
+  constexpr double st0 = -1460.0 / 576.0, st1 = 783.0 / 576.0, st2 = -54.0 / 576.0, st3 = 1.0 / 576.0;
+  auto wz = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
+  const double *cc = phi + col + (long long)nx * jc;   // this column at row jc
+  const double *hc = phi + hcol + (long long)nx * jc;  // halo column at row jc
+  const double *fc = f + col + (long long)nx * jc;
+  const bool halo = lane < 6;
+  double w[NW], fq[D], qm[D], qp[D], hq[D];
+#pragma unroll
+  for (int s = 0; s < NW; ++s) w[s] = cc[wz(k0 + s - 3) * pl];
+#pragma unroll
+  for (int s = 0; s < D; ++s) {
+    const int kk = wz(k0 + s);
+    const long long kp = (long long)kk * pl;
+    fq[s] = fc[kp];
+    qm[s] = cc[kp - nx];
+    qp[s] = cc[kp + nx];
+    hq[s] = halo ? hc[kp] : 0.0;
+  }
+  double mx = 0.0, ps = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    double *R = xrow[wv][k & 1];
+    const double c0 = w[3];
+    R[lane + 3] = c0;
+    if (halo) R[hslot] = hq[0];
+    wave_sync();
+    const double lx = st0 * c0 + st1 * (R[lane + 2] + R[lane + 4]) + st2 * (R[lane + 1] + R[lane + 5]) +
+                      st3 * (R[lane] + R[lane + 6]);
+    const double lz = st0 * c0 + st1 * (w[2] + w[4]) + st2 * (w[1] + w[5]) + st3 * (w[0] + w[6]);
+    const double ym = jc > 1 ? qm[0] : c0;
+    const double yp = jc < ny ? qp[0] : c0;
+    const double lap = rdx2 * lx + rdz2 * lz + (am * ym + ac * c0 + ap * yp);
+    const double res = fq[0] - lap;
+    if (act) {
+      if (WRITE) rout[col + (long long)nx * jc + (long long)k * pl] = res;
+      mx = fmax(mx, fabs(res));
+      if (jc == 1) ps += c0;
+    }
+    // advance the window and the queues by one plane; the new entries are
+    // the planes k + 1 + 3 + (D - 1) (window) and k + D (queues)
+#pragma unroll
+    for (int s = 0; s < NW - 1; ++s) w[s] = w[s + 1];
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) {
+      fq[s] = fq[s + 1];
+      qm[s] = qm[s + 1];
+      qp[s] = qp[s + 1];
+      hq[s] = hq[s + 1];
+    }
+    const long long kq = (long long)wz(k + D) * pl;
+    w[NW - 1] = cc[(long long)wz(k + 3 + D) * pl];
+    fq[D - 1] = fc[kq];
+    qm[D - 1] = cc[kq - nx];
+    qp[D - 1] = cc[kq + nx];
+    if (halo) hq[D - 1] = hc[kq];
+  }
+  const int t = threadIdx.x + 64 * threadIdx.y;
+  const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
   if (part) {
-    sh[threadIdx.x] = mx;
+    sh[t] = mx;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) sh[t] = fmax(sh[t], sh[t + h]);
       __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (t == 0) part[blk] = sh[0];
     __syncthreads();
   }
   if (psum) {
-    sh[threadIdx.x] = ps;
+    sh[t] = ps;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) sh[t] += sh[t + h];
       __syncthreads();
     }
-    if (threadIdx.x == 0) psum[blockIdx.x] = sh[0];
+    if (t == 0) psum[blk] = sh[0];
   }
+}
+
+// launch shape of k_residual: (i, j) tiles of 64 x 4, k split into chunks
+// until there are about sixteen workgroups per CU
+static dim3 residual_grid(const ptc_ctx *c, int *kc) {
+  const int gx = (c->nx + 63) / 64, gy = (c->ny + 3) / 4;
+  const int want = std::max(1, (16 * c->n_cu + gx * gy - 1) / (gx * gy));
+  const int nch = std::min(c->nz, want);
+  *kc = (c->nz + nch - 1) / nch;
+  return dim3((unsigned)gx, (unsigned)gy, (unsigned)((c->nz + *kc - 1) / *kc));
+}
+
+int residual_blocks(const ptc_ctx *c) {
+  int kc = 0;
+  const dim3 gr = residual_grid(c, &kc);
+  return (int)(gr.x * gr.y * gr.z);
 }
 
 // out = (sum of the nblk partial sums, in a fixed order) * scale
@@ -365,9 +451,11 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__
 }
 
 int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
-  const int nblk = std::min(8192, c->n_cu * 16);
+  int kc = 0;
+  const dim3 grid = residual_grid(c, &kc);
+  const int nblk = (int)(grid.x * grid.y * grid.z);
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual<true>, dim3(nblk), dim3(256), 0, c->stream, f, phi, c->geo, rdx2, rdz2, r, nullptr,
+  hipLaunchKernelGGL(k_residual<true>, grid, dim3(64, 4), 0, c->stream, f, phi, c->geo, rdx2, rdz2, kc, r, nullptr,
                      c->pp.d_psum);
   PTC_HIP(hipGetLastError());
   // the x-z mean of phi's first row: the (0,0) mode's pinned value, which no
@@ -379,14 +467,17 @@ int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
 }
 
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax) {
-  const int nblk = std::min(2048, c->n_cu * 8);
+  int kc = 0;
+  const dim3 grid = residual_grid(c, &kc);
+  const int nblk = (int)(grid.x * grid.y * grid.z);
   if (c->red_cap < nblk) {
     if (c->d_red) (void)hipFree(c->d_red);
+    c->d_red = nullptr;
     PTC_HIP(hipMalloc(&c->d_red, sizeof(double) * nblk));
     c->red_cap = nblk;
   }
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual<false>, dim3(nblk), dim3(256), 0, c->stream, src, dst, c->geo, rdx2, rdz2, nullptr,
+  hipLaunchKernelGGL(k_residual<false>, grid, dim3(64, 4), 0, c->stream, src, dst, c->geo, rdx2, rdz2, kc, nullptr,
                      c->d_red, nullptr);
   PTC_HIP(hipGetLastError());
   std::vector<double> h((size_t)nblk);

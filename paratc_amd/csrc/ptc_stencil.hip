@@ -257,6 +257,362 @@ __global__ void __launch_bounds__(256) k_rhs_xyz(Geo g, RhsCoef cf, const double
   }
 }
 
+// ---- clcRhsXYZ, row-marching form -----------------------------------------
+// The same right-hand sides as k_rhs_xyz, for rows of at most 1024 points: a
+// workgroup owns one whole x row (one thread per column) at fixed j and
+// marches along k over planes [k0, k1).  Each thread keeps its column's values
+// in register windows along z (ux, uy, uz, scalar: k-3..k+3; p and uy at j+1:
+// k-2..k+1; uz at j-1: k-1..k+2), so a value is loaded once per plane instead
+// of once per use, and the next plane's loads are in flight while a plane is
+// computed.  The x neighbours are the other threads' values of the same plane,
+// exchanged through LDS rows; x wraps inside the row.  The two products whose
+// interpolations cross x and z, uxhz * uzhx in h13 (a z difference) and
+// uzhx * (uxhz + ucrf) in h31 (an x difference), are formed two planes ahead
+// and kept in short queues.  Operand order follows k_rhs_xyz except that
+// those products are formed before their difference weight is applied.
+enum {
+  X_UX = 0,  // ux(., j, k)
+  X_UY,      // uy(., j, k)
+  X_UZ,      // uz(., j, k)
+  X_UZ2,     // uz(., j, k + 2)
+  X_P,       // p(., j, k)
+  X_UXM,     // ux(., j - 1, k)
+  X_UYP,     // uy(., j + 1, k)
+  X_R,       // uzhx * (uxhz + ucrf) at (., j, k)
+  X_SC,      // scalar(., j, k)
+  X_NROWS
+};
+
+// Raw buffer access to one plane: the plane's base address and size live in
+// scalar registers and a lane adds a 32-bit byte offset, so a load costs one
+// VGPR for the address whatever the number of fields in flight.  Offsets past
+// the plane read zero and drop stores (the row kernels never form them).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const double *base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+  using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+}
+
+template <bool SC, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, const double *__restrict__ ux,
+                                                  const double *__restrict__ uy, const double *__restrict__ uz,
+                                                  const double *__restrict__ sc, const double *__restrict__ p,
+                                                  double *__restrict__ rhsx, double *__restrict__ rhsy,
+                                                  double *__restrict__ rhsz, double *__restrict__ rhsc,
+                                                  double *__restrict__ hx, double *__restrict__ hy,
+                                                  double *__restrict__ hz, double *__restrict__ hc,
+                                                  double *__restrict__ s3part) {
+  extern __shared__ double xr[];  // X_NROWS rows of nx values
+  const int nx = g.nx, nz = g.nz;
+  const int i = threadIdx.x;
+  const bool act = i < nx;
+  const int ic = act ? i : 0;
+  const int j = blockIdx.y + 1, jp = j + 1;
+  const int nyp = g.ny + 1;
+  const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, nz);
+  const long long pl = g.plane;
+  const double *M = g.met;
+  const int W = g.nyh;
+  const double rdyp = M[M_RDYP * W + j], rdyc = M[M_RDYC * W + j];
+  const double IY1 = M[M_YINT * W + j], IY2 = 1.0 - IY1;
+  const double IY3 = M[M_YINT * W + jp], IY4 = 1.0 - IY3;
+  const double c1 = g.ic1, c2 = g.ic2, c3 = g.ic3, c4 = g.ic4;
+  const double ucrf = cf.ucrf;
+  const bool top = jp == nyp;  // uy interpolations on row ny+1 are zero (src/m_TScheme.f90:157-160)
+  // wrapped x neighbours i-3..i+3
+  int nb[7];
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    int q = ic + s - 3;
+    q = q < 0 ? q + nx : (q >= nx ? q - nx : q);
+    nb[s] = q;
+  }
+  auto X = [&](int row, int d) { return xr[row * nx + nb[d + 3]]; };
+  auto wz = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
+  auto at = [&](int kk) { return (long long)wz(kk) * pl; };
+  // byte offsets of this column at rows j, j-1, j+1 inside a plane: loads and
+  // stores address a wave-uniform plane pointer plus a 32-bit lane offset
+  const unsigned bj = 8u * (unsigned)(ic + nx * j), bjm = bj - 8u * nx, bjp = bj + 8u * nx;
+  const int pbytes = (int)(8 * pl);
+  auto ldp = [&](const double *a, int kk, unsigned off) { return bld(plane_rsrc(a + at(kk), pbytes), off); };
+
+  double wux[7], wuy[7], wuz[7], wsc[7], wp[4], wuyp[4], wuzm[4];
+  double ux_m, ux_p, uy_m, uz_p, p_m, sc_m = 0.0, sc_p = 0.0;
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    wux[s] = ldp(ux, k0 + s - 3, bj);
+    wuy[s] = ldp(uy, k0 + s - 3, bj);
+    wuz[s] = ldp(uz, k0 + s - 3, bj);
+    wsc[s] = SC ? ldp(sc, k0 + s - 3, bj) : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    wp[s] = ldp(p, k0 + s - 2, bj);
+    wuyp[s] = ldp(uy, k0 + s - 2, bjp);
+    wuzm[s] = ldp(uz, k0 + s - 1, bjm);
+  }
+  {
+    ux_m = ldp(ux, k0, bjm);
+    ux_p = ldp(ux, k0, bjp);
+    uy_m = ldp(uy, k0, bjm);
+    uz_p = ldp(uz, k0, bjp);
+    p_m = ldp(p, k0, bjm);
+    if (SC) {
+      sc_m = ldp(sc, k0, bjm);
+      sc_p = ldp(sc, k0, bjp);
+    }
+  }
+  // queues: Q(k') = uxhz * uzhx for k' = k-1..k+2, R(k') = uzhx * (uxhz + ucrf)
+  // for k' = k..k+2, uzhx from the uz rows of planes k0-1..k0+1 first
+  double Q[4], R[3];
+  auto uxhz_at = [&](int s) {  // uxhz at plane k-3+s+2 from the ux window entries s..s+3
+    return c1 * wux[s] + c2 * wux[s + 1] + c3 * wux[s + 2] + c4 * wux[s + 3];
+  };
+  if (act) {
+    xr[0 * nx + i] = wuz[2];
+    xr[1 * nx + i] = wuz[3];
+    xr[2 * nx + i] = wuz[4];
+  }
+  __syncthreads();
+  {
+    double zx[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) zx[r] = c1 * X(r, -2) + c2 * X(r, -1) + c3 * X(r, 0) + c4 * X(r, 1);
+    const double hz0 = uxhz_at(0), hz1 = uxhz_at(1), hz2 = uxhz_at(2);  // planes k0-1, k0, k0+1
+    Q[0] = hz0 * zx[0];
+    Q[1] = hz1 * zx[1];
+    Q[2] = hz2 * zx[2];
+    R[0] = zx[1] * (hz1 + ucrf);
+    R[1] = zx[2] * (hz2 + ucrf);
+  }
+  double s3 = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const long long kp = (long long)k * pl;
+    const double hxo = bld(plane_rsrc(hx + kp, pbytes), bj), hyo = bld(plane_rsrc(hy + kp, pbytes), bj),
+                 hzo = bld(plane_rsrc(hz + kp, pbytes), bj);
+    const double hco = SC ? bld(plane_rsrc(hc + kp, pbytes), bj) : 0.0;
+    __syncthreads();  // the previous plane's reads are done
+    if (act) {
+      xr[X_UX * nx + i] = wux[3];
+      xr[X_UY * nx + i] = wuy[3];
+      xr[X_UZ * nx + i] = wuz[3];
+      xr[X_UZ2 * nx + i] = wuz[5];
+      xr[X_P * nx + i] = wp[2];
+      xr[X_UXM * nx + i] = ux_m;
+      xr[X_UYP * nx + i] = wuyp[2];
+      xr[X_R * nx + i] = R[0];
+      if (SC) xr[X_SC * nx + i] = wsc[3];
+    }
+    __syncthreads();
+    // the products two planes ahead
+    {
+      const double zx2 = c1 * X(X_UZ2, -2) + c2 * X(X_UZ2, -1) + c3 * X(X_UZ2, 0) + c4 * X(X_UZ2, 1);
+      const double hz2 = uxhz_at(3);  // plane k+2
+      Q[3] = hz2 * zx2;
+      R[2] = zx2 * (hz2 + ucrf);
+    }
+    double xu[7], xy[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      xu[s] = s == 3 ? wux[3] : X(X_UX, s - 3);
+      xy[s] = s == 3 ? wuy[3] : X(X_UY, s - 3);
+    }
+    const double ux0 = wux[3], uy0 = wuy[3], uz0 = wuz[3];
+
+    // ---- RhsX (src/m_TScheme.f90:169-204)
+    double rx, cx, d22q1, dcq13x;
+    {
+      const double uxhyc = IY1 * ux_m + IY2 * ux0;
+      const double uxhyp = IY3 * ux0 + IY4 * ux_p;
+      const double a0 = c1 * xu[0] + c2 * xu[1] + c3 * xu[2] + c4 * xu[3];
+      const double a1 = c1 * xu[1] + c2 * xu[2] + c3 * xu[3] + c4 * xu[4];
+      const double a2 = c1 * xu[2] + c2 * xu[3] + c3 * xu[4] + c4 * xu[5];
+      const double a3 = c1 * xu[3] + c2 * xu[4] + c3 * xu[5] + c4 * xu[6];
+      const double h11 = g.dxc1 * a0 * (a0 + ucrf) + g.dxc2 * a1 * (a1 + ucrf) + g.dxc3 * a2 * (a2 + ucrf) +
+                         g.dxc4 * a3 * (a3 + ucrf);
+      const double uyhx_p =
+          top ? 0.0 : c1 * X(X_UYP, -2) + c2 * X(X_UYP, -1) + c3 * wuyp[2] + c4 * X(X_UYP, 1);
+      const double uyhx_c = c1 * xy[1] + c2 * xy[2] + c3 * xy[3] + c4 * xy[4];
+      const double h12 = (uyhx_p * uxhyp - uyhx_c * uxhyc) * rdyp;
+      const double h13 = g.dzc1 * Q[0] + g.dzc2 * Q[1] + g.dzc3 * Q[2] + g.dzc4 * Q[3];
+      const double d11 =
+          g.dxxc1 * xu[1] + g.dxxc2 * xu[2] + g.dxxc3 * xu[3] + g.dxxc4 * xu[4] + g.dxxc5 * xu[5];
+      d22q1 = M[M_AP2C * W + j] * ux_p + M[M_AC2C * W + j] * ux0 + M[M_AM2C * W + j] * ux_m;
+      const double d33 =
+          g.dzzc1 * wux[1] + g.dzzc2 * wux[2] + g.dzzc3 * wux[3] + g.dzzc4 * wux[4] + g.dzzc5 * wux[5];
+      dcq13x = d11 + d33;
+      cx = -h11 - h12 - h13 + cf.xnu * dcq13x + cf.gx;
+      const double gp = g.dxc1 * X(X_P, -2) + g.dxc2 * X(X_P, -1) + g.dxc3 * wp[2] + g.dxc4 * X(X_P, 1);
+      rx = cf.gamma * cx + cf.theta * hxo - cf.alpha * gp + 2.0 * cf.beta * d22q1;
+    }
+    if (act) {
+      if (cf.s3_mode == 1) s3 += (dcq13x + d22q1) * M[M_DYP * W + j];
+      else if (cf.s3_mode == 2) s3 += ux0 * M[M_DYP * W + j];
+    }
+
+    // ---- RhsY (:206-236)
+    double ry, cy;
+    {
+      const double qs = 0.25 * rdyc;
+      const double uxhym = IY1 * X(X_UXM, -1) + IY2 * xu[2] + ucrf;
+      const double uxhyc = IY1 * ux_m + IY2 * xu[3] + ucrf;
+      const double uxhyp = IY1 * X(X_UXM, 1) + IY2 * xu[4] + ucrf;
+      const double uxhyu = IY1 * X(X_UXM, 2) + IY2 * xu[5] + ucrf;
+      const double uzhym = IY1 * wuzm[0] + IY2 * wuz[2];
+      const double uzhyc = IY1 * wuzm[1] + IY2 * wuz[3];
+      const double uzhyp = IY1 * wuzm[2] + IY2 * wuz[4];
+      const double uzhyu = IY1 * wuzm[3] + IY2 * wuz[5];
+      const double y0 = c1 * xy[0] + c2 * xy[1] + c3 * xy[2] + c4 * xy[3];
+      const double y1 = c1 * xy[1] + c2 * xy[2] + c3 * xy[3] + c4 * xy[4];
+      const double y2 = c1 * xy[2] + c2 * xy[3] + c3 * xy[4] + c4 * xy[5];
+      const double y3 = c1 * xy[3] + c2 * xy[4] + c3 * xy[5] + c4 * xy[6];
+      const double h21 = g.dxc1 * y0 * uxhym + g.dxc2 * y1 * uxhyc + g.dxc3 * y2 * uxhyp + g.dxc4 * y3 * uxhyu;
+      const double s1 = wuyp[2] + uy0;
+      const double s2 = uy0 + uy_m;
+      const double h22 = (s1 * s1 - s2 * s2) * qs;
+      const double z0 = c1 * wuy[0] + c2 * wuy[1] + c3 * wuy[2] + c4 * wuy[3];
+      const double z1 = c1 * wuy[1] + c2 * wuy[2] + c3 * wuy[3] + c4 * wuy[4];
+      const double z2 = c1 * wuy[2] + c2 * wuy[3] + c3 * wuy[4] + c4 * wuy[5];
+      const double z3 = c1 * wuy[3] + c2 * wuy[4] + c3 * wuy[5] + c4 * wuy[6];
+      const double h23 = g.dzc1 * z0 * uzhym + g.dzc2 * z1 * uzhyc + g.dzc3 * z2 * uzhyp + g.dzc4 * z3 * uzhyu;
+      const double d11 =
+          g.dxxc1 * xy[1] + g.dxxc2 * xy[2] + g.dxxc3 * xy[3] + g.dxxc4 * xy[4] + g.dxxc5 * xy[5];
+      const double d22 = M[M_AP2P * W + j] * wuyp[2] + M[M_AC2P * W + j] * uy0 + M[M_AM2P * W + j] * uy_m;
+      const double d33 =
+          g.dzzc1 * wuy[1] + g.dzzc2 * wuy[2] + g.dzzc3 * wuy[3] + g.dzzc4 * wuy[4] + g.dzzc5 * wuy[5];
+      cy = -h21 - h22 - h23 + cf.xnu * (d11 + d33) + cf.gy;
+      const double gp = (wp[2] - p_m) * rdyc;
+      ry = cf.gamma * cy + cf.theta * hyo - cf.alpha * gp + 2.0 * cf.beta * d22;
+    }
+
+    // ---- RhsZ (:238-267)
+    double rz, cz;
+    {
+      const double uzhyc = IY1 * wuzm[1] + IY2 * uz0;
+      const double uzhyp = IY3 * uz0 + IY4 * uz_p;
+      const double h31 = g.dxc1 * X(X_R, -1) + g.dxc2 * R[0] + g.dxc3 * X(X_R, 1) + g.dxc4 * X(X_R, 2);
+      const double uyhz_p = top ? 0.0 : c1 * wuyp[0] + c2 * wuyp[1] + c3 * wuyp[2] + c4 * wuyp[3];
+      const double uyhz_c = c1 * wuy[1] + c2 * wuy[2] + c3 * wuy[3] + c4 * wuy[4];
+      const double h32 = (uyhz_p * uzhyp - uyhz_c * uzhyc) * rdyp;
+      const double b0 = c1 * wuz[0] + c2 * wuz[1] + c3 * wuz[2] + c4 * wuz[3];
+      const double b1 = c1 * wuz[1] + c2 * wuz[2] + c3 * wuz[3] + c4 * wuz[4];
+      const double b2 = c1 * wuz[2] + c2 * wuz[3] + c3 * wuz[4] + c4 * wuz[5];
+      const double b3 = c1 * wuz[3] + c2 * wuz[4] + c3 * wuz[5] + c4 * wuz[6];
+      const double h33 = g.dzc1 * b0 * b0 + g.dzc2 * b1 * b1 + g.dzc3 * b2 * b2 + g.dzc4 * b3 * b3;
+      const double d11 = g.dxxc1 * X(X_UZ, -2) + g.dxxc2 * X(X_UZ, -1) + g.dxxc3 * uz0 + g.dxxc4 * X(X_UZ, 1) +
+                         g.dxxc5 * X(X_UZ, 2);
+      const double d22 = M[M_AP2C * W + j] * uz_p + M[M_AC2C * W + j] * uz0 + M[M_AM2C * W + j] * wuzm[1];
+      const double d33 =
+          g.dzzc1 * wuz[1] + g.dzzc2 * wuz[2] + g.dzzc3 * wuz[3] + g.dzzc4 * wuz[4] + g.dzzc5 * wuz[5];
+      cz = -h31 - h32 - h33 + cf.xnu * (d11 + d33) + cf.gz;
+      const double gp = g.dzc1 * wp[0] + g.dzc2 * wp[1] + g.dzc3 * wp[2] + g.dzc4 * wp[3];
+      rz = cf.gamma * cz + cf.theta * hzo - cf.alpha * gp + 2.0 * cf.beta * d22;
+    }
+
+    if (SC) {
+      // ---- scalar transport (:302-357)
+      double xs[7];
+#pragma unroll
+      for (int s = 0; s < 7; ++s) xs[s] = s == 3 ? wsc[3] : X(X_SC, s - 3);
+      const double sh_m = c1 * xs[0] + c2 * xs[1] + c3 * xs[2] + c4 * xs[3];
+      const double x0 = c1 * xs[1] + c2 * xs[2] + c3 * xs[3] + c4 * xs[4];
+      const double sh_p = c1 * xs[2] + c2 * xs[3] + c3 * xs[4] + c4 * xs[5];
+      const double sh_u = c1 * xs[3] + c2 * xs[4] + c3 * xs[5] + c4 * xs[6];
+      const double sz_m = c1 * wsc[0] + c2 * wsc[1] + c3 * wsc[2] + c4 * wsc[3];
+      const double z0 = c1 * wsc[1] + c2 * wsc[2] + c3 * wsc[3] + c4 * wsc[4];
+      const double sz_p = c1 * wsc[2] + c2 * wsc[3] + c3 * wsc[4] + c4 * wsc[5];
+      const double sz_u = c1 * wsc[3] + c2 * wsc[4] + c3 * wsc[5] + c4 * wsc[6];
+      const double shyc = IY1 * sc_m + IY2 * wsc[3];
+      const double shyp = IY3 * wsc[3] + IY4 * sc_p;
+      const double e1 = cf.gev[0] * x0, e2 = cf.gev[1] * shyc, e3 = cf.gev[2] * z0;
+      rx = rx + cf.gamma * e1;
+      ry = ry + cf.gamma * e2;
+      rz = rz + cf.gamma * e3;
+      cx = cx + e1;
+      cy = cy + e2;
+      cz = cz + e3;
+      const double f0 = cf.fvv[0], f1 = cf.fvv[1], f2 = cf.fvv[2];
+      const double hc1 = g.dxc1 * (xu[2] + f0) * sh_m + g.dxc2 * (xu[3] + f0) * x0 + g.dxc3 * (xu[4] + f0) * sh_p +
+                         g.dxc4 * (xu[5] + f0) * sh_u;
+      const double hc2 = ((wuyp[2] + f1) * shyp - (uy0 + f1) * shyc) * rdyp;
+      const double hc3 = g.dzc1 * (wuz[2] + f2) * sz_m + g.dzc2 * (wuz[3] + f2) * z0 + g.dzc3 * (wuz[4] + f2) * sz_p +
+                         g.dzc4 * (wuz[5] + f2) * sz_u;
+      const double d11 =
+          g.dxxc1 * xs[1] + g.dxxc2 * xs[2] + g.dxxc3 * xs[3] + g.dxxc4 * xs[4] + g.dxxc5 * xs[5];
+      const double d22 = M[M_AP2CSC * W + j] * sc_p + M[M_AC2CSC * W + j] * wsc[3] + M[M_AM2CSC * W + j] * sc_m;
+      const double d33 =
+          g.dzzc1 * wsc[1] + g.dzzc2 * wsc[2] + g.dzzc3 * wsc[3] + g.dzzc4 * wsc[4] + g.dzzc5 * wsc[5];
+      const double ccv = -hc1 - hc2 - hc3 + cf.sdiff * (d11 + d33);
+      if (act) {
+        bst(plane_rsrc(rhsc + kp, pbytes), bj, cf.gamma * ccv + cf.theta * hco + 2.0 * cf.beta_sc * d22);
+        bst(plane_rsrc(hc + kp, pbytes), bj, ccv);
+      }
+    }
+    if (act) {
+      bst(plane_rsrc(rhsx + kp, pbytes), bj, rx);
+      bst(plane_rsrc(rhsy + kp, pbytes), bj, ry);
+      bst(plane_rsrc(rhsz + kp, pbytes), bj, rz);
+      bst(plane_rsrc(hx + kp, pbytes), bj, cx);
+      bst(plane_rsrc(hy + kp, pbytes), bj, cy);
+      bst(plane_rsrc(hz + kp, pbytes), bj, cz);
+    }
+
+    // advance one plane: shift the windows and queues, load the new entries
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      wux[s] = wux[s + 1];
+      wuy[s] = wuy[s + 1];
+      wuz[s] = wuz[s + 1];
+      if (SC) wsc[s] = wsc[s + 1];
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      wp[s] = wp[s + 1];
+      wuyp[s] = wuyp[s + 1];
+      wuzm[s] = wuzm[s + 1];
+      Q[s] = Q[s + 1];
+    }
+    R[0] = R[1];
+    R[1] = R[2];
+    {
+      wux[6] = ldp(ux, k + 4, bj);
+      wuy[6] = ldp(uy, k + 4, bj);
+      wuz[6] = ldp(uz, k + 4, bj);
+      if (SC) wsc[6] = ldp(sc, k + 4, bj);
+      wp[3] = ldp(p, k + 2, bj);
+      wuyp[3] = ldp(uy, k + 2, bjp);
+      wuzm[3] = ldp(uz, k + 3, bjm);
+      ux_m = ldp(ux, k + 1, bjm);
+      ux_p = ldp(ux, k + 1, bjp);
+      uy_m = ldp(uy, k + 1, bjm);
+      uz_p = ldp(uz, k + 1, bjp);
+      p_m = ldp(p, k + 1, bjm);
+      if (SC) {
+        sc_m = ldp(sc, k + 1, bjm);
+        sc_p = ldp(sc, k + 1, bjp);
+      }
+    }
+  }
+  if (cf.s3_mode) {
+    // fixed-order pairwise tree over the workgroup's threads
+    __syncthreads();
+    xr[i] = s3;
+    __syncthreads();
+    for (int n = blockDim.x; n > 1;) {
+      const int h = (n + 1) / 2;
+      if (i < n - h) xr[i] += xr[i + h];
+      __syncthreads();
+      n = h;
+    }
+    if (i == 0) s3part[blockIdx.y + gridDim.y * blockIdx.z] = xr[0];
+  }
+}
+
 // constant flow rate (:269-300): sums the partials in a fixed order and stores
 // the shift added to RhsX and the PrGradAver increment
 __global__ void __launch_bounds__(256) k_flowrate(const double *__restrict__ part, int n, double *__restrict__ scal,
@@ -470,6 +826,25 @@ static int pt_nblocks(const ptc_ctx *c) {
   return (int)(gr.x * gr.y * gr.z);
 }
 
+// row-marching RHS (rows of at most 1024 points): one workgroup per (j, k
+// chunk); the chunks are at least 8 planes long and many enough for about
+// 24 wavefronts per CU over the launch
+constexpr int kRhsRowsMaxNx = 1024;
+static bool rhs_rows(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx; }
+static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
+  const int wpb = (c->nx + 63) / 64;
+  const int want = (24 * c->n_cu + c->ny * wpb - 1) / (c->ny * wpb);
+  const int nch = std::max(1, std::min(want, c->nz / 8));
+  *kc = (c->nz + nch - 1) / nch;
+  return dim3(1u, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
+}
+static int rhs_nblocks(const ptc_ctx *c) {
+  if (!rhs_rows(c)) return pt_nblocks(c);
+  int kc = 0;
+  const dim3 gr = rhs_rows_grid(c, &kc);
+  return (int)(gr.x * gr.y * gr.z);
+}
+
 static int ensure_red(ptc_ctx *c, int n) {
   if (c->red_cap < n) {
     if (c->d_red) (void)hipFree(c->d_red);
@@ -501,10 +876,31 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   }
   cf.scalar = c->prm.scalar ? 1 : 0;
   cf.s3_mode = c->prm.is_ux_const ? (c->prm.scalar ? 2 : 1) : 0;
-  const int nb = pt_nblocks(c);
+  const int nb = rhs_nblocks(c);
   PTC_TRY(ensure_red(c, nb));
-  hipLaunchKernelGGL(k_rhs_xyz, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, cf, ux, uy, uz, sc, p, rhsx, rhsy,
-                     rhsz, rhsc, hx, hy, hz, hc, c->d_red);
+  if (rhs_rows(c)) {
+    int kc = 0;
+    const dim3 grid = rhs_rows_grid(c, &kc);
+    const int nt = (c->nx + 63) / 64 * 64;
+    const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * c->nx, nt);
+#define PTC_ROWS(SCAL, MAXT)                                                                                    \
+  hipLaunchKernelGGL((k_rhs_rows<SCAL, MAXT>), grid, dim3(nt), lds, c->stream, c->geo, cf, kc, ux, uy, uz, sc, p, \
+                     rhsx, rhsy, rhsz, rhsc, hx, hy, hz, hc, c->d_red)
+    if (nt <= 256) {
+      if (cf.scalar) PTC_ROWS(true, 256);
+      else PTC_ROWS(false, 256);
+    } else if (nt <= 512) {
+      if (cf.scalar) PTC_ROWS(true, 512);
+      else PTC_ROWS(false, 512);
+    } else {
+      if (cf.scalar) PTC_ROWS(true, 1024);
+      else PTC_ROWS(false, 1024);
+    }
+#undef PTC_ROWS
+  } else {
+    hipLaunchKernelGGL(k_rhs_xyz, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, cf, ux, uy, uz, sc, p, rhsx, rhsy,
+                       rhsz, rhsc, hx, hy, hz, hc, c->d_red);
+  }
   PTC_HIP(hipGetLastError());
   if (cf.s3_mode) {
     hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,
@@ -525,7 +921,7 @@ const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
 
 // reduction partials of the RHS and PrSrc kernels, allocated ahead of a
 // captured step
-int stencil_prepare(ptc_ctx *c) { return ensure_red(c, pt_nblocks(c)); }
+int stencil_prepare(ptc_ctx *c) { return ensure_red(c, std::max(pt_nblocks(c), rhs_nblocks(c))); }
 
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host) {

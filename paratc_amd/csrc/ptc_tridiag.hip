@@ -242,15 +242,33 @@ __global__ void __launch_bounds__(1024) k_tri_poisson_split(S *__restrict__ spec
   const int kx = blockIdx.x * (LPB / 2) + lane / 2, kz = blockIdx.y;
   const bool valid = kx <= h;
   const int s = p * MM;
-  const double lam = valid ? wnx[kx] + wnz[kz] : 1.0;
   const bool pin = (kx == 0 && kz == 0);
   // row base addresses are wave-uniform (scalar), the lane adds a 32-bit offset
   using R = decltype(spec->x);
   R *base = reinterpret_cast<R *>(spec + (long long)kz * ny * pitch) + (long long)blockIdx.x * LPB;
   const long long rs = 2LL * pitch;
   double f[MM];
+  double lam;
+  if constexpr (sizeof(R) == 8) {
+    // FP64 storage: predicated row loads (measured fastest for this width)
+    lam = valid ? wnx[kx] + wnz[kz] : 1.0;
 #pragma unroll
-  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? (double)base[(s + r) * rs + lane] : 0.0;
+    for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? base[(s + r) * rs + lane] : 0.0;
+  } else {
+    // FP32 storage: a predicated load feeding a conversion is compiled as
+    // load-wait-convert per row, so the loads are issued unconditionally
+    // (lanes past the last line read column 0, padding rows read row ny-1:
+    // finite values, masked by a multiply), the wavenumbers first: the
+    // elimination starts on row 0 while the later rows are still arriving.
+    const double wx_k = wnx[valid ? kx : 0], wz_k = wnz[kz];
+    const int lc = valid ? lane : -(int)(blockIdx.x * LPB);
+#pragma unroll
+    for (int r = 0; r < MM; ++r) {
+      const double v = (double)base[min(s + r, ny - 1) * rs + lc];
+      f[r] = v * ((valid && s + r < ny) ? 1.0 : 0.0);
+    }
+    lam = valid ? wx_k + wz_k : 1.0;
+  }
   if (pin && s == 0) f[0] = 0.0;  // phi(1) = 0 on the (0,0) line
   PoissonRowsRe getc{met + M_AM2PH * nyh, met + M_AC2PH * nyh, met + M_AP2PH * nyh, lam, ny, pin};
   double A[MM], C[MM];
@@ -326,6 +344,81 @@ __global__ void __launch_bounds__(512) k_tri_poisson_lds(S *__restrict__ spec, c
   }
 }
 
+// ---- Poisson y-solve, partitioned, one complex line per lane ----------------
+// The same elimination with the real and imaginary parts of a line in one
+// lane: the row coefficients (and so the divisions of the elimination and of
+// the cyclic reduction) are shared by the two parts instead of being formed
+// twice.  LC complex lines per workgroup, P chunks: P * LC threads, chunk
+// p = thread / LC.  A wavefront holds 64 / LC chunks, so the row
+// coefficients are per lane; they are staged in LDS once per workgroup.  A
+// row is a contiguous LC * 16-byte access (LC * 8 bytes with FP32 storage).
+// spectrum storage <-> FP64 arithmetic
+__device__ __forceinline__ double2 cld(const double2 &v) { return v; }
+__device__ __forceinline__ double2 cld(const float2 &v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void cst(double2 &d, double2 v) { d = v; }
+__device__ __forceinline__ void cst(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+
+// LDS of the complex-line kernel in doubles: the three coefficient rows
+// (padded to 16 bytes) and a, c, f (f complex) of 2P reduced rows x LC lines,
+// double-buffered
+constexpr size_t tri_cplx_lds_doubles(int nyh, int P, int LC) {
+  return (size_t)((3 * nyh + 1) / 2 * 2) + (size_t)16 * P * LC;
+}
+
+// chunks of 12 or more complex rows need more than 128 VGPRs: their
+// workgroups hold at most 512 threads
+constexpr int tri_cplx_cap(int mm) { return mm >= 12 ? 512 : 1024; }
+
+template <int MM, int LC, typename S>
+__global__ void __launch_bounds__(tri_cplx_cap(MM)) k_tri_poisson_cplx(S *__restrict__ spec, const double *__restrict__ met,
+                                                          int nyh, const double *__restrict__ wnx,
+                                                          const double *__restrict__ wnz, int ny, int pitch, int h,
+                                                          int P) {
+  extern __shared__ double smem[];
+  double *mA = smem, *mC = mA + nyh, *mP = mC + nyh;
+  double *RA = smem + (3 * nyh + 1) / 2 * 2;  // 16-byte aligned
+  double *RC = RA + 4 * P * LC;
+  double2 *RF = reinterpret_cast<double2 *>(RC + 4 * P * LC);
+  const int t = threadIdx.x;
+  const int lane = t % LC, p = t / LC;
+  const int kx = blockIdx.x * LC + lane, kz = blockIdx.y;
+  const bool valid = kx <= h;
+  const int s = p * MM;
+  const bool pin = (kx == 0 && kz == 0);
+  // wavenumbers first, then every row of the chunk (lanes past the last line
+  // read line 0, padding rows read row ny-1; masked by a multiply), all in
+  // flight while the coefficients go to LDS
+  const double wx_k = wnx[valid ? kx : 0], wz_k = wnz[kz];
+  S *col = spec + (long long)kz * ny * pitch + (valid ? kx : 0);
+  double2 f[MM];
+#pragma unroll
+  for (int r = 0; r < MM; ++r) {
+    const double2 v = cld(col[(long long)min(s + r, ny - 1) * pitch]);
+    const double m = (valid && s + r < ny) ? 1.0 : 0.0;
+    f[r] = make_double2(v.x * m, v.y * m);
+  }
+  for (int j = t; j < nyh; j += blockDim.x) {
+    mA[j] = met[M_AM2PH * nyh + j];
+    mC[j] = met[M_AC2PH * nyh + j];
+    mP[j] = met[M_AP2PH * nyh + j];
+  }
+  __syncthreads();
+  const double lam = valid ? wx_k + wz_k : 1.0;
+  if (pin && s == 0) f[0] = make_double2(0.0, 0.0);  // phi(1) = 0 on the (0,0) line
+  PoissonRowsRe getc{mA, mC, mP, lam, ny, pin};
+  double A[MM], C[MM];
+  double aS, cS, aE, cE;
+  double2 fS, fE, xs, xe;
+  chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
+  solve_reduced<LC>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
+  chunk_backsub<MM>(f, A, C, xs, xe);
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < MM; ++r)
+      if (s + r < ny) cst(col[(long long)(s + r) * pitch], f[r]);
+  }
+}
+
 // ---- Poisson y-solve, Thomas (reference InverseTridiagonal) --------------
 __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict__ spec, double *__restrict__ cs,
                                                            const double *__restrict__ met, int nyh,
@@ -390,7 +483,10 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *
   const double sh = shift ? shift[0] : 0.0;
   double f[MM];
 #pragma unroll
-  for (int r = 0; r < MM; ++r) f[r] = (valid && s + r < ny) ? rhs[col + (long long)nx * (s + r + 1)] + sh : 0.0;
+  for (int r = 0; r < MM; ++r) {  // unconditional loads (padding rows read row ny), masked by a multiply
+    const double v = rhs[col + (long long)nx * (min(s + r, ny - 1) + 1)];
+    f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
+  }
   if (kind == 1 && s == 0) f[0] = valid ? duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
   CnRows getc{met + rowA * nyh, met + rowB * nyh, met + rowC * nyh, beta, ny, kind};
   double A[MM], C[MM];
@@ -417,15 +513,17 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *
 struct TriShape {
   int mm, lpb, cap;  // cap: the instantiation's launch bound
 };
-static const TriShape kPoissonShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {12, 64, 1024}, {16, 64, 1024}};
+static const TriShape kPoissonShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {12, 64, 1024}, {16, 64, 1024},
+                                          {8, 32, 1024},  {12, 32, 1024}, {16, 32, 512},  {16, 16, 256}};
 static const TriShape kCnShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {8, 32, 1024}, {16, 32, 512}};
 
-static int tri_choose(int ny, size_t lds_limit, int mm_req, const TriShape *shapes, int nshapes, int *P_out,
-                      int *mm_out, int *lpb_out) {
+static int tri_choose(int ny, size_t lds_limit, int mm_req, int lpb_req, const TriShape *shapes, int nshapes,
+                      int *P_out, int *mm_out, int *lpb_out) {
   for (int pass = 0; pass < 2; ++pass)
     for (int q = 0; q < nshapes; ++q) {
       const int mm = shapes[q].mm, lpb = shapes[q].lpb;
       if (mm_req > 0 && mm != mm_req) continue;
+      if (lpb_req > 0 && lpb != lpb_req) continue;
       const int P = (ny + mm - 1) / mm;
       if (lpb * P > shapes[q].cap) continue;
       if (2 * P * lpb > 1 << 16) continue;
@@ -438,6 +536,7 @@ static int tri_choose(int ny, size_t lds_limit, int mm_req, const TriShape *shap
       return PTC_OK;
     }
   set_error("ny = " + std::to_string(ny) + (mm_req > 0 ? " with " + std::to_string(mm_req) + "-row chunks" : "") +
+            (lpb_req > 0 ? " and " + std::to_string(lpb_req) + " lines per workgroup" : "") +
             " is outside the partitioned tridiagonal kernels");
   return PTC_E_UNSUPPORTED;
 }
@@ -466,9 +565,29 @@ int tri_plan_lds(int ny, size_t lds_limit, int ll, int mm_req, int *P_out, int *
   return PTC_E_UNSUPPORTED;
 }
 
-int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int *P_out, int *mm_out, int *lpb_out) {
-  return poisson ? tri_choose(ny, lds_limit, mm_req, kPoissonShapes, 4, P_out, mm_out, lpb_out)
-                 : tri_choose(ny, lds_limit, mm_req, kCnShapes, 4, P_out, mm_out, lpb_out);
+// complex-line kernel: P * CL threads within the instantiation's launch
+// bound, LDS = coefficients + the double-buffered reduction rows
+int tri_plan_cplx(int ny, size_t lds_limit, int cl, int mm_req, int *P_out, int *mm_out) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int mm : {4, 8, 12, 16}) {
+      if (mm_req > 0 && mm != mm_req) continue;
+      const int P = (ny + mm - 1) / mm;
+      if (P * cl > tri_cplx_cap(mm)) continue;
+      if (pass == 0 && (P * mm - ny) * 8 > ny) continue;
+      if (sizeof(double) * tri_cplx_lds_doubles(ny + 2, P, cl) > lds_limit) continue;
+      *P_out = P;
+      *mm_out = mm;
+      return PTC_OK;
+    }
+  set_error("ny = " + std::to_string(ny) + " does not fit the complex-line tridiagonal kernel with " +
+            std::to_string(cl) + " lines per workgroup");
+  return PTC_E_UNSUPPORTED;
+}
+
+int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int lpb_req, int *P_out, int *mm_out,
+             int *lpb_out) {
+  return poisson ? tri_choose(ny, lds_limit, mm_req, lpb_req, kPoissonShapes, 8, P_out, mm_out, lpb_out)
+                 : tri_choose(ny, lds_limit, mm_req, lpb_req, kCnShapes, 4, P_out, mm_out, lpb_out);
 }
 
 int launch_tri_poisson(ptc_ctx *c, void *spec) {
@@ -482,6 +601,33 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
                        c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, c->nz);
     PTC_HIP(hipGetLastError());
     return PTC_OK;
+  }
+  if (P.tri_cl > 0) {
+    const int nP = P.tri_chunks, cl = P.tri_cl;
+    dim3 grid((unsigned)((P.h + 1 + cl - 1) / cl), (unsigned)c->nz);
+    const size_t lds = sizeof(double) * tri_cplx_lds_doubles(c->ny + 2, nP, cl);
+#define PTC_CPLX(MM, CL)                                                                                       \
+  if (P.tri_mmax == MM && cl == CL) {                                                                          \
+    if (f32)                                                                                                   \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, float2>), grid, dim3(CL * nP), lds, c->stream,            \
+                         (float2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);    \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, double2>), grid, dim3(CL * nP), lds, c->stream,           \
+                         (double2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);   \
+    PTC_HIP(hipGetLastError());                                                                                \
+    return PTC_OK;                                                                                             \
+  }
+    PTC_CPLX(4, 16)
+    PTC_CPLX(8, 16)
+    PTC_CPLX(12, 16)
+    PTC_CPLX(16, 16)
+    PTC_CPLX(4, 32)
+    PTC_CPLX(8, 32)
+    PTC_CPLX(12, 32)
+    PTC_CPLX(16, 32)
+#undef PTC_CPLX
+    set_error("bad complex-line tridiagonal launch shape");
+    return PTC_E_STATE;
   }
   if (P.tri_ll > 0) {
     const int nP = P.tri_chunks, ll = P.tri_ll;
@@ -529,6 +675,10 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
   PTC_CASE(8, 64)
   PTC_CASE(12, 64)
   PTC_CASE(16, 64)
+  PTC_CASE(8, 32)
+  PTC_CASE(12, 32)
+  PTC_CASE(16, 32)
+  PTC_CASE(16, 16)
 #undef PTC_CASE
   set_error("bad tridiagonal launch shape");
   return PTC_E_STATE;
@@ -538,7 +688,7 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
                   const double *shift) {
   int nP = 1, mmax = 16, lpb = 64;
-  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, &nP, &mmax, &lpb));
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
   int rA, rB, rC;
   if (kind == 0) { rA = M_AM2CCN; rB = M_AC2CCN; rC = M_AP2CCN; }
   else if (kind == 1) { rA = M_AM2P; rB = M_AC2P; rC = M_AP2P; }

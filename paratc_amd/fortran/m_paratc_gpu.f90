@@ -48,12 +48,14 @@ module m_paratc_gpu
   public :: ptc_clc_u3hat, ptc_clc_scalar, ptc_clc_prsrc, ptc_fluid_vel_update, ptc_pressure_update
   public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
   public :: ptc_clc_mean_value, ptc_poisson_config, ptc_poisson_set, ptc_poisson_get_config
+  public :: ptc_poisson_get_knob
   public :: gpu_check, gpu_put, gpu_get, gpu_clcPPE
 
   integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1
   integer(c_int), parameter, public :: PTC_TRI_AUTO = 0, PTC_TRI_THOMAS = 1, PTC_TRI_PARTITIONED = 2
   integer(c_int), parameter, public :: PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2
   integer(c_int), parameter, public :: PTC_POISSON_TRI_LINES = 3, PTC_POISSON_FAST_ZTILE = 4
+  integer(c_int), parameter, public :: PTC_POISSON_TRI_LANES = 5, PTC_POISSON_TRI_COMPLEX = 6
 
   interface
     integer(c_int) function ptc_create(grid, prm, yp, ctx) bind(C, name="ptc_create")
@@ -229,6 +231,12 @@ module m_paratc_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: knob, val
+    end function
+    integer(c_int) function ptc_poisson_get_knob(ctx, knob, val) bind(C, name="ptc_poisson_get_knob")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: knob
+      integer(c_int), intent(out) :: val
     end function
     integer(c_int) function ptc_poisson_get_config(ctx, out10) bind(C, name="ptc_poisson_get_config")
       import :: c_int, c_ptr

@@ -132,6 +132,17 @@ class Solver:
     def poisson_tri_lines(self, lines: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_LINES, lines), "ptc_poisson_set")
 
+    def poisson_tri_lanes(self, lanes: int = 0):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_LANES, lanes), "ptc_poisson_set")
+
+    def poisson_tri_complex(self, lines: int = 0):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_COMPLEX, lines), "ptc_poisson_set")
+
+    def poisson_get_knob(self, knob: int) -> int:
+        v = C.c_int()
+        check(self.L.ptc_poisson_get_knob(self.h, knob, C.byref(v)), "ptc_poisson_get_knob")
+        return v.value
+
     def poisson_get_config(self) -> dict:
         o = (C.c_int * 10)()
         check(self.L.ptc_poisson_get_config(self.h, o), "ptc_poisson_get_config")

@@ -49,3 +49,22 @@ def random_halo(shape_interior, seed, amp=1.0, stream=0):
     kk = (np.arange(-1, nz + 3) - 1) % nz
     h[:, :, :] = core[ii][:, :, kk]
     return h
+
+
+def apply_laplacian(o, case, p):
+    """The discrete Poisson operator clcPPE inverts (src/m_Poisson.f90:383-430)
+    applied to an interior field p: the modified-wavenumber symbols of the 4th-
+    order x/z second differences as their 7-point stencils (1, -54, 783, -1460,
+    783, -54, 1) / 576 dx^2 with periodic wrap, and am2ph / ac2ph / ap2ph in y
+    with the Neumann ghosts p_0 = p_1, p_{ny+1} = p_ny.  o: the oracle (its
+    metrics)."""
+    ny = case.ny
+    am, ac, ap = (o.metric(n)[1:ny + 1] for n in ("am2ph", "ac2ph", "ap2ph"))
+    rdx2, rdz2 = (case.nx / case.xlx) ** 2, (case.nz / case.zlz) ** 2
+    st = np.array([1, -54, 783, -1460, 783, -54, 1]) / 576.0
+    lp = np.zeros_like(p)
+    for s in range(-3, 4):
+        lp += st[s + 3] * (rdx2 * np.roll(p, -s, axis=0) + rdz2 * np.roll(p, -s, axis=2))
+    pm = np.concatenate([p[:, :1], p, p[:, -1:]], axis=1)
+    lp += am[None, :, None] * pm[:, :-2] + ac[None, :, None] * pm[:, 1:-1] + ap[None, :, None] * pm[:, 2:]
+    return lp

@@ -7,7 +7,7 @@ same operator, so they agree to rounding: the bar is 1e-12 relative (FP64).
 import numpy as np
 import pytest
 
-from helpers import inner, rel_err
+from helpers import apply_laplacian, inner, rel_err
 from oracle.oracle import Oracle
 from paratc_amd import capi, case as CS, ic
 from paratc_amd.solver import Solver
@@ -30,7 +30,7 @@ GRIDS = [
 ]
 
 
-def solve_both(grid, tri, seed=1, compat=True, rows=0, lines=0):
+def solve_both(grid, tri, seed=1, compat=True, rows=0, lines=0, lanes=0, cplx=0):
     nx, ny, nz, gam = grid
     case = CS.case_poisson(nx, ny, nz, gam)
     f = ic.poisson_rhs(case, seed=seed) if compat else np.asfortranarray(
@@ -50,6 +50,15 @@ def solve_both(grid, tri, seed=1, compat=True, rows=0, lines=0):
             cfg = s.poisson_get_config()
             assert cfg["tri"] == "partitioned" and cfg["chunk_rows"] == rows, cfg
             assert cfg["tri_lds_lines"] == lines, cfg
+        if lanes:
+            s.poisson_tri_lanes(lanes)
+            cfg = s.poisson_get_config()
+            assert cfg["lanes_per_block"] == lanes and cfg["chunk_rows"] == rows, cfg
+        if cplx:
+            s.poisson_tri_complex(cplx)
+            assert s.poisson_get_knob(capi.POISSON_TRI_COMPLEX) == cplx
+            assert s.poisson_get_knob(capi.POISSON_CHUNK_ROWS) == rows
+            assert s.poisson_get_knob(capi.POISSON_TRI_LANES) == 0
         s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         out = s.download(capi.F_HALO, capi.L_INTERIOR)
@@ -70,6 +79,57 @@ def test_poisson_matches_oracle(grid, tri):
     assert abs(out[:, 0, :].mean()) <= 1e-13 * np.abs(out).max()
     # compatible RHS: residual at round-off relative to |L| |phi| (res is divided by |L|)
     assert res <= 1e-13 * np.abs(out).max()
+
+
+@pytest.mark.parametrize("ny", [64, 50])
+@pytest.mark.parametrize("rows,lanes", [(8, 32), (12, 32), (16, 32), (16, 16)])
+def test_partitioned_lane_widths_match_oracle(ny, rows, lanes):
+    """Workgroups of 32 and 16 real lines (a wavefront holds several chunks);
+    nx = 40 leaves the last workgroup part-filled."""
+    f, ref, out, _, res = solve_both((40, ny, 24, 1.9), capi.TRI_PARTITIONED, seed=4, rows=rows, lanes=lanes)
+    assert rel_err(out, ref) <= TOL
+
+
+@pytest.mark.parametrize("ny", [64, 50])
+@pytest.mark.parametrize("rows,cplx", [(4, 32), (8, 16), (8, 32), (12, 16), (16, 16), (16, 32)])
+def test_partitioned_complex_lines_match_oracle(ny, rows, cplx):
+    """One lane per complex line (shared coefficients); nx = 40 gives 21
+    complex lines, so the last workgroup is part-filled."""
+    f, ref, out, _, res = solve_both((40, ny, 24, 1.9), capi.TRI_PARTITIONED, seed=6, rows=rows, cplx=cplx)
+    assert rel_err(out, ref) <= TOL
+
+
+def test_complex_lines_at_ny_256():
+    f, ref, out, _, res = solve_both((64, 256, 32, 2.0), capi.TRI_PARTITIONED, seed=7, rows=16, cplx=16)
+    assert rel_err(out, ref) <= TOL
+
+
+@pytest.mark.parametrize("grid", [(8, 4, 8, 1.5), (96, 20, 12, 1.8), (126, 34, 24, 2.0), (200, 10, 70, 1.6),
+                                  (64, 48, 32, 2.0)], ids=lambda g: "x".join(map(str, g[:3])))
+def test_residual_matches_the_operator(grid):
+    """ptc_poisson_residual = max |f - L phi| with L the operator clcPPE
+    inverts.  With f = L phi (numpy) every point's residual is round-off, so a
+    wrong neighbour anywhere (x and z wrap, the y walls, partial 64-column
+    tiles, nx below one wavefront) shows; with an unrelated f the maximum
+    itself is compared."""
+    nx, ny, nz, gam = grid
+    case = CS.case_poisson(nx, ny, nz, gam)
+    orc = Oracle(case)
+    rg = np.random.default_rng(nx + ny + nz)
+    phi = np.asfortranarray(rg.standard_normal((nx, ny, nz)))
+    lphi = np.asfortranarray(apply_laplacian(orc, case, phi))
+    f = np.asfortranarray(rg.standard_normal((nx, ny, nz)))
+    with Solver(case) as s:
+        s.upload(capi.F_HALO, phi, capi.L_INTERIOR)
+        s.upload(capi.F_ARR1, lphi, capi.L_INTERIOR)
+        r0 = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        r1 = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+    scale = float(np.abs(lphi).max())
+    ref1 = float(np.abs(f - lphi).max())
+    print(grid, f"r0={r0 / scale:.2e} r1={r1:.6e} ref={ref1:.6e}", flush=True)
+    assert r0 <= 1e-13 * scale
+    assert abs(r1 - ref1) <= 1e-13 * scale
 
 
 @pytest.mark.parametrize("ny", [64, 60, 50])
@@ -135,13 +195,26 @@ def test_wide_z_tiles_match_oracle(grid, ztile):
 
 
 def test_automatic_variant_falls_back_to_thomas():
-    """ny beyond every partitioned shape: AUTO runs Thomas, PARTITIONED refuses."""
-    case = CS.case_poisson(16, 272, 8, 2.0)
+    """ny beyond every partitioned shape (at most 32 chunks of 12 rows in a
+    1024-thread workgroup): AUTO runs Thomas, PARTITIONED refuses."""
+    case = CS.case_poisson(16, 400, 8, 2.0)
     with Solver(case) as s:
         s.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
         assert s.poisson_get_config()["tri"] == "thomas"
         with pytest.raises(capi.PtcError, match="partitioned"):
             s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
+    f, ref, out, _, _ = solve_both((16, 400, 8, 2.0), capi.TRI_AUTO, seed=3)
+    assert rel_err(out, ref) <= TOL
+
+
+def test_tall_lines_take_narrow_workgroups():
+    """ny = 272 is past the 64-line shapes; 32-line workgroups of 23 chunks of
+    12 rows take it."""
+    case = CS.case_poisson(16, 272, 8, 2.0)
+    with Solver(case) as s:
+        s.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
+        cfg = s.poisson_get_config()
+        assert cfg["tri"] == "partitioned" and cfg["lanes_per_block"] == 32 and cfg["chunk_rows"] == 12, cfg
     f, ref, out, _, _ = solve_both((16, 272, 8, 2.0), capi.TRI_AUTO, seed=3)
     assert rel_err(out, ref) <= TOL
 
@@ -235,7 +308,7 @@ def test_specialised_fft_kernels_match_generic_and_oracle(grid):
 # ---- FP32 storage + FP64 defect correction ---------------------------------
 @pytest.mark.parametrize("grid,slab,lines", [((64, 48, 32, 2.0), 0, 0), ((128, 96, 128, 1.8), 0, 0),
                                              ((128, 96, 128, 1.8), 16, 0), ((96, 140, 64, 2.2), 0, 0),
-                                             ((128, 96, 128, 1.8), 0, 16)])
+                                             ((128, 96, 128, 1.8), 0, 16), ((128, 96, 128, 1.8), 0, -16)])
 def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab, lines):
     """Spectra stored in FP32 between passes (FP64 arithmetic); each sweep
     adds the solve of the FP64 residual f - L phi."""
@@ -253,8 +326,10 @@ def test_fp32_refinement_converges_to_the_fp64_solution(grid, slab, lines):
             s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, sweeps)
             if slab:
                 s.poisson_slab_planes(slab)
-            if lines:
+            if lines > 0:
                 s.poisson_tri_lines(lines)
+            elif lines < 0:  # the complex-line kernel with -lines lines per workgroup
+                s.poisson_tri_complex(-lines)
             cfg = s.poisson_get_config()
             assert cfg["prec"] == "fp32+refine" and cfg["sweeps"] == sweeps, cfg
             s.clcPPE(capi.F_ARR1, capi.F_HALO)

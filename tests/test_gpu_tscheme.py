@@ -64,7 +64,23 @@ def upload_state(s, ux, uy, uz, p, sc=None):
 @pytest.mark.parametrize("kind", ["channel", "gravity"])
 @pytest.mark.parametrize("ns", [1, 2, 3])
 def test_clc_rhs_xyz(kind, ns):
-    case = small_channel() if kind == "channel" else small_gravity()
+    rhs_against_oracle(kind, ns)
+
+
+# rows of 8 and 96 points (part-filled wavefronts), 256 (several wavefronts
+# per row) and 1050 (beyond the row-marching kernel: the per-point kernel)
+@pytest.mark.parametrize("kind", ["channel", "gravity"])
+@pytest.mark.parametrize("grid", [(8, 8, 8), (96, 20, 24), (256, 10, 12), (1050, 8, 8)],
+                         ids=lambda g: "x".join(map(str, g)))
+def test_clc_rhs_xyz_row_lengths(kind, grid):
+    rhs_against_oracle(kind, 2, grid)
+
+
+def rhs_against_oracle(kind, ns, grid=None):
+    if grid is None:
+        case = small_channel() if kind == "channel" else small_gravity()
+    else:
+        case = small_channel(*grid) if kind == "channel" else small_gravity(*grid)
     orc = Oracle(case)
     ux, uy, uz, p, sc = rand_state(case, orc, seed=11 + ns)
     rng = np.random.default_rng(100 + ns)

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import scipy.linalg
 
-from helpers import inner, rel_err
+from helpers import apply_laplacian, inner, rel_err
 from oracle import oracle as O
 from paratc_amd import case as CS, ic
 
@@ -214,19 +214,6 @@ def test_modified_wavenumber_closed_form():
 
 
 # ----------------------------------------------------------- Poisson solve
-def apply_laplacian(o, case, p):
-    ny = case.ny
-    am, ac, ap = (o.metric(n)[1:ny + 1] for n in ("am2ph", "ac2ph", "ap2ph"))
-    rdx2, rdz2 = (case.nx / case.xlx) ** 2, (case.nz / case.zlz) ** 2
-    st = np.array([1, -54, 783, -1460, 783, -54, 1]) / 576.0
-    lp = np.zeros_like(p)
-    for s in range(-3, 4):
-        lp += st[s + 3] * (rdx2 * np.roll(p, -s, axis=0) + rdz2 * np.roll(p, -s, axis=2))
-    pm = np.concatenate([p[:, :1], p, p[:, -1:]], axis=1)
-    lp += am[None, :, None] * pm[:, :-2] + ac[None, :, None] * pm[:, 1:-1] + ap[None, :, None] * pm[:, 2:]
-    return lp
-
-
 @pytest.mark.parametrize("choice", [1, 2, 3, 4])
 @pytest.mark.parametrize("grid", [(16, 12, 10), (24, 8, 14), (20, 16, 30)])
 def test_poisson_inverts_manufactured_field(choice, grid):
