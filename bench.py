@@ -41,14 +41,30 @@ HBM_PEAK_GBS = 8000.0
 STAGES = ["x_fwd", "z_fwd", "y_solve", "z_inv", "x_inv"]
 
 
-def stage_kernels(plan: dict) -> list:
-    """Kernel behind each pass for the plan in use (as rocprofv3 names them).
-    With slabbed x/z passes the x entries carry the x and z kernels together."""
-    xf = "k2_fft_x_fwd" if plan["x_fast"] else "k_fft_x_fwd"
-    xi = "k2_fft_x_inv" if plan["x_fast"] else "k_fft_x_inv"
-    z = "k2_fft_z" if plan["z_fast"] else "k_fft_z"
-    y = "k_tri_poisson_split" if plan["tri"] == "partitioned" else "k_tri_poisson_thomas"
-    return [xf, z + "<fwd>", y, z + "<inv>", xi]
+def stage_kernels(plan: dict, nx: int, nz: int) -> list:
+    """Kernel behind each pass for the plan in use, as rocprofv3 names it (the
+    name up to the storage type: FP64 spectra).  With slabbed x/z passes the x
+    entries carry the x and z kernels together."""
+    d = "HIP_vector_type<double"
+    h = nx // 2
+    if plan["x_fast"]:
+        xf, xi = f"k2_fft_x_fwd<{h}, {d}", f"k2_fft_x_inv<{h}, {d}, 2u>, false>"
+    else:
+        xf, xi = "k_fft_x_fwd<", "k_fft_x_inv<"
+    if plan["z_fast"]:
+        zf, zi = (f"k2_fft_z<{nz}, {plan['z_tile']}, {inv}, {d}" for inv in ("false", "true"))
+    else:
+        zf, zi = "k_fft_z<" + str(plan["z_tile"]) + ", false>", "k_fft_z<" + str(plan["z_tile"]) + ", true>"
+    mm = plan["chunk_rows"]
+    if plan["tri"] != "partitioned":
+        y = "k_tri_poisson_thomas"
+    elif plan.get("tri_complex_lines"):
+        y = f"k_tri_poisson_cplx<{mm}, {plan['tri_complex_lines']}, {d}"
+    elif plan.get("tri_lds_lines"):
+        y = f"k_tri_poisson_lds<{mm}, {plan['tri_lds_lines']}, {d}"
+    else:
+        y = f"k_tri_poisson_split<{mm}, {plan['lanes_per_block']}, {d}"
+    return [xf, zf, y, zi, xi]
 
 
 class HipEvents:
@@ -314,7 +330,7 @@ def main():
     rk = None if args.no_rk else bench_rk_substep(ev)
     ens = None if args.no_ensemble else bench_ensemble()
     n = case.npts
-    kernel = stage_kernels(r["plan"])[r["dom"]]
+    kernel = stage_kernels(r["plan"], case.nx, case.nz)[r["dom"]]
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
     # WRITE_SIZE summary of this workload (null when it does not cover it)
     from paratc_amd import profsum

@@ -183,10 +183,13 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *   PTC_POISSON_TRI_LANES   real lines per workgroup of the register y-solve
  *                           (0 = automatic, 64, 32 or 16): narrower
  *                           workgroups are smaller, so several share a CU.
- *   PTC_POISSON_TRI_COMPLEX 0: the partitioned y-solve treats the real and
- *                           imaginary parts as separate lanes; 16 or 32: one
- *                           lane per complex line (shared row coefficients and
- *                           divisions), that many lines per workgroup. */
+ *   PTC_POISSON_TRI_COMPLEX 16 or 32: the partitioned y-solve runs one lane
+ *                           per complex line (shared row coefficients and
+ *                           divisions), that many lines per workgroup; -1:
+ *                           the real and imaginary parts as separate lanes;
+ *                           0 (automatic): complex lines, 16 per workgroup,
+ *                           where a shape fits and no other y-solve knob is
+ *                           set. */
 enum {
   PTC_POISSON_CHUNK_ROWS = 1,
   PTC_POISSON_SLAB_PLANES = 2,

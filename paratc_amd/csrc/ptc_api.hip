@@ -449,8 +449,8 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
     PTC_CHECK(value == 0 || value == 16 || value == 32 || value == 64, PTC_E_ARG,
               "y-solve lanes per workgroup must be 0, 16, 32 or 64");
   } else if (knob == PTC_POISSON_TRI_COMPLEX) {
-    PTC_CHECK(value == 0 || value == 16 || value == 32, PTC_E_ARG,
-              "complex lines per workgroup must be 0, 16 or 32");
+    PTC_CHECK(value == -1 || value == 0 || value == 16 || value == 32, PTC_E_ARG,
+              "complex lines per workgroup must be -1, 0, 16 or 32");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }

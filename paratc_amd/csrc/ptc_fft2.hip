@@ -219,14 +219,16 @@ __device__ __forceinline__ void stage_twiddles(int g, const double2 *__restrict_
   }
 }
 
-template <int N, int S, int G, bool INV, bool SYNC, class Load, class Store>
-__device__ __forceinline__ void stage(int g, const double2 (&wpre)[4], const Load &load, const Store &store) {
-  constexpr PlanC P = make_plan(N);
-  constexpr int R = P.r[S];
-  constexpr int NS = plan_span(N, S);
+// stage S's inputs of one thread: v[i][r] = element b + r * NB of butterfly
+// b = g + i * G
+template <int N, int S, int G>
+using StageRegs = double2[stage_bpt<N, S, G>()][make_plan(N).r[S]];
+
+template <int N, int S, int G, class Load>
+__device__ __forceinline__ void stage_load(int g, const Load &load, StageRegs<N, S, G> &v) {
+  constexpr int R = make_plan(N).r[S];
   constexpr int NB = N / R;
   constexpr int BPT = stage_bpt<N, S, G>();
-  double2 v[BPT][R];
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int b = g + i * G;
@@ -235,7 +237,15 @@ __device__ __forceinline__ void stage(int g, const double2 (&wpre)[4], const Loa
       for (int r = 0; r < R; ++r) v[i][r] = load(b + r * NB);
     }
   }
-  if (SYNC) __syncthreads();
+}
+
+template <int N, int S, int G, bool INV, class Store>
+__device__ __forceinline__ void stage_exec(int g, const double2 (&wpre)[4], StageRegs<N, S, G> &v,
+                                           const Store &store) {
+  constexpr int R = make_plan(N).r[S];
+  constexpr int NS = plan_span(N, S);
+  constexpr int NB = N / R;
+  constexpr int BPT = stage_bpt<N, S, G>();
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
     const int b = g + i * G;
@@ -258,18 +268,23 @@ __device__ __forceinline__ void stage(int g, const double2 (&wpre)[4], const Loa
   }
 }
 
+template <int N, int S, int G, bool INV, bool SYNC, class Load, class Store>
+__device__ __forceinline__ void stage(int g, const double2 (&wpre)[4], const Load &load, const Store &store) {
+  StageRegs<N, S, G> v;
+  stage_load<N, S, G>(g, load, v);
+  if (SYNC) __syncthreads();
+  stage_exec<N, S, G, INV>(g, wpre, v, store);
+}
+
 // Full transform of one line group.  Stage 0 loads through load0, the last
 // stage stores through storeL; intermediate data lives in LDS through lds(m).
 // LOAD0_LDS: load0 itself reads the LDS buffer, so stage 0 needs a barrier
 // between its loads and its stores.
-template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class LdsAt, class StoreL>
-__device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
-                                        const StoreL &storeL) {
+// the first twiddles of every stage a thread runs
+template <int N, int G, bool INV>
+__device__ __forceinline__ void load_twiddles(int g, const double2 *__restrict__ tw, double2 (&W)[7][4]) {
   constexpr PlanC P = make_plan(N);
   static_assert(P.ns <= 7, "too many stages");
-  auto ldsload = [&](int m) { return lds(m); };
-  auto ldsstore = [&](int m, double2 v) { lds(m) = v; };
-  double2 W[7][4];
   stage_twiddles<N, 0, G, INV>(g, tw, W[0]);
   if constexpr (P.ns > 1) stage_twiddles<N, (P.ns > 1 ? 1 : 0), G, INV>(g, tw, W[1]);
   if constexpr (P.ns > 2) stage_twiddles<N, (P.ns > 2 ? 2 : 0), G, INV>(g, tw, W[2]);
@@ -277,32 +292,52 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
   if constexpr (P.ns > 4) stage_twiddles<N, (P.ns > 4 ? 4 : 0), G, INV>(g, tw, W[4]);
   if constexpr (P.ns > 5) stage_twiddles<N, (P.ns > 5 ? 5 : 0), G, INV>(g, tw, W[5]);
   if constexpr (P.ns > 6) stage_twiddles<N, (P.ns > 6 ? 6 : 0), G, INV>(g, tw, W[6]);
+}
+
+// Stages 1 .. ns-1 of a transform whose stage 0 has stored into LDS (and a
+// barrier has passed); the last stage stores through storeL.
+template <int N, int G, bool INV, class LdsAt, class StoreL>
+__device__ __forceinline__ void run_rest(int g, const double2 (&W)[7][4], const LdsAt &lds, const StoreL &storeL) {
+  constexpr PlanC P = make_plan(N);
+  static_assert(P.ns >= 2, "a single-stage transform has no later stages");
+  auto ldsload = [&](int m) { return lds(m); };
+  auto ldsstore = [&](int m, double2 v) { lds(m) = v; };
+  if constexpr (P.ns > 2) {
+    stage<N, 1, G, INV, true>(g, W[1], ldsload, ldsstore);
+    __syncthreads();
+  }
+  if constexpr (P.ns > 3) {
+    stage<N, 2, G, INV, true>(g, W[2], ldsload, ldsstore);
+    __syncthreads();
+  }
+  if constexpr (P.ns > 4) {
+    stage<N, 3, G, INV, true>(g, W[3], ldsload, ldsstore);
+    __syncthreads();
+  }
+  if constexpr (P.ns > 5) {
+    stage<N, 4, G, INV, true>(g, W[4], ldsload, ldsstore);
+    __syncthreads();
+  }
+  if constexpr (P.ns > 6) {
+    stage<N, 5, G, INV, true>(g, W[5], ldsload, ldsstore);
+    __syncthreads();
+  }
+  stage<N, P.ns - 1, G, INV, true>(g, W[P.ns - 1], ldsload, storeL);
+}
+
+template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class LdsAt, class StoreL>
+__device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
+                                        const StoreL &storeL) {
+  constexpr PlanC P = make_plan(N);
+  double2 W[7][4];
+  load_twiddles<N, G, INV>(g, tw, W);
   if constexpr (P.ns == 1) {
     stage<N, 0, G, INV, false>(g, W[0], load0, storeL);
   } else {
+    auto ldsstore = [&](int m, double2 v) { lds(m) = v; };
     stage<N, 0, G, INV, LOAD0_LDS>(g, W[0], load0, ldsstore);
     __syncthreads();
-    if constexpr (P.ns > 2) {
-      stage<N, 1, G, INV, true>(g, W[1], ldsload, ldsstore);
-      __syncthreads();
-    }
-    if constexpr (P.ns > 3) {
-      stage<N, 2, G, INV, true>(g, W[2], ldsload, ldsstore);
-      __syncthreads();
-    }
-    if constexpr (P.ns > 4) {
-      stage<N, 3, G, INV, true>(g, W[3], ldsload, ldsstore);
-      __syncthreads();
-    }
-    if constexpr (P.ns > 5) {
-      stage<N, 4, G, INV, true>(g, W[4], ldsload, ldsstore);
-      __syncthreads();
-    }
-    if constexpr (P.ns > 6) {
-      stage<N, 5, G, INV, true>(g, W[5], ldsload, ldsstore);
-      __syncthreads();
-    }
-    stage<N, P.ns - 1, G, INV, true>(g, W[P.ns - 1], ldsload, storeL);
+    run_rest<N, G, INV>(g, W, lds, storeL);
   }
 }
 

@@ -103,7 +103,7 @@ struct PoissonPlan {
   int tri_lpb_req = 0; // register y-solve: real lines per workgroup requested (0 = automatic)
   int tri_ll_req = 0; // LDS-staged y-solve: real lines per workgroup requested (0 = register kernel)
   int tri_ll = 0;     // in use
-  int tri_cl_req = 0; // complex-line y-solve: complex lines per workgroup requested (0 = real-line kernels)
+  int tri_cl_req = 0; // complex-line y-solve lines per workgroup (0 = automatic, -1 = real-line kernels)
   int tri_cl = 0;     // in use
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables

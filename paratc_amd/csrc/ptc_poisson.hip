@@ -124,11 +124,18 @@ int poisson_build_plan(ptc_ctx *c) {
   }
 
   // y pass: partitioned unless Thomas is asked for, or (automatic choice) no
-  // partitioned shape fits this ny
+  // partitioned shape fits this ny.  Without a kernel asked for, the
+  // complex-line kernel with 16 lines per workgroup where it fits, else the
+  // real-line register kernel.
   P.tri = P.tri_req == PTC_TRI_THOMAS ? PTC_TRI_THOMAS : PTC_TRI_PARTITIONED;
   P.tri_ll = 0;
   P.tri_cl = 0;
-  if (P.tri == PTC_TRI_PARTITIONED && P.tri_cl_req > 0) {
+  const bool cl_auto = P.tri_cl_req == 0 && P.tri_ll_req == 0 && P.tri_lpb_req == 0;
+  if (P.tri == PTC_TRI_PARTITIONED && cl_auto &&
+      tri_plan_cplx(c->ny, c->lds_limit, 16, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax) == PTC_OK) {
+    P.tri_cl = 16;
+    P.tri_lpb = 16;
+  } else if (P.tri == PTC_TRI_PARTITIONED && P.tri_cl_req > 0) {
     PTC_TRY(tri_plan_cplx(c->ny, c->lds_limit, P.tri_cl_req, P.tri_mm_req, &P.tri_chunks, &P.tri_mmax));
     P.tri_cl = P.tri_cl_req;
     P.tri_lpb = P.tri_cl;

@@ -150,7 +150,7 @@ class Solver:
         return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
                 "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2),
                 "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7], "slab_planes": o[8],
-                "tri_lds_lines": o[9]}
+                "tri_lds_lines": o[9], "tri_complex_lines": self.poisson_get_knob(capi.POISSON_TRI_COMPLEX)}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()

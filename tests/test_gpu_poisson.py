@@ -54,11 +54,14 @@ def solve_both(grid, tri, seed=1, compat=True, rows=0, lines=0, lanes=0, cplx=0)
             s.poisson_tri_lanes(lanes)
             cfg = s.poisson_get_config()
             assert cfg["lanes_per_block"] == lanes and cfg["chunk_rows"] == rows, cfg
-        if cplx:
+        if cplx > 0:
             s.poisson_tri_complex(cplx)
             assert s.poisson_get_knob(capi.POISSON_TRI_COMPLEX) == cplx
             assert s.poisson_get_knob(capi.POISSON_CHUNK_ROWS) == rows
             assert s.poisson_get_knob(capi.POISSON_TRI_LANES) == 0
+        elif cplx < 0:
+            s.poisson_tri_complex(cplx)
+            assert s.poisson_get_knob(capi.POISSON_TRI_COMPLEX) == 0
         s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         out = s.download(capi.F_HALO, capi.L_INTERIOR)
@@ -134,9 +137,11 @@ def test_residual_matches_the_operator(grid):
 
 @pytest.mark.parametrize("ny", [64, 60, 50])
 @pytest.mark.parametrize("rows", [4, 8, 12, 16])
-def test_partitioned_chunk_lengths_match_oracle(ny, rows):
-    """Every compiled chunk length, with and without identity padding rows."""
-    f, ref, out, _, res = solve_both((32, ny, 24, 1.9), capi.TRI_PARTITIONED, seed=2, rows=rows)
+@pytest.mark.parametrize("lanes", [0, 64], ids=["auto", "real64"])
+def test_partitioned_chunk_lengths_match_oracle(ny, rows, lanes):
+    """Every compiled chunk length, with and without identity padding rows, in
+    the automatic kernel (complex lines) and the 64-lane real-line kernel."""
+    f, ref, out, _, res = solve_both((32, ny, 24, 1.9), capi.TRI_PARTITIONED, seed=2, rows=rows, lanes=lanes)
     assert rel_err(out, ref) <= TOL
     assert res <= 1e-13 * np.abs(out).max()
 
@@ -195,27 +200,35 @@ def test_wide_z_tiles_match_oracle(grid, ztile):
 
 
 def test_automatic_variant_falls_back_to_thomas():
-    """ny beyond every partitioned shape (at most 32 chunks of 12 rows in a
-    1024-thread workgroup): AUTO runs Thomas, PARTITIONED refuses."""
-    case = CS.case_poisson(16, 400, 8, 2.0)
+    """ny beyond every partitioned shape (the complex-line kernel reaches 512
+    rows: 64 chunks of 8 in 16-line workgroups): AUTO runs Thomas,
+    PARTITIONED refuses."""
+    case = CS.case_poisson(16, 600, 8, 2.0)
     with Solver(case) as s:
         s.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
         assert s.poisson_get_config()["tri"] == "thomas"
         with pytest.raises(capi.PtcError, match="partitioned"):
             s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED, 0, 0)
-    f, ref, out, _, _ = solve_both((16, 400, 8, 2.0), capi.TRI_AUTO, seed=3)
+    f, ref, out, _, _ = solve_both((16, 600, 8, 2.0), capi.TRI_AUTO, seed=3)
     assert rel_err(out, ref) <= TOL
 
 
 def test_tall_lines_take_narrow_workgroups():
-    """ny = 272 is past the 64-line shapes; 32-line workgroups of 23 chunks of
-    12 rows take it."""
+    """ny = 272: the automatic plan runs complex lines (34 chunks of 8 rows);
+    with complex lines off it is past the 64-lane real shapes and 32-lane
+    workgroups of 23 chunks of 12 rows take it."""
     case = CS.case_poisson(16, 272, 8, 2.0)
     with Solver(case) as s:
         s.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
+        assert s.poisson_get_knob(capi.POISSON_TRI_COMPLEX) == 16
+        assert s.poisson_get_knob(capi.POISSON_CHUNK_ROWS) == 8
+        s.poisson_tri_complex(-1)
         cfg = s.poisson_get_config()
         assert cfg["tri"] == "partitioned" and cfg["lanes_per_block"] == 32 and cfg["chunk_rows"] == 12, cfg
+        assert s.poisson_get_knob(capi.POISSON_TRI_COMPLEX) == 0
     f, ref, out, _, _ = solve_both((16, 272, 8, 2.0), capi.TRI_AUTO, seed=3)
+    assert rel_err(out, ref) <= TOL
+    f, ref, out, _, _ = solve_both((16, 272, 8, 2.0), capi.TRI_AUTO, seed=3, cplx=-1)
     assert rel_err(out, ref) <= TOL
 
 
