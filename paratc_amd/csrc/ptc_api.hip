@@ -177,17 +177,19 @@ int channel_step(ptc_ctx *c, double *mon) {
     PTC_TRY(pm_coe_update(c, ns, dt));
     PTC_TRY(launch_uy_ym(c, uym, duym));
     PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p));
-    if (scal) PTC_TRY(launch_tri_cn(c, 2, sc, ac, nullptr, c->pmBetaSc, nullptr));
-    PTC_TRY(launch_tri_cn(c, 0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr));
-    PTC_TRY(launch_tri_cn(c, 1, uy, a2, duym, c->pmBeta, nullptr));
-    PTC_TRY(launch_tri_cn(c, 0, uz, hl, nullptr, c->pmBeta, nullptr));
+    {  // clcScalar, clcU1Hat, clcU2Hat, clcU3Hat (:111,116-118): independent solves, one launch
+      CnJobs cn{};
+      if (scal) cn.j[cn.n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
+      cn.j[cn.n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
+      cn.j[cn.n++] = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
+      cn.j[cn.n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
+      PTC_TRY(launch_tri_cn_jobs(c, cn));
+    }
     PTC_TRY(launch_bc_vel(c, ux, uy, uz, uym, true));
     PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr));
-    PTC_TRY(poisson_solve(c, a1, hl));
-    PTC_TRY(launch_bc_pr(c, hl));
+    PTC_TRY(poisson_solve(c, a1, hl, true));  // with SetBC_and_UpdateHalo_pr(prphiHalo) (:126)
     PTC_TRY(launch_vel_p_update(c, hl, ux, uy, uz, p));
-    PTC_TRY(launch_bc_vel(c, ux, uy, uz, uym, false));
-    PTC_TRY(launch_bc_pr(c, p));
+    PTC_TRY(launch_bc_vel_pr(c, ux, uy, uz, uym, p));
     if (scal) PTC_TRY(launch_bc_scalar(c, sc));
   }
   if (mon) {

@@ -401,9 +401,15 @@ static int launch_x_inv_t(ptc_ctx *c, const double2 *spec, double *dst, double s
   return PTC_OK;
 }
 
-int launch_fft_x_inv(ptc_ctx *c, const void *spec_, double *dst, double scale, bool acc) {
-  if (c->pp.x_fast) return fft2_x_inv(c, spec_, dst, scale, 0, c->ny, acc);
+// ghost: the specialised kernel also fills the Neumann ghost rows; the generic
+// one fills them with a separate boundary launch
+int launch_fft_x_inv(ptc_ctx *c, const void *spec_, double *dst, double scale, bool acc, bool ghost) {
+  if (c->pp.x_fast) return fft2_x_inv(c, spec_, dst, scale, 0, c->ny, acc, ghost);
   PTC_CHECK(!acc, PTC_E_UNSUPPORTED, "accumulating x-inverse needs the specialised kernels");
+  if (ghost) {
+    PTC_TRY(launch_fft_x_inv(c, spec_, dst, scale, false, false));
+    return launch_bc_pr(c, dst);
+  }
   const double2 *spec = static_cast<const double2 *>(spec_);
   return stage_fits(c->pp.fx, c->pp.x_lines, c->pp.x_threads, 16) ? launch_x_inv_t<16>(c, spec, dst, scale)
                                                                   : launch_x_inv_t<32>(c, spec, dst, scale);

@@ -397,12 +397,14 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
 
 // ---- x inverse: half spectra -> real rows times scale ----------------------
 // ACC: add the result to dst (a correction) instead of storing it, minus the
-// device scalar *shift (the mean the correction removes)
+// device scalar *shift (the mean the correction removes).  ghost: the rows
+// j = 1 and j = ny also go to the Neumann ghost rows j = 0 and j = ny+1
+// (SetBC_and_UpdateHalo_pr, src/m_BC_and_Halo.f90:115-120).
 template <int N, typename S, bool ACC>
 __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
-                                                    double scale, const double *__restrict__ shift) {
+                                                    double scale, const double *__restrict__ shift, int ghost) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = f2::padded(N);
@@ -439,15 +441,22 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
   };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
+  // the ghost row this line also fills (or none)
+  double2 *grow = nullptr;
+  if (ghost && jl == 0) grow = reinterpret_cast<double2 *>(dst + rowlen * ((long long)(ny + 2) * k));
+  if (ghost && jl == ny - 1) grow = reinterpret_cast<double2 *>(dst + rowlen * (ny + 1 + (long long)(ny + 2) * k));
   const double sh0 = ACC ? shift[0] : 0.0;
   auto store_out = [&](int m, double2 v) {
     if (!live) return;
+    double2 x;
     if (ACC) {
       const double2 o = row[m];
-      row[m] = make_double2(o.x + v.x * scale - sh0, o.y + v.y * scale - sh0);
+      x = make_double2(o.x + v.x * scale - sh0, o.y + v.y * scale - sh0);
     } else {
-      row[m] = make_double2(v.x * scale, v.y * scale);
+      x = make_double2(v.x * scale, v.y * scale);
     }
+    row[m] = x;
+    if (grow) grow[m] = x;
   };
   f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
 }
@@ -485,7 +494,7 @@ static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
 }
 
 template <int N>
-static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc) {
+static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const long long nlines = (long long)nj * c->nz;
@@ -499,17 +508,17 @@ static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int 
   if (c->pp.prec == PTC_PREC_FP32_REFINE) {
     if (acc)
       hipLaunchKernelGGL((k2_fft_x_inv<N, float2, true>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw, wx,
-                         ny, nz, j0, nj, pitch, rl, scale, shift);
+                         ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
     else
       hipLaunchKernelGGL((k2_fft_x_inv<N, float2, false>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
   } else {
     if (acc)
       hipLaunchKernelGGL((k2_fft_x_inv<N, double2, true>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
     else
       hipLaunchKernelGGL((k2_fft_x_inv<N, double2, false>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift);
+                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
   }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
@@ -561,9 +570,9 @@ int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
   }
 }
 
-int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc) {
+int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale, j0, nj, acc);
+#define PTC_CASE(N) case N: return x_inv_t<N>(c, spec, dst, scale, j0, nj, acc, ghost);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;

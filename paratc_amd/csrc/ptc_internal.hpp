@@ -165,7 +165,8 @@ int check_field(ptc_ctx *c, int id);
 // Poisson
 int poisson_build_plan(ptc_ctx *c);
 int poisson_free_plan(ptc_ctx *c);
-int poisson_solve(ptc_ctx *c, const double *src, double *dst);
+// ghosts: also fill dst's Neumann ghost rows j = 0, ny+1 (SetBC_and_UpdateHalo_pr)
+int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts = false);
 int poisson_prepare(ptc_ctx *c);
 int stencil_prepare(ptc_ctx *c);
 // one ChannelIterate step (monitors into mon[7] when mon != NULL)
@@ -190,7 +191,7 @@ int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 
 // FFT launchers (ptc_fft.hip)
 int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec);
-int launch_fft_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, bool acc);
+int launch_fft_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, bool acc, bool ghost);
 int launch_fft_z(ptc_ctx *c, void *spec, bool inverse);
 int fft_make_len(int n, FftLen *L);
 bool fft2_has_x(int h);
@@ -199,11 +200,27 @@ bool fft2_has_z(int nz, int req, int *tile);
 // the spectrum holds double2 or, in the FP32-storage form, float2 values;
 // acc: the x-inverse adds its result to dst
 int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj);
-int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc);
+int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost);
 int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj);
 
 // tridiagonal launchers (ptc_tridiag.hip)
 int launch_tri_poisson(ptc_ctx *c, void *spec);
+// Crank-Nicolson y-solves u += (I - beta L_y)^-1 rhs: one job per field;
+// kind 0 (u/w), 1 (v, wall row from duy_ym), 2 (scalar); shift: optional
+// device scalar added to rhs
+struct CnJob {
+  double *u;
+  const double *rhs, *duy_ym, *shift;
+  double beta;
+  int kind, rA, rB, rC;
+};
+constexpr int kCnMaxJobs = 4;
+struct CnJobs {
+  CnJob j[kCnMaxJobs];
+  int n;
+};
+CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, double beta, const double *shift);
+int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs);
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym,
                   double beta, const double *shift);
 
@@ -218,10 +235,40 @@ int launch_pressure_update(ptc_ctx *c, double *p, const double *phi);
 int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p);
 int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, bool prsrc_only);
 int launch_bc_pr(ptc_ctx *c, double *p);
+int launch_bc_vel_pr(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, double *p);
 int launch_bc_scalar(ptc_ctx *c, double *s);
 int launch_uy_ym(ptc_ctx *c, double *uy_ym, double *duy_ym);
 int launch_monitor(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz,
                    const double *s, double *out);
 int reduce_sum_s3(ptc_ctx *c, int nparts, double *host_out);
+
+// The partial results of a launch reduced by one workgroup of
+// kReduceThreads threads: every thread has eight loads in flight (indices past
+// n read the last partial: a duplicate, masked out of sums, harmless to
+// maxima), then a fixed-order tree in LDS.  n >= 1.
+constexpr int kReduceThreads = 1024;
+template <bool MAX>
+__device__ __forceinline__ double reduce_partials(const double *__restrict__ part, int n) {
+  __shared__ double sh[kReduceThreads];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int q0 = 0; q0 < n; q0 += 8 * kReduceThreads) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[min(q0 + u * kReduceThreads + t, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (MAX) acc = fmax(acc, v[u]);
+      else acc += v[u] * ((q0 + u * kReduceThreads + t < n) ? 1.0 : 0.0);
+    }
+  }
+  sh[t] = acc;
+  __syncthreads();
+  for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+    if (t < w) sh[t] = MAX ? fmax(sh[t], sh[t + w]) : sh[t] + sh[t + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
 
 }  // namespace ptc

@@ -213,7 +213,7 @@ int poisson_prepare(ptc_ctx *c) {
 
 // One pass sequence src -> spectrum -> dst (acc: dst += result); ev (may be
 // NULL) receives six events around the five passes.
-static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, hipEvent_t *ev) {
+static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, bool ghosts, hipEvent_t *ev) {
   PoissonPlan &P = c->pp;
   if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
   if (P.slab == 0) {
@@ -235,13 +235,13 @@ static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, 
   if (P.slab == 0) {
     PTC_TRY(launch_fft_z(c, P.d_spec, true));
     if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
-    PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft, acc));
+    PTC_TRY(launch_fft_x_inv(c, P.d_spec, dst, c->normfft, acc, ghosts));
   } else {
     if (ev) PTC_HIP(hipEventRecord(ev[4], c->stream));
     for (int j0 = 0; j0 < c->ny; j0 += P.slab) {
       const int nj = std::min(P.slab, c->ny - j0);
       PTC_TRY(fft2_z(c, P.d_spec, true, j0, nj));
-      PTC_TRY(fft2_x_inv(c, P.d_spec, dst, c->normfft, j0, nj, acc));
+      PTC_TRY(fft2_x_inv(c, P.d_spec, dst, c->normfft, j0, nj, acc, ghosts));
     }
   }
   if (ev) PTC_HIP(hipEventRecord(ev[5], c->stream));
@@ -250,7 +250,7 @@ static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, 
 
 int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r);
 
-int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
+int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts) {
   PTC_TRY(poisson_build_plan(c));
   PoissonPlan &P = c->pp;
   PTC_TRY(ensure_thomas_scratch(c));
@@ -265,7 +265,7 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
     ev = P.prof_ev.data() + (size_t)P.prof_used * 6;
     ++P.prof_used;
   }
-  PTC_TRY(solve_sequence(c, src, dst, false, ev));
+  PTC_TRY(solve_sequence(c, src, dst, false, ghosts, ev));
   if (P.prec == PTC_PREC_FP32_REFINE) {
     // defect correction: phi += L^-1 (f - L phi), the residual in FP64 and
     // the correction through the same FP32-storage passes; the correction's
@@ -273,7 +273,7 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst) {
     // keeps the reference's pin phi(j=1) = 0 (src/m_Poisson.f90:399-404)
     for (int s = 0; s < P.sweeps; ++s) {
       PTC_TRY(launch_residual(c, src, dst, P.d_res));
-      PTC_TRY(solve_sequence(c, P.d_res, dst, true, nullptr));
+      PTC_TRY(solve_sequence(c, P.d_res, dst, true, ghosts, nullptr));
     }
   }
   return PTC_OK;
@@ -443,18 +443,10 @@ int residual_blocks(const ptc_ctx *c) {
 }
 
 // out = (sum of the nblk partial sums, in a fixed order) * scale
-__global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ psum, int nblk, double scale,
-                                                      double *__restrict__ out) {
-  __shared__ double sh[256];
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += 256) s += psum[b];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sh[0] * scale;
+__global__ void __launch_bounds__(kReduceThreads) k_sum_partials(const double *__restrict__ psum, int nblk,
+                                                                  double scale, double *__restrict__ out) {
+  const double s = reduce_partials<false>(psum, nblk);
+  if (threadIdx.x == 0) out[0] = s * scale;
 }
 
 int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
@@ -467,7 +459,7 @@ int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
   PTC_HIP(hipGetLastError());
   // the x-z mean of phi's first row: the (0,0) mode's pinned value, which no
   // residual sees (a constant is in the Neumann operator's null space)
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->pp.d_psum, nblk,
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kReduceThreads), 0, c->stream, c->pp.d_psum, nblk,
                      1.0 / ((double)c->nx * (double)c->nz), c->d_scal + kSlotPinShift);
   PTC_HIP(hipGetLastError());
   return PTC_OK;

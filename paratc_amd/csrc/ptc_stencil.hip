@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) k_rhs_xyz(Geo g, RhsCoef cf, const double
 }
 
 // ---- clcRhsXYZ, row-marching form -----------------------------------------
-// The same right-hand sides as k_rhs_xyz, for rows of at most 1024 points: a
+// The same right-hand sides as k_rhs_xyz, for rows of at most 512 points: a
 // workgroup owns one whole x row (one thread per column) at fixed j and
 // marches along k over planes [k0, k1).  Each thread keeps its column's values
 // in register windows along z (ux, uy, uz, scalar: k-3..k+3; p and uy at j+1:
@@ -615,13 +615,11 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
 
 // constant flow rate (:269-300): sums the partials in a fixed order and stores
 // the shift added to RhsX and the PrGradAver increment
-__global__ void __launch_bounds__(256) k_flowrate(const double *__restrict__ part, int n, double *__restrict__ scal,
-                                                 int scalar_build, double xnu, double ubulk, double ucrf,
-                                                 double nxnz, double yly, double alpha, double alphaC, double dt) {
-  __shared__ double sh[256];
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
-  const double tot = block_sum(s, sh);
+__global__ void __launch_bounds__(kReduceThreads) k_flowrate(const double *__restrict__ part, int n,
+                                                            double *__restrict__ scal, int scalar_build, double xnu,
+                                                            double ubulk, double ucrf, double nxnz, double yly,
+                                                            double alpha, double alphaC, double dt) {
+  const double tot = reduce_partials<false>(part, n);
   if (threadIdx.x == 0) {
     if (scalar_build) {
       const double s3 = tot / nxnz / yly;
@@ -673,19 +671,15 @@ __global__ void __launch_bounds__(256) k_prsrc(Geo g, const double *__restrict__
   if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
 }
 
-__global__ void __launch_bounds__(256) k_final_max(const double *__restrict__ part, int n, double *__restrict__ out) {
-  __shared__ double sh[256];
-  double m = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
-  const double t = block_max(m, sh);
+__global__ void __launch_bounds__(kReduceThreads) k_final_max(const double *__restrict__ part, int n,
+                                                             double *__restrict__ out) {
+  const double t = reduce_partials<true>(part, n);
   if (threadIdx.x == 0) out[0] = t;
 }
 
-__global__ void __launch_bounds__(256) k_final_sum(const double *__restrict__ part, int n, double *__restrict__ out) {
-  __shared__ double sh[256];
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) s += part[q];
-  const double t = block_sum(s, sh);
+__global__ void __launch_bounds__(kReduceThreads) k_final_sum(const double *__restrict__ part, int n,
+                                                             double *__restrict__ out) {
+  const double t = reduce_partials<false>(part, n);
   if (threadIdx.x == 0) out[0] = t;
 }
 
@@ -737,6 +731,10 @@ __global__ void k_bc(Geo g, BcArgs b, double *__restrict__ a0, double *__restric
     const long long r0 = base, r1 = base + g.nx, rn = base + (long long)g.nx * g.ny,
                     rnp = base + (long long)g.nx * (g.ny + 1);
     switch (b.mode) {
+      case 5:  // SetBC_and_UpdateHalo_pr on pl1 (:115-120), then case 0 in the same pass
+        pl1[r0] = pl1[r1];
+        pl1[rnp] = pl1[rn];
+        [[fallthrough]];
       case 0:  // src/m_BC_and_Halo.f90:32-58
         if (b.flow_type == 1) {
           a0[rnp] = b.uxb_p * 2.0 - a0[rn];
@@ -826,15 +824,15 @@ static int pt_nblocks(const ptc_ctx *c) {
   return (int)(gr.x * gr.y * gr.z);
 }
 
-// row-marching RHS (rows of at most 1024 points): one workgroup per (j, k
-// chunk); the chunks are at least 8 planes long and many enough for about
-// 24 wavefronts per CU over the launch
-constexpr int kRhsRowsMaxNx = 1024;
+// row-marching RHS (rows of at most 512 points: a 1024-thread workgroup would
+// spill): one workgroup per (j, k chunk); the chunks are at least 16 planes
+// long and many enough for about 24 wavefronts per CU over the launch
+constexpr int kRhsRowsMaxNx = 512;
 static bool rhs_rows(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx; }
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
   const int wpb = (c->nx + 63) / 64;
   const int want = (24 * c->n_cu + c->ny * wpb - 1) / (c->ny * wpb);
-  const int nch = std::max(1, std::min(want, c->nz / 8));
+  const int nch = std::max(1, std::min(want, c->nz / 16));
   *kc = (c->nz + nch - 1) / nch;
   return dim3(1u, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
 }
@@ -889,12 +887,9 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
     if (nt <= 256) {
       if (cf.scalar) PTC_ROWS(true, 256);
       else PTC_ROWS(false, 256);
-    } else if (nt <= 512) {
+    } else {
       if (cf.scalar) PTC_ROWS(true, 512);
       else PTC_ROWS(false, 512);
-    } else {
-      if (cf.scalar) PTC_ROWS(true, 1024);
-      else PTC_ROWS(false, 1024);
     }
 #undef PTC_ROWS
   } else {
@@ -903,7 +898,7 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   }
   PTC_HIP(hipGetLastError());
   if (cf.s3_mode) {
-    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,
+    hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,
                        c->prm.xnu, c->prm.ubulk, c->prm.ucrf, (double)(c->nx * c->nz), c->yly, c->pmAlpha,
                        c->pmAlphaC, c->dt);
     PTC_HIP(hipGetLastError());
@@ -930,7 +925,7 @@ int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *u
   hipLaunchKernelGGL(k_prsrc, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, ux, uy, uz, prsrc, 1.0 / c->pmAlpha,
                      c->d_red);
   PTC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_DIVMAX);
+  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal + S_DIVMAX);
   PTC_HIP(hipGetLastError());
   if (divmax_host) {
     PTC_HIP(hipMemcpyAsync(c->h_pinned, c->d_scal + S_DIVMAX, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -978,6 +973,19 @@ int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *
   return launch_bc(c, b, ux, uy, uz, const_cast<double *>(uy_ym), nullptr);
 }
 
+// the velocity wall fills and the pressure's Neumann ghosts in one launch
+// (the end of a substep, src/m_ChannelSystem.f90:135-136)
+int launch_bc_vel_pr(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, double *p) {
+  BcArgs b{};
+  b.mode = 5;
+  b.flow_type = c->prm.flow_type;
+  b.uxb_m = c->prm.uxbc[2];
+  b.uxb_p = c->prm.uxbc[3];
+  b.uyb_p = c->prm.uybc[3];
+  b.uzb_m = c->prm.uzbc[2];
+  b.uzb_p = c->prm.uzbc[3];
+  return launch_bc(c, b, ux, uy, uz, const_cast<double *>(uy_ym), p);
+}
 int launch_bc_pr(ptc_ctx *c, double *p) {
   BcArgs b{};
   b.mode = 2;
@@ -1026,9 +1034,9 @@ int launch_monitor(ptc_ctx *c, int which, const double *ux, const double *uy, co
   hipLaunchKernelGGL(k_monitor, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, which, ux, uy, uz, s, c->d_red);
   PTC_HIP(hipGetLastError());
   if (which <= 4)
-    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
   else
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal + S_MON0);
   PTC_HIP(hipGetLastError());
   PTC_HIP(hipMemcpyAsync(c->h_pinned, c->d_scal + S_MON0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PTC_HIP(hipStreamSynchronize(c->stream));

@@ -465,30 +465,35 @@ __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict_
 // ---- Crank-Nicolson y-solves (u += du), partitioned -----------------------
 // kind: 0 = u/w (ForCN tables), 1 = v (row 1 Dirichlet), 2 = scalar (ScCN tables)
 template <int MM, int LPB>
-__global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *__restrict__ u, const double *__restrict__ rhs,
-                                                     const double *__restrict__ duy_ym,
-                                                     const double *__restrict__ shift, const double *__restrict__ met,
-                                                     int nyh, int rowA, int rowB, int rowC, double beta, int kind,
-                                                     int nx, int ny, int P) {
+__global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs jobs, const double *__restrict__ met,
+                                                                      int nyh, int nx, int ny, int P) {
   extern __shared__ double smem[];
   double *RA = smem;
   double *RC = RA + 4 * P * LPB;
   double *RF = RC + 4 * P * LPB;
+  const CnJob &J = jobs.j[blockIdx.z];
+  double *__restrict__ u = J.u;
+  const double *__restrict__ rhs = J.rhs;
+  const int kind = J.kind;
   const int lane = threadIdx.x % LPB;
   const int p = chunk_of_thread<LPB>();
   const int i = blockIdx.x * LPB + lane, k = blockIdx.y;
   const bool valid = i < nx;
   const int s = p * MM;
   const long long col = (long long)(valid ? i : 0) + (long long)nx * ((long long)(ny + 2) * k);
-  const double sh = shift ? shift[0] : 0.0;
-  double f[MM];
+  const double sh = J.shift ? J.shift[0] : 0.0;
+  // unconditional loads (padding rows read row ny), masked by a multiply; u
+  // is read here too, so the final update waits on no load
+  double f[MM], u0[MM];
 #pragma unroll
-  for (int r = 0; r < MM; ++r) {  // unconditional loads (padding rows read row ny), masked by a multiply
-    const double v = rhs[col + (long long)nx * (min(s + r, ny - 1) + 1)];
+  for (int r = 0; r < MM; ++r) {
+    const long long o = col + (long long)nx * (min(s + r, ny - 1) + 1);
+    const double v = rhs[o];
+    u0[r] = u[o];
     f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   }
-  if (kind == 1 && s == 0) f[0] = valid ? duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
-  CnRows getc{met + rowA * nyh, met + rowB * nyh, met + rowC * nyh, beta, ny, kind};
+  if (kind == 1 && s == 0) f[0] = valid ? J.duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
+  CnRows getc{met + J.rA * nyh, met + J.rB * nyh, met + J.rC * nyh, J.beta, ny, kind};
   double A[MM], C[MM];
   double aS, cS, aE, cE, fS, fE, xs, xe;
   chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
@@ -497,10 +502,7 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(double *
   if (valid) {
 #pragma unroll
     for (int r = 0; r < MM; ++r)
-      if (s + r < ny) {
-        const long long o = col + (long long)nx * (s + r + 1);
-        u[o] = u[o] + f[r];
-      }
+      if (s + r < ny) u[col + (long long)nx * (s + r + 1)] = u0[r] + f[r];
   }
 }
 
@@ -687,22 +689,33 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
 }
 
 // kind 0 (u/w), 1 (v), 2 (scalar); shift: optional device scalar added to rhs
-int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
-                  const double *shift) {
+CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, double beta, const double *shift) {
+  CnJob J{};
+  J.u = u;
+  J.rhs = rhs;
+  J.duy_ym = duy_ym;
+  J.shift = shift;
+  J.beta = beta;
+  J.kind = kind;
+  if (kind == 0) { J.rA = M_AM2CCN; J.rB = M_AC2CCN; J.rC = M_AP2CCN; }
+  else if (kind == 1) { J.rA = M_AM2P; J.rB = M_AC2P; J.rC = M_AP2P; }
+  else { J.rA = M_AM2CSCCN; J.rB = M_AC2CSCCN; J.rC = M_AP2CSCCN; }
+  return J;
+}
+
+// one launch for all the jobs (grid z = job)
+int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
+  PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
   int nP = 1, mmax = 16, lpb = 64;
   PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
-  int rA, rB, rC;
-  if (kind == 0) { rA = M_AM2CCN; rB = M_AC2CCN; rC = M_AP2CCN; }
-  else if (kind == 1) { rA = M_AM2P; rB = M_AC2P; rC = M_AP2P; }
-  else { rA = M_AM2CSCCN; rB = M_AC2CSCCN; rC = M_AP2CSCCN; }
-  dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz);
+  dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz, (unsigned)jobs.n);
   const size_t lds = (size_t)4 * nP * lpb * 24;
-#define PTC_CN(MM, LPB)                                                                                       \
-  if (mmax == MM && lpb == LPB) {                                                                             \
-    hipLaunchKernelGGL((k_tri_cn_part<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, u, rhs, duy_ym, shift,  \
-                       c->d_met, c->ny + 2, rA, rB, rC, beta, kind, c->nx, c->ny, nP);                        \
-    PTC_HIP(hipGetLastError());                                                                               \
-    return PTC_OK;                                                                                            \
+#define PTC_CN(MM, LPB)                                                                                      \
+  if (mmax == MM && lpb == LPB) {                                                                            \
+    hipLaunchKernelGGL((k_tri_cn_part<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_met,       \
+                       c->ny + 2, c->nx, c->ny, nP);                                                         \
+    PTC_HIP(hipGetLastError());                                                                              \
+    return PTC_OK;                                                                                           \
   }
   PTC_CN(4, 64)
   PTC_CN(8, 64)
@@ -711,6 +724,14 @@ int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const doub
 #undef PTC_CN
   set_error("bad tridiagonal launch shape");
   return PTC_E_STATE;
+}
+
+int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
+                  const double *shift) {
+  CnJobs jobs{};
+  jobs.j[0] = cn_job(kind, u, rhs, duy_ym, beta, shift);
+  jobs.n = 1;
+  return launch_tri_cn_jobs(c, jobs);
 }
 
 }  // namespace ptc
