@@ -47,8 +47,9 @@ def stage_kernels(plan: dict, nx: int, nz: int) -> list:
     entries carry the x and z kernels together."""
     d = "HIP_vector_type<double"
     h = nx // 2
+    st = "true" if plan.get("streaming") else "false"
     if plan["x_fast"]:
-        xf, xi = f"k2_fft_x_fwd<{h}, {d}", f"k2_fft_x_inv<{h}, {d}, 2u>, false>"
+        xf, xi = f"k2_fft_x_fwd<{h}, {d}, 2u>, {st}>", f"k2_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
     else:
         xf, xi = "k_fft_x_fwd<", "k_fft_x_inv<"
     if plan["z_fast"]:
@@ -59,7 +60,7 @@ def stage_kernels(plan: dict, nx: int, nz: int) -> list:
     if plan["tri"] != "partitioned":
         y = "k_tri_poisson_thomas"
     elif plan.get("tri_complex_lines"):
-        y = f"k_tri_poisson_cplx<{mm}, {plan['tri_complex_lines']}, {d}"
+        y = f"k_tri_poisson_cplx<{mm}, {plan['tri_complex_lines']}, {d}, 2u>, {st}>"
     elif plan.get("tri_lds_lines"):
         y = f"k_tri_poisson_lds<{mm}, {plan['tri_lds_lines']}, {d}"
     else:

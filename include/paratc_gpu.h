@@ -189,19 +189,25 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *                           the real and imaginary parts as separate lanes;
  *                           0 (automatic): complex lines, 16 per workgroup,
  *                           where a shape fits and no other y-solve knob is
- *                           set. */
+ *                           set.
+ *   PTC_POISSON_STREAMING   nontemporal (cache-bypassing) loads and stores in
+ *                           the x passes and the complex-line y-solve: 1 on,
+ *                           -1 off, 0 (automatic) on where the spectrum is
+ *                           larger than the last-level cache. */
 enum {
   PTC_POISSON_CHUNK_ROWS = 1,
   PTC_POISSON_SLAB_PLANES = 2,
   PTC_POISSON_TRI_LINES = 3,
   PTC_POISSON_FAST_ZTILE = 4,
   PTC_POISSON_TRI_LANES = 5,
-  PTC_POISSON_TRI_COMPLEX = 6
+  PTC_POISSON_TRI_COMPLEX = 6,
+  PTC_POISSON_STREAMING = 7
 };
 int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
 /* the value a plan knob has in the plan in use (chunk rows, slab planes,
  * LDS-staged lines, z tile, lanes per workgroup, complex lines per
- * workgroup; 0 where the knob's variant is not in use) */
+ * workgroup, streaming on (1) or off (0); 0 where the knob's variant is not
+ * in use) */
 int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
 /* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per
@@ -269,6 +275,26 @@ int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
  *  its stats file (:944).  The spectra of clcStat are not part of this. */
 int ptc_clc_stat(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar);
 int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double *prgradsum, int reset);
+
+/* dump_visu (src/m_IOAndVisu.f90:182-385): the cell-centre field `which` of
+ * the current ux, uy, uz, pressure (and scalar) at every iskip/jskip/kskip-th
+ * point from the first, as decomp_2d_write_every keeps them with from1
+ * (src/m_Decomp2d.f90:1232-1389), x fastest, as real_bytes = 4 or 8 reals
+ * (mytype_save) into host_out of count = ceil(nx/iskip) ceil(ny/jskip)
+ * ceil(nz/kskip) values.  ux carries uCRF; PTC_VISU_WMAG is |vorticity|. */
+enum {
+  PTC_VISU_UX = 0,
+  PTC_VISU_UY = 1,
+  PTC_VISU_UZ = 2,
+  PTC_VISU_PR = 3,
+  PTC_VISU_WX = 4,
+  PTC_VISU_WY = 5,
+  PTC_VISU_WZ = 6,
+  PTC_VISU_WMAG = 7,
+  PTC_VISU_SCALAR = 8
+};
+int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
+                   long long count);
 
 /* ---- Ensembles: many independent cases on one GPU.  No counterpart in the
  *      reference, which runs one case per job; this is the serving layer

@@ -88,6 +88,9 @@ def lib():
         L.orc_calc_ux_aver.argtypes = [vp, dp]
         L.orc_clc_mean_value.argtypes = [vp, dp, dp, dp]
         L.orc_clc_stat.argtypes = [vp] + [dp] * 6
+        L.orc_dump_visu.argtypes = [vp, C.c_int] + [dp] * 5 + [C.c_int] * 3 + [dp]
+        L.orc_visu_count.restype = C.c_longlong
+        L.orc_visu_count.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.orc_channel_iterate.argtypes = [vp, C.c_int] + [dp] * 15 + [dp]
         _lib = L
     return _lib
@@ -263,6 +266,16 @@ class Oracle:
     def clcStat(self, ux, uy, uz, p, scalar, sumstat):
         """Adds this sample's plane means to sumstat (NCHASTAT, ny+1), Fortran order."""
         self.L.orc_clc_stat(self.h, _p(ux), _p(uy), _p(uz), _p(p), _p(scalar), _p(sumstat))
+
+    def dump_visu(self, which: int, ux, uy, uz, p, scalar=None, skip=(1, 1, 1)):
+        """dump_visu's field `which` (0 ux .. 8 scalar) at the kept cell centres,
+        shape (nxo, nyo, nzo) in Fortran order."""
+        iskip, jskip, kskip = skip
+        shape = tuple((n + s - 1) // s for n, s in zip((self.case.nx, self.case.ny, self.case.nz), skip))
+        out = np.zeros(shape, order="F")
+        sc = scalar if scalar is not None else ux
+        self.L.orc_dump_visu(self.h, which, _p(ux), _p(uy), _p(uz), _p(p), _p(sc), iskip, jskip, kskip, _p(out))
+        return out
 
     def ChannelIterate(self, st: "State", choice: int = 1):
         mon = np.zeros(7)

@@ -1351,3 +1351,58 @@ void orc_channel_iterate(orc_ctx *c, int poisson_choice,
   mon[5] = cflmp;
   mon[6] = dt;
 }
+
+/* ---- dump_visu (src/m_IOAndVisu.f90:182-385), one rank, cell-centre form --
+ * The field `which` (0 ux, 1 uy, 2 uz, 3 pr, 4 wx, 5 wy, 6 wz, 7 wMag,
+ * 8 scalar) at the cell centres that decomp_2d_write_every keeps with
+ * from1 = .true. (src/m_Decomp2d.f90:1232-1389): i = 1, 1+iskip, ... (and the
+ * same in j and k), written x fastest.  The velocity, pressure and scalar
+ * arrays are haloed (x/z halos current, y ghost rows set). */
+static double visu_point(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz,
+                         const double *pr, const double *sc, int ic, int jc, int kc) {
+  const double half = 0.5, quarter = 0.25;
+  const int ip = ic + 1, im = ic - 1, jp = jc + 1, jm = jc - 1, kp = kc + 1, km = kc - 1;
+  const double rdx = c->rdx, rdz = c->rdz;
+  const double cac1 = c->rdyc[jc], cac2 = c->rdyc[jp], cac12 = cac1 - cac2;
+  switch (which) {
+    case 0: return half * (H(c, ux, ip, jc, kc) + H(c, ux, ic, jc, kc)) + c->cfg.ucrf; /* :199-214 */
+    case 1: return half * (H(c, uy, ic, jp, kc) + H(c, uy, ic, jc, kc));               /* :216-231 */
+    case 2: return half * (H(c, uz, ic, jc, kp) + H(c, uz, ic, jc, kc));               /* :233-248 */
+    case 3: return H(c, pr, ic, jc, kc);                                               /* :250-254 */
+    case 8: return H(c, sc, ic, jc, kc);                                               /* :257-260 */
+    default: break;
+  }
+  /* :263-369 */
+  const double dudy = ((H(c, ux, ip, jp, kc) + H(c, ux, ic, jp, kc)) * cac2 +
+                       (H(c, ux, ip, jc, kc) + H(c, ux, ic, jc, kc)) * cac12 -
+                       (H(c, ux, ip, jm, kc) + H(c, ux, ic, jm, kc)) * cac1) * quarter;
+  const double dudz = (H(c, ux, ip, jc, kp) + H(c, ux, ic, jc, kp) - H(c, ux, ip, jc, km) - H(c, ux, ic, jc, km)) * rdz * quarter;
+  const double dvdx = (H(c, uy, ip, jp, kc) - H(c, uy, im, jp, kc) + H(c, uy, ip, jc, kc) - H(c, uy, im, jc, kc)) * rdx * quarter;
+  const double dvdz = (H(c, uy, ic, jp, kp) + H(c, uy, ic, jc, kp) - H(c, uy, ic, jp, km) - H(c, uy, ic, jc, km)) * rdz * quarter;
+  const double dwdx = (H(c, uz, ip, jc, kp) - H(c, uz, im, jc, kp) + H(c, uz, ip, jc, kc) - H(c, uz, im, jc, kc)) * rdx * quarter;
+  const double dwdy = ((H(c, uz, ic, jp, kp) + H(c, uz, ic, jp, kc)) * cac2 +
+                       (H(c, uz, ic, jc, kp) + H(c, uz, ic, jc, kc)) * cac12 -
+                       (H(c, uz, ic, jm, kp) + H(c, uz, ic, jm, kc)) * cac1) * quarter;
+  const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+  switch (which) {
+    case 4: return vor_x;
+    case 5: return vor_y;
+    case 6: return vor_z;
+    default: return sqrt(vor_x * vor_x + vor_y * vor_y + vor_z * vor_z);
+  }
+}
+
+long long orc_visu_count(const orc_ctx *c, int iskip, int jskip, int kskip) {
+  const long long nxo = (c->nx + iskip - 1) / iskip, nyo = (c->ny + jskip - 1) / jskip, nzo = (c->nz + kskip - 1) / kskip;
+  return nxo * nyo * nzo;
+}
+
+void orc_dump_visu(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *pr,
+                   const double *sc, int iskip, int jskip, int kskip, double *out) {
+  const int nxo = (c->nx + iskip - 1) / iskip, nyo = (c->ny + jskip - 1) / jskip, nzo = (c->nz + kskip - 1) / kskip;
+  size_t q = 0;
+  for (int k = 0; k < nzo; k++)
+    for (int j = 0; j < nyo; j++)
+      for (int i = 0; i < nxo; i++)
+        out[q++] = visu_point(c, which, ux, uy, uz, pr, sc, 1 + i * iskip, 1 + j * jskip, 1 + k * kskip);
+}

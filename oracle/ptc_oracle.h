@@ -107,6 +107,14 @@ void orc_clc_mean_value(const orc_ctx *c, const double *ux, const double *scalar
 void orc_clc_stat(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pressure,
                   const double *scalar, double *sumstat);
 
+/* dump_visu's cell-centre fields (src/m_IOAndVisu.f90:182-385) on the points
+ * decomp_2d_write_every keeps (every iskip/jskip/kskip-th, from the first),
+ * x fastest: which = 0 ux (+uCRF), 1 uy, 2 uz, 3 pr, 4 wx, 5 wy, 6 wz,
+ * 7 |w|, 8 scalar.  orc_visu_count: the number of points written. */
+long long orc_visu_count(const orc_ctx *c, int iskip, int jskip, int kskip);
+void orc_dump_visu(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *pr,
+                   const double *sc, int iskip, int jskip, int kskip, double *out);
+
 /* One ChannelIterate time step (m_ChannelSystem.f90:80-145).  Buffers follow
  * the reference's aliasing: arr1 = RealArr1, arr2 = RealArr2, halo = RealHalo,
  * arrc = RealArrC.  mon receives [divmax1 (last substep), divmax2, vmax x3,

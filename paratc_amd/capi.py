@@ -27,8 +27,10 @@ PTC_E_LIMIT = -6
 L_HALO, L_INTERIOR, L_PLANE = 0, 1, 2
 PREC_FP64, PREC_FP32_REFINE = 0, 1
 TRI_AUTO, TRI_THOMAS, TRI_PARTITIONED = 0, 1, 2
+# dump_visu fields (include/paratc_gpu.h PTC_VISU_*)
+VISU_UX, VISU_UY, VISU_UZ, VISU_PR, VISU_WX, VISU_WY, VISU_WZ, VISU_WMAG, VISU_SCALAR = range(9)
 (POISSON_CHUNK_ROWS, POISSON_SLAB_PLANES, POISSON_TRI_LINES, POISSON_FAST_ZTILE, POISSON_TRI_LANES,
- POISSON_TRI_COMPLEX) = 1, 2, 3, 4, 5, 6
+ POISSON_TRI_COMPLEX, POISSON_STREAMING) = 1, 2, 3, 4, 5, 6, 7
 S_DIVMAX_PRSRC, S_PRGRADAVER, S_SIMTIME, S_DT = 0, 1, 2, 3
 
 # every symbol declared in include/paratc_gpu.h
@@ -42,7 +44,7 @@ EXPORTS = [
     "ptc_pressure_update", "ptc_set_bc_and_update_halo", "ptc_set_bc_and_update_halo_for_prsrc",
     "ptc_set_bc_and_update_halo_pr", "ptc_set_bc_and_update_halo_scalar", "ptc_update_uy_ym",
     "ptc_calc_max_cfl", "ptc_check_divergence", "ptc_calc_vmax", "ptc_calc_ux_aver",
-    "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar", "ptc_clc_stat", "ptc_get_stat", "ptc_step_graphs",
+    "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar", "ptc_clc_stat", "ptc_get_stat", "ptc_visu_field", "ptc_step_graphs",
     "ptc_ensemble_create", "ptc_ensemble_destroy", "ptc_case_footprint", "ptc_ensemble_admit",
     "ptc_ensemble_step", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
     "ptc_ensemble_latency",
@@ -132,6 +134,7 @@ def load(path: str = LIB_PATH):
     L.ptc_clc_stat.argtypes = [vp, ip, ip, ip, ip, ip]
     L.ptc_step_graphs.argtypes = [vp, ip]
     L.ptc_get_stat.argtypes = [vp, dp, C.POINTER(ip), C.POINTER(ip), dp, ip]
+    L.ptc_visu_field.argtypes = [vp, ip, ip, ip, ip, ip, vp, C.c_longlong]
     sz, ll = C.c_size_t, C.c_longlong
     L.ptc_ensemble_create.argtypes = [sz, C.POINTER(vp)]
     L.ptc_ensemble_destroy.argtypes = [vp]

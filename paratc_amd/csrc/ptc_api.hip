@@ -181,12 +181,17 @@ int channel_step(ptc_ctx *c, double *mon) {
       CnJobs cn{};
       if (scal) cn.j[cn.n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
       cn.j[cn.n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
-      cn.j[cn.n++] = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
+      // clcU2Hat with SetBC_and_UpdateHaloForPrSrc's v wall rows (:122)
+      CnJob v = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
+      v.wall = uym;
+      v.top = c->prm.flow_type == 1 ? c->prm.uybc[3] : 0.0;
+      cn.j[cn.n++] = v;
       cn.j[cn.n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
       PTC_TRY(launch_tri_cn_jobs(c, cn));
     }
-    PTC_TRY(launch_bc_vel(c, ux, uy, uz, uym, true));
-    PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr));
+    // clcPrSrc; its divergence maximum is reduced only where it is read (the
+    // last substep's, reported by the monitors)
+    PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr, ns == c->iadvance));
     PTC_TRY(poisson_solve(c, a1, hl, true));  // with SetBC_and_UpdateHalo_pr(prphiHalo) (:126)
     PTC_TRY(launch_vel_p_update(c, hl, ux, uy, uz, p));
     PTC_TRY(launch_bc_vel_pr(c, ux, uy, uz, uym, p));
@@ -453,6 +458,8 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   } else if (knob == PTC_POISSON_TRI_COMPLEX) {
     PTC_CHECK(value == -1 || value == 0 || value == 16 || value == 32, PTC_E_ARG,
               "complex lines per workgroup must be -1, 0, 16 or 32");
+  } else if (knob == PTC_POISSON_STREAMING) {
+    PTC_CHECK(value >= -1 && value <= 1, PTC_E_ARG, "streaming must be -1, 0 or 1");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
@@ -463,6 +470,7 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   else if (knob == PTC_POISSON_TRI_LINES) ctx->pp.tri_ll_req = value;
   else if (knob == PTC_POISSON_TRI_LANES) ctx->pp.tri_lpb_req = value;
   else if (knob == PTC_POISSON_TRI_COMPLEX) ctx->pp.tri_cl_req = value;
+  else if (knob == PTC_POISSON_STREAMING) ctx->pp.stream_req = value;
   else ctx->pp.zfast_req = value;
   return poisson_build_plan(ctx);
 }
@@ -479,6 +487,7 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value) {
     case PTC_POISSON_FAST_ZTILE: *value = P.z_fast ? P.ztile : 0; break;
     case PTC_POISSON_TRI_LANES: *value = part && !P.tri_ll && !P.tri_cl ? P.tri_lpb : 0; break;
     case PTC_POISSON_TRI_COMPLEX: *value = part ? P.tri_cl : 0; break;
+    case PTC_POISSON_STREAMING: *value = P.stream ? 1 : 0; break;
     default: PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
   return PTC_OK;
@@ -761,6 +770,38 @@ int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double
   if (nstat) *nstat = ctx->prm.scalar ? 40 : 35;
   if (nfstime) *nfstime = ctx->nfstime;
   return read_stat(ctx, sumstat, prgradsum, reset);
+}
+
+int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
+                   long long count) {
+  PTC_CTX(ctx);
+  PTC_CHECK(which >= PTC_VISU_UX && which <= PTC_VISU_SCALAR, PTC_E_ARG, "unknown visualisation field");
+  PTC_CHECK(iskip >= 1 && jskip >= 1 && kskip >= 1, PTC_E_ARG, "skips must be >= 1");
+  PTC_CHECK(real_bytes == 4 || real_bytes == 8, PTC_E_ARG, "visualisation reals are 4 or 8 bytes");
+  PTC_CHECK(host_out != nullptr, PTC_E_ARG, "null output");
+  PTC_CHECK(which != PTC_VISU_SCALAR || ctx->prm.scalar, PTC_E_ARG, "no scalar in this build");
+  const long long nxo = (ctx->nx + iskip - 1) / iskip, nyo = (ctx->ny + jskip - 1) / jskip,
+                  nzo = (ctx->nz + kskip - 1) / kskip;
+  PTC_CHECK(count == nxo * nyo * nzo, PTC_E_ARG,
+            "output holds " + std::to_string(count) + " values, the field " + std::to_string(nxo * nyo * nzo));
+  double *ux, *uy, *uz, *p, *sc = nullptr;
+  PTC_TRY(field(ctx, PTC_F_UX, &ux));
+  PTC_TRY(field(ctx, PTC_F_UY, &uy));
+  PTC_TRY(field(ctx, PTC_F_UZ, &uz));
+  PTC_TRY(field(ctx, PTC_F_PR, &p));
+  if (ctx->prm.scalar) PTC_TRY(field(ctx, PTC_F_SCALAR, &sc));
+  const size_t bytes = (size_t)count * (size_t)real_bytes;
+  if (ctx->visu_bytes < bytes) {
+    if (ctx->d_visu) (void)hipFree(ctx->d_visu);
+    ctx->d_visu = nullptr;
+    ctx->visu_bytes = 0;
+    PTC_HIP(hipMalloc(&ctx->d_visu, bytes));
+    ctx->visu_bytes = bytes;
+  }
+  PTC_TRY(launch_visu(ctx, which, ux, uy, uz, p, sc, iskip, jskip, kskip, real_bytes, ctx->d_visu));
+  PTC_HIP(hipMemcpyAsync(host_out, ctx->d_visu, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  return PTC_OK;
 }
 
 int ptc_step_graphs(ptc_ctx *ctx, int enable) {

@@ -347,6 +347,7 @@ void destroy(ptc_ctx *c) {
   if (c->d_wnz) (void)hipFree(c->d_wnz);
   if (c->d_red) (void)hipFree(c->d_red);
   if (c->d_stat) (void)hipFree(c->d_stat);
+  if (c->d_visu) (void)hipFree(c->d_visu);
   if (c->d_scal) (void)hipFree(c->d_scal);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -73,6 +73,38 @@ __device__ __forceinline__ double2 ld2(const float2 &v) { return make_double2(v.
 __device__ __forceinline__ void st2(double2 &d, double2 v) { d = v; }
 __device__ __forceinline__ void st2(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
 
+// Streaming (nontemporal) forms: the access is marked as not to be kept in
+// the caches.  A pass over a field larger than the last-level cache reads
+// and writes each value once, so keeping it only evicts other data.
+typedef double st_d2 __attribute__((ext_vector_type(2)));
+typedef float st_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld2s(const double2 &v) {
+  const st_d2 w = __builtin_nontemporal_load(reinterpret_cast<const st_d2 *>(&v));
+  return make_double2(w.x, w.y);
+}
+__device__ __forceinline__ double2 ld2s(const float2 &v) {
+  const st_f2 w = __builtin_nontemporal_load(reinterpret_cast<const st_f2 *>(&v));
+  return make_double2(w.x, w.y);
+}
+__device__ __forceinline__ void st2s(double2 &d, double2 v) {
+  st_d2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<st_d2 *>(&d));
+}
+__device__ __forceinline__ void st2s(float2 &d, double2 v) {
+  st_f2 w = {(float)v.x, (float)v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<st_f2 *>(&d));
+}
+template <bool STREAM, typename S>
+__device__ __forceinline__ double2 ld2x(const S &v) {
+  if constexpr (STREAM) return ld2s(v);
+  else return ld2(v);
+}
+template <bool STREAM, typename S>
+__device__ __forceinline__ void st2x(S &d, double2 v) {
+  if constexpr (STREAM) st2s(d, v);
+  else st2(d, v);
+}
+
 // LDS padding of one line: one slot per 8 elements
 __device__ __forceinline__ int pad8(int m) { return m + (m >> 3); }
 constexpr int padded(int n) { return n + n / 8 + 1; }
@@ -356,7 +388,7 @@ constexpr int f2_xblock() {
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
-template <int N, typename S>
+template <int N, typename S, bool STREAM>
 __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
@@ -374,7 +406,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
   const long long sq = jl + (long long)ny * k;  // spectral row S[k][jl]
   const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   double2 *ls = sh + (l < L ? l : 0) * PN;
-  auto load0 = [&](int m) { return live ? row[m] : make_double2(0.0, 0.0); };
+  auto load0 = [&](int m) { return live ? f2::ld2x<STREAM>(row[m]) : make_double2(0.0, 0.0); };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
@@ -391,7 +423,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
       const double2 wd = f2::cmul(wx[kx], d);
       X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
     }
-    f2::st2(out[kx], X);
+    f2::st2x<STREAM>(out[kx], X);
   }
 }
 
@@ -400,7 +432,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
 // device scalar *shift (the mean the correction removes).  ghost: the rows
 // j = 1 and j = ny also go to the Neumann ghost rows j = 0 and j = ny+1
 // (SetBC_and_UpdateHalo_pr, src/m_BC_and_Halo.f90:115-120).
-template <int N, typename S, bool ACC>
+template <int N, typename S, bool ACC, bool STREAM>
 __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
@@ -425,7 +457,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
   constexpr int NT = N / G + 1;
   double2 pre[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) pre[t] = f2::ld2(in[min(g + t * G, N)]);
+  for (int t = 0; t < NT; ++t) pre[t] = f2::ld2x<STREAM>(in[min(g + t * G, N)]);
 #pragma unroll
   for (int t = 0; t < NT; ++t) ls[f2::pad8(min(g + t * G, N))] = pre[t];
   __syncthreads();
@@ -450,13 +482,13 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
     if (!live) return;
     double2 x;
     if (ACC) {
-      const double2 o = row[m];
+      const double2 o = f2::ld2x<STREAM>(row[m]);
       x = make_double2(o.x + v.x * scale - sh0, o.y + v.y * scale - sh0);
     } else {
       x = make_double2(v.x * scale, v.y * scale);
     }
-    row[m] = x;
-    if (grow) grow[m] = x;
+    f2::st2x<STREAM>(row[m], x);
+    if (grow) f2::st2x<STREAM>(grow[m], x);
   };
   f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
 }
@@ -477,48 +509,53 @@ __global__ void __launch_bounds__(f2_threads<N>() * T) k2_fft_z(S *__restrict__ 
 }
 
 // ---- dispatch ----------------------------------------------------------------
-template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
+template <int N, typename S, bool STREAM>
+static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
-  if (c->pp.prec == PTC_PREC_FP32_REFINE)
-    hipLaunchKernelGGL((k2_fft_x_fwd<N, float2>), dim3(nblk), dim3(L * G), 0, c->stream, src, (float2 *)spec,
-                       c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
-  else
-    hipLaunchKernelGGL((k2_fft_x_fwd<N, double2>), dim3(nblk), dim3(L * G), 0, c->stream, src, (double2 *)spec,
-                       c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
+  hipLaunchKernelGGL((k2_fft_x_fwd<N, S, STREAM>), dim3(nblk), dim3(L * G), 0, c->stream, src, spec, c->pp.fx.tw,
+                     c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
+}
+
+template <int N>
+static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
+  const bool f32 = c->pp.prec == PTC_PREC_FP32_REFINE;
+  if (f32 && c->pp.stream) x_fwd_launch<N, float2, true>(c, src, (float2 *)spec, j0, nj);
+  else if (f32) x_fwd_launch<N, float2, false>(c, src, (float2 *)spec, j0, nj);
+  else if (c->pp.stream) x_fwd_launch<N, double2, true>(c, src, (double2 *)spec, j0, nj);
+  else x_fwd_launch<N, double2, false>(c, src, (double2 *)spec, j0, nj);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
-template <int N>
-static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost) {
+template <int N, typename S, bool ACC, bool STREAM>
+static void x_inv_launch(ptc_ctx *c, const S *spec, double *dst, double scale, int j0, int nj, bool ghost) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const long long nlines = (long long)nj * c->nz;
   const unsigned nblk = (unsigned)((nlines + L - 1) / L);
-  const dim3 grid(nblk), blk(L * G);
-  const auto *tw = c->pp.fx.tw;
-  const auto *wx = c->pp.d_wx;
-  const int ny = c->ny, nz = c->nz, pitch = c->pp.pitch;
-  const long long rl = c->nx;
-  const double *shift = c->d_scal + kSlotPinShift;
-  if (c->pp.prec == PTC_PREC_FP32_REFINE) {
-    if (acc)
-      hipLaunchKernelGGL((k2_fft_x_inv<N, float2, true>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw, wx,
-                         ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
-    else
-      hipLaunchKernelGGL((k2_fft_x_inv<N, float2, false>), grid, blk, 0, c->stream, (const float2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
+  hipLaunchKernelGGL((k2_fft_x_inv<N, S, ACC, STREAM>), dim3(nblk), dim3(L * G), 0, c->stream, spec, dst,
+                     c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, scale,
+                     c->d_scal + kSlotPinShift, ghost ? 1 : 0);
+}
+
+template <int N>
+static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost) {
+  const bool f32 = c->pp.prec == PTC_PREC_FP32_REFINE, st = c->pp.stream;
+  const float2 *s4 = (const float2 *)spec;
+  const double2 *s8 = (const double2 *)spec;
+  if (f32) {
+    if (acc && st) x_inv_launch<N, float2, true, true>(c, s4, dst, scale, j0, nj, ghost);
+    else if (acc) x_inv_launch<N, float2, true, false>(c, s4, dst, scale, j0, nj, ghost);
+    else if (st) x_inv_launch<N, float2, false, true>(c, s4, dst, scale, j0, nj, ghost);
+    else x_inv_launch<N, float2, false, false>(c, s4, dst, scale, j0, nj, ghost);
   } else {
-    if (acc)
-      hipLaunchKernelGGL((k2_fft_x_inv<N, double2, true>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
-    else
-      hipLaunchKernelGGL((k2_fft_x_inv<N, double2, false>), grid, blk, 0, c->stream, (const double2 *)spec, dst, tw,
-                         wx, ny, nz, j0, nj, pitch, rl, scale, shift, ghost ? 1 : 0);
+    // the accumulating form belongs to the FP32-storage refinement only
+    PTC_CHECK(!acc, PTC_E_STATE, "accumulating x-inverse with FP64 spectra");
+    if (st) x_inv_launch<N, double2, false, true>(c, s8, dst, scale, j0, nj, ghost);
+    else x_inv_launch<N, double2, false, false>(c, s8, dst, scale, j0, nj, ghost);
   }
   PTC_HIP(hipGetLastError());
   return PTC_OK;

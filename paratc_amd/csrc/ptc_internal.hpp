@@ -105,6 +105,8 @@ struct PoissonPlan {
   int tri_ll = 0;     // in use
   int tri_cl_req = 0; // complex-line y-solve lines per workgroup (0 = automatic, -1 = real-line kernels)
   int tri_cl = 0;     // in use
+  int stream_req = 0; // streaming (nontemporal) x and y passes: -1 off, 0 by size, 1 on
+  bool stream = false;  // in use
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
@@ -150,6 +152,8 @@ struct ptc_ctx {
   double divmax_prsrc = 0;
   ptc::PoissonPlan pp;
   double *d_stat = nullptr;  // clcStat accumulator SumStat(NCHASTAT, ny+1)
+  void *d_visu = nullptr;    // dump_visu staging (visu_bytes)
+  size_t visu_bytes = 0;
   int nfstime = 0;           // samples in it
   bool step_graphs = true;               // ptc_channel_iterate replays a captured step when dt is fixed
   hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
@@ -207,11 +211,13 @@ int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj);
 int launch_tri_poisson(ptc_ctx *c, void *spec);
 // Crank-Nicolson y-solves u += (I - beta L_y)^-1 rhs: one job per field;
 // kind 0 (u/w), 1 (v, wall row from duy_ym), 2 (scalar); shift: optional
-// device scalar added to rhs
+// device scalar added to rhs.  wall (kind 1, may be NULL): afterwards set
+// the wall row to the plane wall and row ny+1 to top, as
+// SetBC_and_UpdateHaloForPrSrc does (src/m_BC_and_Halo.f90:78-98).
 struct CnJob {
   double *u;
-  const double *rhs, *duy_ym, *shift;
-  double beta;
+  const double *rhs, *duy_ym, *shift, *wall;
+  double beta, top;
   int kind, rA, rB, rC;
 };
 constexpr int kCnMaxJobs = 4;
@@ -229,7 +235,7 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
                const double *sc, double *rhsx, double *rhsy, double *rhsz, double *rhsc,
                double *hx, double *hy, double *hz, double *hc, const double *p);
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
-                 double *divmax_host);
+                 double *divmax_host, bool reduce = true);
 int launch_vel_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz);
 int launch_pressure_update(ptc_ctx *c, double *p, const double *phi);
 int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p);
@@ -237,6 +243,8 @@ int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *
 int launch_bc_pr(ptc_ctx *c, double *p);
 int launch_bc_vel_pr(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, double *p);
 int launch_bc_scalar(ptc_ctx *c, double *s);
+int launch_visu(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *p,
+                const double *sc, int iskip, int jskip, int kskip, int real_bytes, void *dev_out);
 int launch_uy_ym(ptc_ctx *c, double *uy_ym, double *duy_ym);
 int launch_monitor(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz,
                    const double *s, double *out);

@@ -61,6 +61,7 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.tri_ll_req = P.tri_ll_req;
   keep.tri_lpb_req = P.tri_lpb_req;
   keep.tri_cl_req = P.tri_cl_req;
+  keep.stream_req = P.stream_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.zfast_req = P.zfast_req;
@@ -171,6 +172,11 @@ int poisson_build_plan(ptc_ctx *c) {
   P.spec_elems = (size_t)P.pitch * c->ny * c->nz;
   const size_t spec_bytes = (P.prec == PTC_PREC_FP32_REFINE ? sizeof(float2) : sizeof(double2)) * P.spec_elems;
   P.spec_bytes = spec_bytes;
+  // streaming (nontemporal) x passes and y-solve where the spectrum is larger
+  // than the 256 MB last-level cache: such a pass touches every value once,
+  // and keeping it cached only evicts other data
+  constexpr size_t kStreamBytes = size_t(256) << 20;
+  P.stream = P.stream_req > 0 || (P.stream_req == 0 && spec_bytes >= kStreamBytes);
   if (hipMalloc(&P.d_spec, spec_bytes) != hipSuccess) {
     P.d_spec = nullptr;
     set_error("spectral buffer allocation failed");

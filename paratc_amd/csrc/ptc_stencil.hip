@@ -771,6 +771,72 @@ __global__ void k_bc(Geo g, BcArgs b, double *__restrict__ a0, double *__restric
   }
 }
 
+// ---- dump_visu (src/m_IOAndVisu.f90:182-385) ----------------------------------
+// The cell-centre field `which` at the points decomp_2d_write_every keeps
+// (every iskip/jskip/kskip-th from the first, src/m_Decomp2d.f90:1232-1389),
+// x fastest, as FP64 (R = double) or FP32 (R = float): 0 ux + uCRF, 1 uy,
+// 2 uz, 3 p, 4-6 the vorticity components, 7 its magnitude, 8 the scalar.
+// One thread per written point: x from the block's x index, the kept y row and
+// z plane from the grid's y and z.
+template <typename R>
+__global__ void __launch_bounds__(64) k_visu(Geo g, int which, const double *__restrict__ ux,
+                                             const double *__restrict__ uy, const double *__restrict__ uz,
+                                             const double *__restrict__ p, const double *__restrict__ sc,
+                                             double ucrf, int iskip, int jskip, int kskip, int nxo,
+                                             R *__restrict__ out) {
+  const int io = blockIdx.x * 64 + threadIdx.x, jo = blockIdx.y, ko = blockIdx.z;
+  if (io >= nxo) return;
+  const int nx = g.nx, nz = g.nz;
+  const int i = io * iskip, j = 1 + jo * jskip, k = ko * kskip;
+  const int ip = i + 1 < nx ? i + 1 : i + 1 - nx, im = i > 0 ? i - 1 : nx - 1;
+  const int kp = k + 1 < nz ? k + 1 : k + 1 - nz, km = k > 0 ? k - 1 : nz - 1;
+  const int jp = j + 1, jm = j - 1;
+  auto at = [&](const double *a, int ii, int jj, int kk) { return a[ii + (long long)nx * jj + g.plane * kk]; };
+  const double half = 0.5, quarter = 0.25;
+  double v;
+  if (which == 0) {
+    v = half * (at(ux, ip, j, k) + at(ux, i, j, k)) + ucrf;
+  } else if (which == 1) {
+    v = half * (at(uy, i, jp, k) + at(uy, i, j, k));
+  } else if (which == 2) {
+    v = half * (at(uz, i, j, kp) + at(uz, i, j, k));
+  } else if (which == 3) {
+    v = at(p, i, j, k);
+  } else if (which == 8) {
+    v = at(sc, i, j, k);
+  } else {
+    const double *M = g.met;
+    const int W = g.nyh;
+    const double cac1 = M[M_RDYC * W + j], cac2 = M[M_RDYC * W + jp], cac12 = cac1 - cac2;
+    const double rdx = g.rdx, rdz = g.rdz;
+    const double dudy = ((at(ux, ip, jp, k) + at(ux, i, jp, k)) * cac2 + (at(ux, ip, j, k) + at(ux, i, j, k)) * cac12 -
+                         (at(ux, ip, jm, k) + at(ux, i, jm, k)) * cac1) * quarter;
+    const double dudz = (at(ux, ip, j, kp) + at(ux, i, j, kp) - at(ux, ip, j, km) - at(ux, i, j, km)) * rdz * quarter;
+    const double dvdx = (at(uy, ip, jp, k) - at(uy, im, jp, k) + at(uy, ip, j, k) - at(uy, im, j, k)) * rdx * quarter;
+    const double dvdz = (at(uy, i, jp, kp) + at(uy, i, j, kp) - at(uy, i, jp, km) - at(uy, i, j, km)) * rdz * quarter;
+    const double dwdx = (at(uz, ip, j, kp) - at(uz, im, j, kp) + at(uz, ip, j, k) - at(uz, im, j, k)) * rdx * quarter;
+    const double dwdy = ((at(uz, i, jp, kp) + at(uz, i, jp, k)) * cac2 + (at(uz, i, j, kp) + at(uz, i, j, k)) * cac12 -
+                         (at(uz, i, jm, kp) + at(uz, i, jm, k)) * cac1) * quarter;
+    const double vx = dwdy - dvdz, vy = dudz - dwdx, vz = dvdx - dudy;
+    v = which == 4 ? vx : which == 5 ? vy : which == 6 ? vz : sqrt(vx * vx + vy * vy + vz * vz);
+  }
+  out[io + (long long)nxo * (jo + (long long)gridDim.y * ko)] = (R)v;
+}
+
+int launch_visu(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *p,
+                const double *sc, int iskip, int jskip, int kskip, int real_bytes, void *dev_out) {
+  const int nxo = (c->nx + iskip - 1) / iskip, nyo = (c->ny + jskip - 1) / jskip, nzo = (c->nz + kskip - 1) / kskip;
+  const dim3 grid((unsigned)((nxo + 63) / 64), (unsigned)nyo, (unsigned)nzo);
+  if (real_bytes == 8)
+    hipLaunchKernelGGL(k_visu<double>, grid, dim3(64), 0, c->stream, c->geo, which, ux, uy, uz, p, sc, c->prm.ucrf,
+                       iskip, jskip, kskip, nxo, (double *)dev_out);
+  else
+    hipLaunchKernelGGL(k_visu<float>, grid, dim3(64), 0, c->stream, c->geo, which, ux, uy, uz, p, sc, c->prm.ucrf,
+                       iskip, jskip, kskip, nxo, (float *)dev_out);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
 // ---- monitors (m_Tools) -------------------------------------------------------
 // which: 0 CFL max (:24-49), 1 divergence max (:54-81), 2..4 |ux|,|uy|,|uz| max (:86-110),
 //        5 sum ux*dyp (:256-275), 6..8 sums dyp*(ux, c, c^2) (:281-301)
@@ -918,13 +984,16 @@ const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
 // captured step
 int stencil_prepare(ptc_ctx *c) { return ensure_red(c, std::max(pt_nblocks(c), rhs_nblocks(c))); }
 
+// reduce: form the divergence maximum in the device slot (always when it is
+// read back on the host)
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
-                 double *divmax_host) {
+                 double *divmax_host, bool reduce) {
   const int nb = pt_nblocks(c);
   PTC_TRY(ensure_red(c, nb));
   hipLaunchKernelGGL(k_prsrc, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, ux, uy, uz, prsrc, 1.0 / c->pmAlpha,
                      c->d_red);
   PTC_HIP(hipGetLastError());
+  if (!reduce && !divmax_host) return PTC_OK;
   hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal + S_DIVMAX);
   PTC_HIP(hipGetLastError());
   if (divmax_host) {

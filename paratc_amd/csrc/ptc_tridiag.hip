@@ -352,11 +352,46 @@ __global__ void __launch_bounds__(512) k_tri_poisson_lds(S *__restrict__ spec, c
 // p = thread / LC.  A wavefront holds 64 / LC chunks, so the row
 // coefficients are per lane; they are staged in LDS once per workgroup.  A
 // row is a contiguous LC * 16-byte access (LC * 8 bytes with FP32 storage).
-// spectrum storage <-> FP64 arithmetic
-__device__ __forceinline__ double2 cld(const double2 &v) { return v; }
-__device__ __forceinline__ double2 cld(const float2 &v) { return make_double2(v.x, v.y); }
-__device__ __forceinline__ void cst(double2 &d, double2 v) { d = v; }
-__device__ __forceinline__ void cst(float2 &d, double2 v) { d = make_float2((float)v.x, (float)v.y); }
+// spectrum storage <-> FP64 arithmetic; STREAM: nontemporal access (a pass
+// over a spectrum larger than the last-level cache)
+typedef double tr_d2 __attribute__((ext_vector_type(2)));
+typedef float tr_f2 __attribute__((ext_vector_type(2)));
+template <bool STREAM>
+__device__ __forceinline__ double2 cld(const double2 &v) {
+  if constexpr (STREAM) {
+    const tr_d2 w = __builtin_nontemporal_load(reinterpret_cast<const tr_d2 *>(&v));
+    return make_double2(w.x, w.y);
+  } else {
+    return v;
+  }
+}
+template <bool STREAM>
+__device__ __forceinline__ double2 cld(const float2 &v) {
+  if constexpr (STREAM) {
+    const tr_f2 w = __builtin_nontemporal_load(reinterpret_cast<const tr_f2 *>(&v));
+    return make_double2(w.x, w.y);
+  } else {
+    return make_double2(v.x, v.y);
+  }
+}
+template <bool STREAM>
+__device__ __forceinline__ void cst(double2 &d, double2 v) {
+  if constexpr (STREAM) {
+    tr_d2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<tr_d2 *>(&d));
+  } else {
+    d = v;
+  }
+}
+template <bool STREAM>
+__device__ __forceinline__ void cst(float2 &d, double2 v) {
+  if constexpr (STREAM) {
+    tr_f2 w = {(float)v.x, (float)v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<tr_f2 *>(&d));
+  } else {
+    d = make_float2((float)v.x, (float)v.y);
+  }
+}
 
 // LDS of the complex-line kernel in doubles: the three coefficient rows
 // (padded to 16 bytes) and a, c, f (f complex) of 2P reduced rows x LC lines,
@@ -369,7 +404,7 @@ constexpr size_t tri_cplx_lds_doubles(int nyh, int P, int LC) {
 // workgroups hold at most 512 threads
 constexpr int tri_cplx_cap(int mm) { return mm >= 12 ? 512 : 1024; }
 
-template <int MM, int LC, typename S>
+template <int MM, int LC, typename S, bool STREAM>
 __global__ void __launch_bounds__(tri_cplx_cap(MM)) k_tri_poisson_cplx(S *__restrict__ spec, const double *__restrict__ met,
                                                           int nyh, const double *__restrict__ wnx,
                                                           const double *__restrict__ wnz, int ny, int pitch, int h,
@@ -393,7 +428,7 @@ __global__ void __launch_bounds__(tri_cplx_cap(MM)) k_tri_poisson_cplx(S *__rest
   double2 f[MM];
 #pragma unroll
   for (int r = 0; r < MM; ++r) {
-    const double2 v = cld(col[(long long)min(s + r, ny - 1) * pitch]);
+    const double2 v = cld<STREAM>(col[(long long)min(s + r, ny - 1) * pitch]);
     const double m = (valid && s + r < ny) ? 1.0 : 0.0;
     f[r] = make_double2(v.x * m, v.y * m);
   }
@@ -415,7 +450,7 @@ __global__ void __launch_bounds__(tri_cplx_cap(MM)) k_tri_poisson_cplx(S *__rest
   if (valid) {
 #pragma unroll
     for (int r = 0; r < MM; ++r)
-      if (s + r < ny) cst(col[(long long)(s + r) * pitch], f[r]);
+      if (s + r < ny) cst<STREAM>(col[(long long)(s + r) * pitch], f[r]);
   }
 }
 
@@ -482,14 +517,11 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
   const int s = p * MM;
   const long long col = (long long)(valid ? i : 0) + (long long)nx * ((long long)(ny + 2) * k);
   const double sh = J.shift ? J.shift[0] : 0.0;
-  // unconditional loads (padding rows read row ny), masked by a multiply; u
-  // is read here too, so the final update waits on no load
-  double f[MM], u0[MM];
+  // unconditional loads (padding rows read row ny), masked by a multiply
+  double f[MM];
 #pragma unroll
   for (int r = 0; r < MM; ++r) {
-    const long long o = col + (long long)nx * (min(s + r, ny - 1) + 1);
-    const double v = rhs[o];
-    u0[r] = u[o];
+    const double v = rhs[col + (long long)nx * (min(s + r, ny - 1) + 1)];
     f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   }
   if (kind == 1 && s == 0) f[0] = valid ? J.duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
@@ -500,9 +532,14 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
   solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
   chunk_backsub<MM>(f, A, C, xs, xe);
   if (valid) {
+    const bool wall = kind == 1 && J.wall != nullptr;
 #pragma unroll
     for (int r = 0; r < MM; ++r)
-      if (s + r < ny) u[col + (long long)nx * (s + r + 1)] = u0[r] + f[r];
+      if (s + r < ny) {
+        const long long o = col + (long long)nx * (s + r + 1);
+        u[o] = (wall && s + r == 0) ? J.wall[i + (long long)nx * k] : u[o] + f[r];
+      }
+    if (wall && s <= ny - 1 && ny - 1 < s + MM) u[col + (long long)nx * (ny + 1)] = J.top;
   }
 }
 
@@ -612,11 +649,17 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
     const size_t lds = sizeof(double) * tri_cplx_lds_doubles(c->ny + 2, nP, cl);
 #define PTC_CPLX(MM, CL)                                                                                       \
   if (P.tri_mmax == MM && cl == CL) {                                                                          \
-    if (f32)                                                                                                   \
-      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, float2>), grid, dim3(CL * nP), lds, c->stream,            \
+    if (f32 && P.stream)                                                                                       \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, float2, true>), grid, dim3(CL * nP), lds, c->stream,      \
                          (float2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);    \
+    else if (f32)                                                                                              \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, float2, false>), grid, dim3(CL * nP), lds, c->stream,     \
+                         (float2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);    \
+    else if (P.stream)                                                                                         \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, double2, true>), grid, dim3(CL * nP), lds, c->stream,     \
+                         (double2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);   \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, double2>), grid, dim3(CL * nP), lds, c->stream,           \
+      hipLaunchKernelGGL((k_tri_poisson_cplx<MM, CL, double2, false>), grid, dim3(CL * nP), lds, c->stream,    \
                          (double2 *)spec, c->d_met, c->ny + 2, c->d_wnx, c->d_wnz, c->ny, P.pitch, P.h, nP);   \
     PTC_HIP(hipGetLastError());                                                                                \
     return PTC_OK;                                                                                             \
@@ -691,6 +734,8 @@ int launch_tri_poisson(ptc_ctx *c, void *spec) {
 // kind 0 (u/w), 1 (v), 2 (scalar); shift: optional device scalar added to rhs
 CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, double beta, const double *shift) {
   CnJob J{};
+  J.wall = nullptr;
+  J.top = 0.0;
   J.u = u;
   J.rhs = rhs;
   J.duy_ym = duy_ym;

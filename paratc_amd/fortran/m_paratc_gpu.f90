@@ -48,7 +48,10 @@ module m_paratc_gpu
   public :: ptc_clc_u3hat, ptc_clc_scalar, ptc_clc_prsrc, ptc_fluid_vel_update, ptc_pressure_update
   public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
   public :: ptc_clc_mean_value, ptc_poisson_config, ptc_poisson_set, ptc_poisson_get_config
-  public :: ptc_poisson_get_knob
+  public :: ptc_poisson_get_knob, ptc_clc_stat, ptc_get_stat, ptc_visu_field
+  integer(c_int), parameter, public :: PTC_VISU_UX = 0, PTC_VISU_UY = 1, PTC_VISU_UZ = 2, PTC_VISU_PR = 3
+  integer(c_int), parameter, public :: PTC_VISU_WX = 4, PTC_VISU_WY = 5, PTC_VISU_WZ = 6, PTC_VISU_WMAG = 7
+  integer(c_int), parameter, public :: PTC_VISU_SCALAR = 8
   public :: gpu_check, gpu_put, gpu_get, gpu_clcPPE
 
   integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1
@@ -56,6 +59,7 @@ module m_paratc_gpu
   integer(c_int), parameter, public :: PTC_POISSON_CHUNK_ROWS = 1, PTC_POISSON_SLAB_PLANES = 2
   integer(c_int), parameter, public :: PTC_POISSON_TRI_LINES = 3, PTC_POISSON_FAST_ZTILE = 4
   integer(c_int), parameter, public :: PTC_POISSON_TRI_LANES = 5, PTC_POISSON_TRI_COMPLEX = 6
+  integer(c_int), parameter, public :: PTC_POISSON_STREAMING = 7
 
   interface
     integer(c_int) function ptc_create(grid, prm, yp, ctx) bind(C, name="ptc_create")
@@ -237,6 +241,27 @@ module m_paratc_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: knob
       integer(c_int), intent(out) :: val
+    end function
+    integer(c_int) function ptc_clc_stat(ctx, ux, uy, uz, pressure, scalar) bind(C, name="ptc_clc_stat")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz, pressure, scalar
+    end function
+    integer(c_int) function ptc_get_stat(ctx, sumstat, nstat, nfstime, prgradsum, reset) bind(C, name="ptc_get_stat")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sumstat(*)
+      integer(c_int), intent(out) :: nstat, nfstime
+      real(c_double), intent(out) :: prgradsum
+      integer(c_int), value :: reset
+    end function
+    integer(c_int) function ptc_visu_field(ctx, which, iskip, jskip, kskip, real_bytes, host_out, count) &
+        bind(C, name="ptc_visu_field")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which, iskip, jskip, kskip, real_bytes
+      type(c_ptr), value :: host_out
+      integer(c_long_long), value :: count
     end function
     integer(c_int) function ptc_poisson_get_config(ctx, out10) bind(C, name="ptc_poisson_get_config")
       import :: c_int, c_ptr

@@ -138,6 +138,9 @@ class Solver:
     def poisson_tri_complex(self, lines: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_TRI_COMPLEX, lines), "ptc_poisson_set")
 
+    def poisson_streaming(self, mode: int = 0):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_STREAMING, mode), "ptc_poisson_set")
+
     def poisson_get_knob(self, knob: int) -> int:
         v = C.c_int()
         check(self.L.ptc_poisson_get_knob(self.h, knob, C.byref(v)), "ptc_poisson_get_knob")
@@ -150,7 +153,8 @@ class Solver:
         return {"prec": "fp64" if o[0] == capi.PREC_FP64 else "fp32+refine", "tri": tri,
                 "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2),
                 "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7], "slab_planes": o[8],
-                "tri_lds_lines": o[9], "tri_complex_lines": self.poisson_get_knob(capi.POISSON_TRI_COMPLEX)}
+                "tri_lds_lines": o[9], "tri_complex_lines": self.poisson_get_knob(capi.POISSON_TRI_COMPLEX),
+                "streaming": self.poisson_get_knob(capi.POISSON_STREAMING)}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()
@@ -268,6 +272,16 @@ class Solver:
         check(self.L.ptc_get_stat(self.h, _dp(out), C.byref(n), C.byref(nf), C.byref(pg), 1 if reset else 0),
               "ptc_get_stat")
         return out, nf.value, pg.value
+
+    def visu_field(self, which: int, skip=(1, 1, 1), real_bytes: int = 8) -> np.ndarray:
+        """dump_visu's cell-centre field `which` (capi.VISU_*) at the kept points,
+        shape (ceil(nx/iskip), ceil(ny/jskip), ceil(nz/kskip)) in Fortran order,
+        float64 or float32 (src/m_IOAndVisu.f90:182-385)."""
+        shape = tuple((n + s - 1) // s for n, s in zip((self.nx, self.ny, self.nz), skip))
+        out = np.zeros(shape, dtype=np.float64 if real_bytes == 8 else np.float32, order="F")
+        check(self.L.ptc_visu_field(self.h, which, skip[0], skip[1], skip[2], real_bytes,
+                                    out.ctypes.data_as(C.c_void_p), out.size), "ptc_visu_field")
+        return out
 
     def scalar(self, which: int) -> float:
         d = C.c_double()

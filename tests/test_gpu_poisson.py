@@ -318,6 +318,32 @@ def test_specialised_fft_kernels_match_generic_and_oracle(grid):
     assert rel_err(fast, inner(ref)) <= TOL
 
 
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_FP32_REFINE], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("grid", [(64, 48, 32), (128, 96, 128), (96, 20, 32)], ids=lambda g: "x".join(map(str, g)))
+def test_streaming_passes_match_cached_ones(grid, prec):
+    """Nontemporal loads and stores change only the cache policy: the solve is
+    bit-identical with streaming forced on and off, and matches the oracle."""
+    nx, ny, nz = grid
+    case = CS.case_poisson(nx, ny, nz, 2.0)
+    f = ic.poisson_rhs(case, seed=10)
+    outs = []
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        for mode in (-1, 1):
+            s.poisson_config(prec, capi.TRI_AUTO, 0, 1)
+            s.poisson_streaming(mode)
+            assert s.poisson_get_knob(capi.POISSON_STREAMING) == (1 if mode > 0 else 0)
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+            outs.append(s.download(capi.F_HALO, capi.L_INTERIOR))
+        s.poisson_streaming(0)
+        assert s.poisson_get_knob(capi.POISSON_STREAMING) == 0  # small spectra stay cached
+    assert np.array_equal(outs[0], outs[1])
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    assert rel_err(outs[1], inner(ref)) <= (TOL if prec == capi.PREC_FP64 else 1e-10)
+
+
 # ---- FP32 storage + FP64 defect correction ---------------------------------
 @pytest.mark.parametrize("grid,slab,lines", [((64, 48, 32, 2.0), 0, 0), ((128, 96, 128, 1.8), 0, 0),
                                              ((128, 96, 128, 1.8), 16, 0), ((96, 140, 64, 2.2), 0, 0),
