@@ -210,6 +210,41 @@ def bench_fp32_refine(r, ev, steps=20, max_sweeps=4):
             "bytes_per_point": 48 + 80 * chosen[0]}
 
 
+def bench_spectra(r, ev, samples=5):
+    """clcStat's energy spectra at the Poisson workload's grid: 1-D spectra of
+    all eight quantities and 2-D spectra on four rows, one sample = what the
+    reference computes every ivSpec steps.  Device-resident fields: ux, uz copy
+    the first right-hand side, uy and p the second."""
+    from paratc_amd import capi
+
+    s, case = r["solver"], r["case"]
+    s.copy(capi.F_UX, capi.F_ARR1)
+    s.copy(capi.F_UY, capi.F_WORK1)
+    s.copy(capi.F_UZ, capi.F_ARR1)
+    s.copy(capi.F_PR, capi.F_WORK1)
+    rows = (2, 16, 64, 128)
+    s.init_spectra(True, 8, True, rows)
+    s.clc_spectra()
+    s.synchronize()
+    stream = s.stream()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for _ in range(samples):
+        s.clc_spectra()
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1) / samples
+    _, _, _, n = s.get_spectra(reset=True)
+    assert n == samples + 1
+    # per point: cell-centre ux and uy (8 B read + 8 B write each), six FP64
+    # forward transforms (x and z pass, 8 B read + 8 B write each), and the
+    # partial sums reading nine spectra (8 B per point each)
+    alg = 2 * 16 + 6 * 32 + 9 * 8
+    progress(f"spectra 1024x256x512: {ms:.3f} ms per sample")
+    return {"workload": "clcStat energy spectra, 1-D (8 quantities) + 2-D (5 quantities, 4 rows), "
+                        "1024x256x512 FP64", "ms_per_sample": round(ms, 3),
+            "alg_bytes_per_point": alg, "hbm_gbs_alg": round(alg * case.npts / (ms * 1e-3) / 1e9, 1)}
+
+
 def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
     from oracle.oracle import Oracle
 
@@ -316,6 +351,7 @@ def main():
     ap.add_argument("--no-rk", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage + refinement leg")
+    ap.add_argument("--no-spectra", action="store_true", help="skip the statistics-spectra leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -327,6 +363,7 @@ def main():
     r = bench_poisson(args, ev)
     case = r["case"]
     fp32 = None if args.no_fp32 else bench_fp32_refine(r, ev)
+    spec = None if args.no_spectra else bench_spectra(r, ev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
     ens = None if args.no_ensemble else bench_ensemble()
@@ -367,6 +404,7 @@ def main():
                     "plan": r["plan"],
                     "fp32_refine": fp32},
         "rk_substep": rk,
+        "spectra": spec,
         "ensemble": ens,
     }
     print(json.dumps(line), flush=True)

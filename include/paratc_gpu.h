@@ -272,9 +272,35 @@ int ptc_get_scalar(ptc_ctx *ctx, int which, double *value);
  *  scalar = -1 without a scalar.  ptc_get_stat copies SumStat(NCHASTAT, ny+1)
  *  (column-major, as the reference's array; may be NULL), the statistic count,
  *  nfstime and PrGradsum; reset = 1 zeroes them as after the reference writes
- *  its stats file (:944).  The spectra of clcStat are not part of this. */
+ *  its stats file (:944).  The spectra are the three calls below. */
 int ptc_clc_stat(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar);
 int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double *prgradsum, int reset);
+
+/* ---- clcStat's energy spectra (src/m_FlowCase.f90:430-805 and the helpers
+ *      clcEnergySpectraX/Z, clcLTS_*, clcCospectraX/Z, clcEnergySpectra2D,
+ *      clcCosSpectra2D at :954-1375).
+ *  ptc_init_spectra takes InitStatVar's SpectraOptions / Spectra2DOpt
+ *  (:140-251): clc_1d = clcSpectra1D, jfor_lcs = jForLCS (1..ny), clc_2d =
+ *  clcSpectra2D, iy_spec2d[ny_spec2d] = iySpec2D (1-based rows; 2..ny/2 in a
+ *  channel, 2..ny in a half channel, else PTC_E_ARG as the reference aborts;
+ *  ny_spec2d < 1 turns the 2-D spectra off).  It zeroes the accumulators.
+ *  ptc_clc_spectra adds one sample of the current fields, as clcStat does when
+ *  mod(itime, ivSpec) == 0 (the host keeps that schedule); it needs the
+ *  Poisson plan (InitPoissonSolver), whose forward transforms it uses in FP64
+ *  whatever the plan's storage precision.  scalar = -1 without a scalar.
+ *  ptc_get_spectra copies the sums, not yet divided by nSpectime:
+ *    specx  EnergySpecX(nx/2+1, ny, NE1), specz EnergySpecZ(nz/2+1, ny, NE1),
+ *    NE1 = 9 with a scalar, else 8 (1 ux+uCRF, 2 uy, 3 uz, 4 p, 5 co-spectrum
+ *    of ux+uCRF and the cell-centre uy, 6 of the cell-centre ux and uy,
+ *    7/8 real/imaginary linear coherence with row jForLCS, 9 scalar);
+ *    spec2d EnergySpec2D(nx, ny_spec2d, nz, NE2), NE2 = 6 or 5 (1 ux, 2 uy,
+ *    3 uz, 4 p, 5 cell-centre ux x uy, 6 scalar), values in FFTW's
+ *    half-complex order along x and z;
+ *  all column-major, any pointer may be NULL; reset = 1 zeroes the sums and
+ *  nSpectime as after the reference writes SpecX/SpecZ/Spec2D (:943-947). */
+int ptc_init_spectra(ptc_ctx *ctx, int clc_1d, int jfor_lcs, int clc_2d, int ny_spec2d, const int *iy_spec2d);
+int ptc_clc_spectra(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar);
+int ptc_get_spectra(ptc_ctx *ctx, double *specx, double *specz, double *spec2d, int *nspectime, int reset);
 
 /* dump_visu (src/m_IOAndVisu.f90:182-385): the cell-centre field `which` of
  * the current ux, uy, uz, pressure (and scalar) at every iskip/jskip/kskip-th

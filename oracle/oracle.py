@@ -92,6 +92,10 @@ def lib():
         L.orc_visu_count.restype = C.c_longlong
         L.orc_visu_count.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.orc_channel_iterate.argtypes = [vp, C.c_int] + [dp] * 15 + [dp]
+        L.orc_spectra_count.restype = C.c_int
+        L.orc_spectra_count.argtypes = [vp, C.c_int]
+        L.orc_spectra_1d.argtypes = [vp] + [dp] * 5 + [C.c_int, dp, dp]
+        L.orc_spectra_2d.argtypes = [vp] + [dp] * 5 + [C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
 
@@ -276,6 +280,29 @@ class Oracle:
         sc = scalar if scalar is not None else ux
         self.L.orc_dump_visu(self.h, which, _p(ux), _p(uy), _p(uz), _p(p), _p(sc), iskip, jskip, kskip, _p(out))
         return out
+
+    def spectra_1d(self, ux, uy, uz, p, scalar=None, jlcs: int = 1, specx=None, specz=None):
+        """Adds one sample of clcStat's 1-D spectra to (specx, specz), shapes
+        (nx/2+1, ny, NE1) and (nz/2+1, ny, NE1) in Fortran order; returns them."""
+        ne1 = 9 if self.case.scalar else 8
+        if specx is None:
+            specx = np.zeros((self.nx // 2 + 1, self.ny, ne1), order="F")
+        if specz is None:
+            specz = np.zeros((self.nz // 2 + 1, self.ny, ne1), order="F")
+        sc = scalar if scalar is not None else ux
+        self.L.orc_spectra_1d(self.h, _p(ux), _p(uy), _p(uz), _p(p), _p(sc), jlcs, _p(specx), _p(specz))
+        return specx, specz
+
+    def spectra_2d(self, ux, uy, uz, p, scalar=None, rows=(2,), spec2d=None):
+        """Adds one sample of clcStat's 2-D spectra on the 1-based rows to spec2d,
+        shape (nx, len(rows), nz, NE2) in Fortran order; returns it."""
+        ne2 = 6 if self.case.scalar else 5
+        if spec2d is None:
+            spec2d = np.zeros((self.nx, len(rows), self.nz, ne2), order="F")
+        iy = (C.c_int * len(rows))(*rows)
+        sc = scalar if scalar is not None else ux
+        self.L.orc_spectra_2d(self.h, _p(ux), _p(uy), _p(uz), _p(p), _p(sc), len(rows), iy, _p(spec2d))
+        return spec2d
 
     def ChannelIterate(self, st: "State", choice: int = 1):
         mon = np.zeros(7)

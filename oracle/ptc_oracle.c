@@ -1406,3 +1406,218 @@ void orc_dump_visu(const orc_ctx *c, int which, const double *ux, const double *
       for (int i = 0; i < nxo; i++)
         out[q++] = visu_point(c, which, ux, uy, uz, pr, sc, 1 + i * iskip, 1 + j * jskip, 1 + k * kskip);
 }
+
+/* ---- energy spectra (clcStat, src/m_FlowCase.f90:430-805; helpers :954-1375) --
+ * One rank holds the whole field, so every pencil is the full (nx, ny, nz)
+ * box and the transposes are copies.  Work arrays are interior boxes indexed
+ * B(i, j, k) = i + nx*(j + ny*k), 0-based.  R2HC lines go through the
+ * oracle's own FFTW-semantics transform. */
+#define B(i, j, k) ((size_t)(i) + (size_t)nx * ((size_t)(j) + (size_t)ny * (size_t)(k)))
+
+static double *spec_box(const orc_ctx *c) { return (double *)calloc((size_t)c->nx * c->ny * c->nz, sizeof(double)); }
+
+/* interior copy of a haloed array plus a constant (transpose_y1_to_x1 + arrFFT + uCRF) */
+static void spec_copy(const orc_ctx *c, const double *a, double add, double *o) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) o[B(i, j, k)] = H(c, a, i + 1, j + 1, k + 1) + add;
+}
+
+/* ux at the cell centre + uCRF (:433-444) */
+static void spec_ux_cell(const orc_ctx *c, const double *ux, double *o) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int ic = i + 1, jc = j + 1, kc = k + 1;
+        o[B(i, j, k)] = c->ic1 * H(c, ux, ic - 1, jc, kc) + c->ic2 * H(c, ux, ic, jc, kc) +
+                        c->ic3 * H(c, ux, ic + 1, jc, kc) + c->ic4 * H(c, ux, ic + 2, jc, kc) + c->cfg.ucrf;
+      }
+}
+
+/* uy at the cell centre (:576-587) */
+static void spec_uy_cell(const orc_ctx *c, const double *uy, double *o) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        o[B(i, j, k)] = 0.5 * (H(c, uy, i + 1, j + 1, k + 1) + H(c, uy, i + 1, j + 2, k + 1));
+}
+
+static void spec_fft_x(const orc_ctx *c, double *a) { orc_r2hc(c->nx, c->ny * c->nz, 1, c->nx, a); }
+static void spec_fft_z(const orc_ctx *c, double *a) { orc_r2hc(c->nz, c->nx * c->ny, c->nx * c->ny, 1, a); }
+
+/* The 1-D accumulators of one quantity m (1-based as the reference):
+ * kind 0 = energy / cospectrum (clcEnergySpectraX, clcCospectraX and their z
+ * forms, :954-980, :1040-1066, :1071-1097, :1157-1183), 1 = LCS real part
+ * (:985-1010, :1102-1127), 2 = LCS imaginary part (:1015-1035, :1132-1152).
+ * a and b are the transformed boxes (b = a for an energy spectrum). */
+static void spec_acc_x(const orc_ctx *c, const double *a, const double *b, int kind, int m, double *specx) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = nx / 2, nxhp = nxh + 1;
+  const double norm = 1.0 / ((double)nx * (double)nx * (double)nz);
+  double *egy = (double *)malloc(sizeof(double) * (size_t)nxhp);
+  for (int j = 0; j < ny; j++) {
+    for (int q = 0; q < nxhp; q++) egy[q] = 0.0;
+    for (int k = 0; k < nz; k++) {
+      if (kind == 0) {
+        egy[0] += a[B(0, j, k)] * b[B(0, j, k)];
+        egy[nxhp - 1] += a[B(nxhp - 1, j, k)] * b[B(nxhp - 1, j, k)];
+      } else if (kind == 1) {
+        egy[0] += a[B(0, j, k)];
+        egy[nxhp - 1] += a[B(nxhp - 1, j, k)];
+      } else {
+        egy[0] = 0.0;
+        egy[nxhp - 1] = 0.0;
+      }
+      for (int ic = 2; ic <= nxh; ic++) {
+        const int cn = nx + 2 - ic;  /* 1-based counterpart */
+        const size_t p = B(ic - 1, j, k), q = B(cn - 1, j, k);
+        if (kind == 0) egy[ic - 1] += (a[p] * b[p] + a[q] * b[q]) * 2.0;
+        else if (kind == 1) egy[ic - 1] += a[p] + a[q];
+        else egy[ic - 1] += a[p] - a[q];
+      }
+    }
+    for (int q = 0; q < nxhp; q++) specx[(size_t)q + (size_t)nxhp * ((size_t)j + (size_t)ny * (size_t)(m - 1))] += egy[q] * norm;
+  }
+  free(egy);
+}
+
+static void spec_acc_z(const orc_ctx *c, const double *a, const double *b, int kind, int m, double *specz) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2, nzhp = nzh + 1;
+  const double norm = 1.0 / ((double)nx * (double)nz * (double)nz);
+  double *egy = (double *)malloc(sizeof(double) * (size_t)nzhp);
+  for (int j = 0; j < ny; j++) {
+    for (int q = 0; q < nzhp; q++) egy[q] = 0.0;
+    for (int i = 0; i < nx; i++) {
+      if (kind == 0) {
+        egy[0] += a[B(i, j, 0)] * b[B(i, j, 0)];
+        egy[nzhp - 1] += a[B(i, j, nzhp - 1)] * b[B(i, j, nzhp - 1)];
+      } else if (kind == 1) {
+        egy[0] += a[B(i, j, 0)];
+        egy[nzhp - 1] += a[B(i, j, nzhp - 1)];
+      }
+      for (int kc = 2; kc <= nzh; kc++) {
+        const int cn = nz + 2 - kc;
+        const size_t p = B(i, j, kc - 1), q = B(i, j, cn - 1);
+        if (kind == 0) egy[kc - 1] += (a[p] * b[p] + a[q] * b[q]) * 2.0;
+        else if (kind == 1) egy[kc - 1] += a[p] + a[q];
+        else egy[kc - 1] += a[p] - a[q];
+      }
+    }
+    for (int q = 0; q < nzhp; q++) specz[(size_t)q + (size_t)nzhp * ((size_t)j + (size_t)ny * (size_t)(m - 1))] += egy[q] * norm;
+  }
+  free(egy);
+}
+
+/* LCS products (:470-549 for x, :632-711 for z): the transformed ux box a (or
+ * its real/imaginary swap when imag) times the reference row's values at the
+ * same position; a channel's upper half uses the mirrored row. */
+static void spec_lcs(const orc_ctx *c, const double *a, int jlcs, int dir, int imag, double *o) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  const int jr1 = jlcs - 1, jr2 = c->cfg.flow_type == 1 ? ny - jlcs : jlcs - 1;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++) {
+      const int jr = (c->cfg.flow_type == 1 && j + 1 > ny / 2) ? jr2 : jr1;
+      for (int i = 0; i < nx; i++) {
+        int is = i, ks = k;
+        if (imag && dir == 0 && i > 0) is = nx - i;  /* arrx2(ic) = arrx1(nxc+2-ic) */
+        if (imag && dir == 1 && k > 0) ks = nz - k;  /* arrz2(kc) = arrz1(nzc+2-kc) */
+        o[B(i, j, k)] = a[B(is, j, ks)] * a[B(i, jr, k)];
+      }
+    }
+}
+
+int orc_spectra_count(const orc_ctx *c, int which) {
+  const int ne1 = c->cfg.scalar ? 9 : 8;
+  if (which == 0) return (c->nx / 2 + 1) * c->ny * ne1;
+  if (which == 1) return (c->nz / 2 + 1) * c->ny * ne1;
+  return c->cfg.scalar ? 6 : 5;
+}
+
+void orc_spectra_1d(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                    const double *sc, int jlcs, double *specx, double *specz) {
+  double *a1 = spec_box(c), *a2 = spec_box(c), *t = spec_box(c);
+  for (int dir = 0; dir < 2; dir++) {
+    void (*fft)(const orc_ctx *, double *) = dir == 0 ? spec_fft_x : spec_fft_z;
+    void (*acc)(const orc_ctx *, const double *, const double *, int, int, double *) = dir == 0 ? spec_acc_x : spec_acc_z;
+    double *spec = dir == 0 ? specx : specz;
+    spec_copy(c, ux, c->cfg.ucrf, a1);
+    fft(c, a1);
+    acc(c, a1, a1, 0, 1, spec);
+    spec_lcs(c, a1, jlcs, dir, 0, t);
+    acc(c, t, t, 1, 7, spec);
+    spec_lcs(c, a1, jlcs, dir, 1, t);
+    acc(c, t, t, 2, 8, spec);
+    const double *f3[3] = {uy, uz, pr};
+    for (int q = 0; q < 3; q++) {
+      spec_copy(c, f3[q], 0.0, a2);
+      fft(c, a2);
+      acc(c, a2, a2, 0, 2 + q, spec);
+    }
+    spec_uy_cell(c, uy, a2);
+    fft(c, a2);
+    acc(c, a1, a2, 0, 5, spec);
+    spec_ux_cell(c, ux, a1);
+    fft(c, a1);
+    acc(c, a1, a2, 0, 6, spec);
+    if (c->cfg.scalar) {
+      spec_copy(c, sc, 0.0, a2);
+      fft(c, a2);
+      acc(c, a2, a2, 0, 9, spec);
+    }
+  }
+  free(a1);
+  free(a2);
+  free(t);
+}
+
+/* clcEnergySpectra2D / clcCosSpectra2D (:1231-1375): R2HC along x, then along
+ * z, of the two boxes; the products at the chosen rows (and, in a channel,
+ * their mirrors) go to spec2d(nx, ny2d, nz, NEnergySpec2D). */
+static void spec_acc_2d(const orc_ctx *c, const double *a, const double *b, int m, int ny2d, const int *iy2d,
+                        int is_uy, double rcoe, double *spec2d) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  double norm = 1.0 / ((double)nx * (double)nz);
+  norm = norm * norm;
+  if (c->cfg.flow_type == 1) norm = norm * 0.5;
+  for (int jc = 0; jc < ny2d; jc++) {
+    const int jct = iy2d[jc], jcs = is_uy ? ny + 2 - jct : ny + 1 - jct;
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nx; i++) {
+        const double t1 = a[B(i, jct - 1, k)] * b[B(i, jct - 1, k)];
+        double v = t1;
+        if (c->cfg.flow_type == 1) v = t1 + rcoe * (a[B(i, jcs - 1, k)] * b[B(i, jcs - 1, k)]);
+        spec2d[(size_t)i + (size_t)nx * ((size_t)jc + (size_t)ny2d * ((size_t)k + (size_t)nz * (size_t)(m - 1)))] += norm * v;
+      }
+  }
+}
+
+void orc_spectra_2d(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                    const double *sc, int ny2d, const int *iy2d, double *spec2d) {
+  double *a = spec_box(c), *b = spec_box(c);
+  const double *f[4] = {ux, uy, uz, pr};
+  for (int q = 0; q < 4; q++) {
+    spec_copy(c, f[q], 0.0, a);
+    spec_fft_x(c, a);
+    spec_fft_z(c, a);
+    /* clcEnergySpectra2D: RealTemp1^2 + RealTemp2^2 (:1303-1306) */
+    spec_acc_2d(c, a, a, 1 + q, ny2d, iy2d, q == 1, 1.0, spec2d);
+  }
+  spec_ux_cell(c, ux, a);
+  spec_fft_x(c, a);
+  spec_fft_z(c, a);
+  spec_uy_cell(c, uy, b);
+  spec_fft_x(c, b);
+  spec_fft_z(c, b);
+  spec_acc_2d(c, a, b, 5, ny2d, iy2d, 0, -1.0, spec2d); /* IsSymmetry = .false.: rCoe = -1 (:1359-1372) */
+  if (c->cfg.scalar) {
+    spec_copy(c, sc, 0.0, a);
+    spec_fft_x(c, a);
+    spec_fft_z(c, a);
+    spec_acc_2d(c, a, a, 6, ny2d, iy2d, 0, 1.0, spec2d);
+  }
+  free(a);
+  free(b);
+}
+#undef B

@@ -115,6 +115,21 @@ long long orc_visu_count(const orc_ctx *c, int iskip, int jskip, int kskip);
 void orc_dump_visu(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *pr,
                    const double *sc, int iskip, int jskip, int kskip, double *out);
 
+/* Energy spectra of clcStat (src/m_FlowCase.f90:430-805, helpers :954-1375),
+ * added to the accumulators as the reference does per sample.
+ * orc_spectra_1d: specx(nx/2+1, ny, NE1) and specz(nz/2+1, ny, NE1), column-
+ * major, NE1 = 9 (scalar build) or 8: 1 ux+uCRF, 2 uy, 3 uz, 4 p, 5 co-spectrum
+ * of ux+uCRF and the cell-centre uy, 6 co-spectrum of the cell-centre ux and
+ * uy, 7/8 real/imaginary linear coherence with row jlcs, 9 scalar.
+ * orc_spectra_2d: spec2d(nx, ny2d, nz, NE2) on the rows iy2d (1-based), NE2 =
+ * 6 or 5: 1 ux, 2 uy, 3 uz, 4 p, 5 cell-centre ux x uy, 6 scalar.
+ * orc_spectra_count: 0 -> specx length, 1 -> specz length, 2 -> NE2. */
+int orc_spectra_count(const orc_ctx *c, int which);
+void orc_spectra_1d(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                    const double *sc, int jlcs, double *specx, double *specz);
+void orc_spectra_2d(const orc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
+                    const double *sc, int ny2d, const int *iy2d, double *spec2d);
+
 /* One ChannelIterate time step (m_ChannelSystem.f90:80-145).  Buffers follow
  * the reference's aliasing: arr1 = RealArr1, arr2 = RealArr2, halo = RealHalo,
  * arrc = RealArrC.  mon receives [divmax1 (last substep), divmax2, vmax x3,

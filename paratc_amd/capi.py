@@ -44,7 +44,8 @@ EXPORTS = [
     "ptc_pressure_update", "ptc_set_bc_and_update_halo", "ptc_set_bc_and_update_halo_for_prsrc",
     "ptc_set_bc_and_update_halo_pr", "ptc_set_bc_and_update_halo_scalar", "ptc_update_uy_ym",
     "ptc_calc_max_cfl", "ptc_check_divergence", "ptc_calc_vmax", "ptc_calc_ux_aver",
-    "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar", "ptc_clc_stat", "ptc_get_stat", "ptc_visu_field", "ptc_step_graphs",
+    "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar", "ptc_clc_stat", "ptc_get_stat", "ptc_init_spectra", "ptc_clc_spectra",
+    "ptc_get_spectra", "ptc_visu_field", "ptc_step_graphs",
     "ptc_ensemble_create", "ptc_ensemble_destroy", "ptc_case_footprint", "ptc_ensemble_admit",
     "ptc_ensemble_step", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
     "ptc_ensemble_latency",
@@ -135,6 +136,9 @@ def load(path: str = LIB_PATH):
     L.ptc_step_graphs.argtypes = [vp, ip]
     L.ptc_get_stat.argtypes = [vp, dp, C.POINTER(ip), C.POINTER(ip), dp, ip]
     L.ptc_visu_field.argtypes = [vp, ip, ip, ip, ip, ip, vp, C.c_longlong]
+    L.ptc_init_spectra.argtypes = [vp, ip, ip, ip, ip, C.POINTER(ip)]
+    L.ptc_clc_spectra.argtypes = [vp, ip, ip, ip, ip, ip]
+    L.ptc_get_spectra.argtypes = [vp, dp, dp, dp, C.POINTER(ip), ip]
     sz, ll = C.c_size_t, C.c_longlong
     L.ptc_ensemble_create.argtypes = [sz, C.POINTER(vp)]
     L.ptc_ensemble_destroy.argtypes = [vp]

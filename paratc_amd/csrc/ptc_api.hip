@@ -772,6 +772,30 @@ int ptc_get_stat(ptc_ctx *ctx, double *sumstat, int *nstat, int *nfstime, double
   return read_stat(ctx, sumstat, prgradsum, reset);
 }
 
+int ptc_init_spectra(ptc_ctx *ctx, int clc_1d, int jfor_lcs, int clc_2d, int ny_spec2d, const int *iy_spec2d) {
+  PTC_CTX(ctx);
+  return spectra_init(ctx, clc_1d, jfor_lcs, clc_2d, ny_spec2d, iy_spec2d);
+}
+
+int ptc_clc_spectra(ptc_ctx *ctx, int ux, int uy, int uz, int pressure, int scalar) {
+  PTC_CTX(ctx);
+  double *a, *b, *d, *p, *s = nullptr;
+  PTC_TRY(field(ctx, ux, &a));
+  PTC_TRY(field(ctx, uy, &b));
+  PTC_TRY(field(ctx, uz, &d));
+  PTC_TRY(field(ctx, pressure, &p));
+  if (ctx->prm.scalar) {
+    PTC_CHECK(scalar >= 0, PTC_E_ARG, "clcStat spectra: the scalar build needs the scalar field");
+    PTC_TRY(field(ctx, scalar, &s));
+  }
+  return spectra_sample(ctx, a, b, d, p, s);
+}
+
+int ptc_get_spectra(ptc_ctx *ctx, double *specx, double *specz, double *spec2d, int *nspectime, int reset) {
+  PTC_CTX(ctx);
+  return spectra_read(ctx, specx, specz, spec2d, nspectime, reset);
+}
+
 int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
                    long long count) {
   PTC_CTX(ctx);

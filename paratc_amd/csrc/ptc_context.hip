@@ -340,6 +340,7 @@ void destroy(ptc_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   poisson_free_plan(c);
+  spectra_free(c);
   for (auto &p : c->d_field)
     if (p) (void)hipFree(p);
   if (c->d_met) (void)hipFree(c->d_met);

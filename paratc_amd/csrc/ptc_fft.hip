@@ -382,8 +382,8 @@ static int launch_x_fwd_t(ptc_ctx *c, const double *src, double2 *spec) {
   return PTC_OK;
 }
 
-int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec_) {
-  if (c->pp.x_fast) return fft2_x_fwd(c, src, spec_, 0, c->ny);
+int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec_, bool f64) {
+  if (c->pp.x_fast) return fft2_x_fwd(c, src, spec_, 0, c->ny, f64);
   double2 *spec = static_cast<double2 *>(spec_);  // generic kernels: FP64 spectra only
   return stage_fits(c->pp.fx, c->pp.x_lines, c->pp.x_threads, 16) ? launch_x_fwd_t<16>(c, src, spec)
                                                                   : launch_x_fwd_t<32>(c, src, spec);
@@ -426,8 +426,8 @@ static int launch_z_t(ptc_ctx *c, double2 *spec) {
   return PTC_OK;
 }
 
-int launch_fft_z(ptc_ctx *c, void *spec_, bool inverse) {
-  if (c->pp.z_fast) return fft2_z(c, spec_, inverse, 0, c->ny);
+int launch_fft_z(ptc_ctx *c, void *spec_, bool inverse, bool f64) {
+  if (c->pp.z_fast) return fft2_z(c, spec_, inverse, 0, c->ny, f64);
   double2 *spec = static_cast<double2 *>(spec_);
   const bool small = stage_fits(c->pp.fz, c->pp.ztile, c->pp.z_threads, 16);
   if (inverse) return small ? launch_z_t<16, true>(c, spec) : launch_z_t<32, true>(c, spec);

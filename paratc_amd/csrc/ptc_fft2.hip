@@ -520,8 +520,8 @@ static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj)
 }
 
 template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
-  const bool f32 = c->pp.prec == PTC_PREC_FP32_REFINE;
+static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64) {
+  const bool f32 = !f64 && c->pp.prec == PTC_PREC_FP32_REFINE;
   if (f32 && c->pp.stream) x_fwd_launch<N, float2, true>(c, src, (float2 *)spec, j0, nj);
   else if (f32) x_fwd_launch<N, float2, false>(c, src, (float2 *)spec, j0, nj);
   else if (c->pp.stream) x_fwd_launch<N, double2, true>(c, src, (double2 *)spec, j0, nj);
@@ -562,12 +562,12 @@ static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int 
 }
 
 template <int N, int T>
-static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj) {
+static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64) {
   constexpr int G = f2_threads<N>();
   dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)nj);
   const auto *tw = c->pp.fz.tw;
   const int ny = c->ny, pitch = c->pp.pitch;
-  if (c->pp.prec == PTC_PREC_FP32_REFINE) {
+  if (!f64 && c->pp.prec == PTC_PREC_FP32_REFINE) {
     if (inverse)
       hipLaunchKernelGGL((k2_fft_z<N, T, true, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny, pitch,
                          j0);
@@ -598,9 +598,9 @@ bool fft2_has_x(int h) {
   return false;
 }
 
-int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj) {
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj);
+#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj, f64);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;
@@ -634,10 +634,10 @@ bool fft2_has_z(int nz, int req, int *tile) {
   return false;
 }
 
-int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj) {
+int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64) {
   const int T = c->pp.ztile;
 #define PTC_CASE(N, TT) \
-  if (c->nz == N && T == TT) return z_t<N, TT>(c, spec, inverse, j0, nj);
+  if (c->nz == N && T == TT) return z_t<N, TT>(c, spec, inverse, j0, nj, f64);
   PTC_Z_SIZES(PTC_CASE)
   PTC_Z_WIDE(PTC_CASE)
 #undef PTC_CASE

@@ -121,6 +121,23 @@ struct PoissonPlan {
   int prof_used = 0;                // profiled solves since the last read
 };
 
+// Energy spectra of clcStat (ptc_spectra.hip): options, accumulators
+// EnergySpecX(nx/2+1, ny, NE1), EnergySpecZ(nz/2+1, ny, NE1) and
+// EnergySpec2D(nx, ny2d, nz, NE2) in the reference's column-major order, and
+// the work buffers of one sample.
+struct SpectraState {
+  bool ready = false, on1d = false, on2d = false;
+  int jlcs = 1, ny2d = 0, ne1 = 8, ne2 = 5;
+  int nspectime = 0;
+  std::vector<int> iy2d;  // 1-based rows of the 2-D spectra
+  int *d_iy2d = nullptr;
+  double *d_x = nullptr, *d_z = nullptr, *d_2d = nullptr;
+  int pitch = 0, nch = 0;                      // spectral row pitch and kz chunks of the work buffers
+  double2 *d_fa = nullptr, *d_fb = nullptr;    // FP64 2-D spectra pitch * ny * nz
+  double *d_cell = nullptr;                    // cell-centre field, field layout
+  double2 *d_px = nullptr, *d_pz = nullptr;    // partial sums
+};
+
 }  // namespace ptc
 
 struct ptc_ctx {
@@ -155,6 +172,7 @@ struct ptc_ctx {
   void *d_visu = nullptr;    // dump_visu staging (visu_bytes)
   size_t visu_bytes = 0;
   int nfstime = 0;           // samples in it
+  ptc::SpectraState spec;
   bool step_graphs = true;               // ptc_channel_iterate replays a captured step when dt is fixed
   hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
   hipGraphExec_t step_exec = nullptr;
@@ -185,6 +203,12 @@ int fill_device(ptc_ctx *c, double *a, long long n, double v);
 int launch_stat(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *pr,
                 const double *sc);
 int read_stat(ptc_ctx *c, double *out, double *prgradsum, int reset);
+// energy spectra (ptc_spectra.hip)
+int spectra_init(ptc_ctx *c, int clc1d, int jlcs, int clc2d, int ny2d, const int *iy2d);
+int spectra_sample(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *p,
+                   const double *sc);
+int spectra_read(ptc_ctx *c, double *sx, double *sz, double *s2, int *nspectime, int reset);
+void spectra_free(ptc_ctx *c);
 // allocate everything a step touches, so the step can be captured in a graph
 int prepare_step(ptc_ctx *c);
 int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec);
@@ -194,18 +218,19 @@ int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *r
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 
 // FFT launchers (ptc_fft.hip)
-int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec);
+int launch_fft_x_fwd(ptc_ctx *c, const double *src, void *spec, bool f64 = false);
 int launch_fft_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, bool acc, bool ghost);
-int launch_fft_z(ptc_ctx *c, void *spec, bool inverse);
+int launch_fft_z(ptc_ctx *c, void *spec, bool inverse, bool f64 = false);
 int fft_make_len(int n, FftLen *L);
 bool fft2_has_x(int h);
 bool fft2_has_z(int nz, int req, int *tile);
 // specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum;
 // the spectrum holds double2 or, in the FP32-storage form, float2 values;
 // acc: the x-inverse adds its result to dst
-int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj);
+// f64: FP64 spectra whatever the plan's storage precision (the statistics spectra)
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64 = false);
 int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost);
-int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj);
+int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64 = false);
 
 // tridiagonal launchers (ptc_tridiag.hip)
 int launch_tri_poisson(ptc_ctx *c, void *spec);

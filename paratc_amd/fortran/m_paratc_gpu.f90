@@ -49,6 +49,7 @@ module m_paratc_gpu
   public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
   public :: ptc_clc_mean_value, ptc_poisson_config, ptc_poisson_set, ptc_poisson_get_config
   public :: ptc_poisson_get_knob, ptc_clc_stat, ptc_get_stat, ptc_visu_field
+  public :: ptc_init_spectra, ptc_clc_spectra, ptc_get_spectra
   integer(c_int), parameter, public :: PTC_VISU_UX = 0, PTC_VISU_UY = 1, PTC_VISU_UZ = 2, PTC_VISU_PR = 3
   integer(c_int), parameter, public :: PTC_VISU_WX = 4, PTC_VISU_WY = 5, PTC_VISU_WZ = 6, PTC_VISU_WMAG = 7
   integer(c_int), parameter, public :: PTC_VISU_SCALAR = 8
@@ -253,6 +254,28 @@ module m_paratc_gpu
       real(c_double), intent(out) :: sumstat(*)
       integer(c_int), intent(out) :: nstat, nfstime
       real(c_double), intent(out) :: prgradsum
+      integer(c_int), value :: reset
+    end function
+    ! InitStatVar's SpectraOptions / Spectra2DOpt and clcStat's spectra
+    ! (src/m_FlowCase.f90:140-251, 430-805); the sums come back column-major as
+    ! EnergySpecX/Z and EnergySpec2D, not yet divided by nSpectime
+    integer(c_int) function ptc_init_spectra(ctx, clc_1d, jfor_lcs, clc_2d, ny_spec2d, iy_spec2d) &
+        bind(C, name="ptc_init_spectra")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: clc_1d, jfor_lcs, clc_2d, ny_spec2d
+      integer(c_int), intent(in) :: iy_spec2d(*)
+    end function
+    integer(c_int) function ptc_clc_spectra(ctx, ux, uy, uz, pressure, scalar) bind(C, name="ptc_clc_spectra")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ux, uy, uz, pressure, scalar
+    end function
+    integer(c_int) function ptc_get_spectra(ctx, specx, specz, spec2d, nspectime, reset) bind(C, name="ptc_get_spectra")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: specx, specz, spec2d
+      integer(c_int), intent(out) :: nspectime
       integer(c_int), value :: reset
     end function
     integer(c_int) function ptc_visu_field(ctx, which, iskip, jskip, kskip, real_bytes, host_out, count) &

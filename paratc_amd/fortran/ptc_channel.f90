@@ -7,25 +7,32 @@
 ! result).  The restart files are raw native FP64 with no header, each field
 ! the full nx*ny*nz interior in Fortran order: ux(+uCRF), uy, uz, p
 ! (write_restart / read_restart, src/m_IOAndVisu.f90:408-512).
+! With spec_out set it also writes one sample of clcStat's 1-D energy spectra
+! of the final fields, as the reference's SpecX / SpecZ files
+! (src/m_FlowCase.f90:430-805, 841-885), to spec_out//'SpecX' and 'SpecZ'.
 program ptc_channel
   use iso_c_binding
   use m_paratc_gpu
   implicit none
   integer :: nx, ny, nz, nsteps, flow_type, is_ux_const, ischeme
   real(8) :: xlx, yly, zlz, xnu, ubulk, dtmax
-  character(len=256) :: yp_file, restart_in, restart_out, ctl
+  character(len=256) :: yp_file, restart_in, restart_out, ctl, spec_out
+  integer :: jforlcs
   namelist /ptc_run/ nx, ny, nz, xlx, yly, zlz, xnu, ubulk, dtmax, nsteps, flow_type, &
-                     is_ux_const, ischeme, yp_file, restart_in, restart_out
+                     is_ux_const, ischeme, yp_file, restart_in, restart_out, spec_out, jforlcs
   type(ptc_grid_t) :: grid
   type(ptc_params_t) :: prm
   type(c_ptr) :: ctx
   real(c_double), allocatable :: yp(:), ux(:,:,:), uy(:,:,:), uz(:,:,:), pr(:,:,:)
   real(c_double) :: mon(7), umean
+  real(c_double), allocatable, target :: specx(:,:,:), specz(:,:,:)
+  integer(c_int) :: nspec, norows(1)
   integer :: u, ios
   integer(8) :: nbytes
 
   call get_command_argument(1, ctl)
   flow_type = 1; is_ux_const = 1; ischeme = 3
+  spec_out = ''; jforlcs = 1
   open(newunit=u, file=trim(ctl), status='old', action='read', iostat=ios)
   if (ios /= 0) then
     write(*,*) 'Cannot open file: ', trim(ctl)
@@ -80,6 +87,23 @@ program ptc_channel
   call gpu_get(ctx, PTC_F_UY, uy, PTC_L_INTERIOR)
   call gpu_get(ctx, PTC_F_UZ, uz, PTC_L_INTERIOR)
   call gpu_get(ctx, PTC_F_PR, pr, PTC_L_INTERIOR)
+  if (len_trim(spec_out) > 0) then
+    allocate(specx(nx/2 + 1, ny, 8), specz(nz/2 + 1, ny, 8))
+    norows = 0
+    call gpu_check(ptc_init_spectra(ctx, 1, jforlcs, 0, 0, norows), 'InitStatVar spectra')
+    call gpu_check(ptc_clc_spectra(ctx, PTC_F_UX, PTC_F_UY, PTC_F_UZ, PTC_F_PR, -1), 'clcStat spectra')
+    call gpu_check(ptc_get_spectra(ctx, c_loc(specx), c_loc(specz), c_null_ptr, nspec, 1), 'ptc_get_spectra')
+    specx = specx / real(nspec, 8)
+    specz = specz / real(nspec, 8)
+    open(newunit=u, file=trim(spec_out)//'SpecX', access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u) specx
+    close(u)
+    open(newunit=u, file=trim(spec_out)//'SpecZ', access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u) specz
+    close(u)
+  endif
   call gpu_check(ptc_destroy(ctx), 'ptc_destroy')
   open(newunit=u, file=trim(restart_out), access='stream', form='unformatted', status='replace', action='write')
   write(u) ux, uy, uz, pr

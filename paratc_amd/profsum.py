@@ -27,6 +27,7 @@ TRAFFIC_HEADER = ("| kernel @ grid (threads) | dispatches | FETCH_SIZE KiB | WRI
 
 def short_name(name: str) -> str:
     """Kernel name without the argument list (template arguments kept)."""
+    name = name.replace("(anonymous namespace)::", "")
     depth = 0
     for q, ch in enumerate(name):
         if ch == "<":

@@ -273,6 +273,35 @@ class Solver:
               "ptc_get_stat")
         return out, nf.value, pg.value
 
+    def init_spectra(self, clc_1d: bool = True, jfor_lcs: int = 1, clc_2d: bool = False, iy_spec2d=()):
+        """SpectraOptions / Spectra2DOpt of InitStatVar (src/m_FlowCase.f90:140-251);
+        zeroes the spectra accumulators."""
+        rows = list(iy_spec2d)
+        arr = (C.c_int * max(1, len(rows)))(*rows)
+        check(self.L.ptc_init_spectra(self.h, 1 if clc_1d else 0, jfor_lcs, 1 if clc_2d else 0, len(rows), arr),
+              "InitStatVar spectra")
+        self._spectra = (bool(clc_1d), bool(clc_2d) and len(rows) > 0, len(rows))
+
+    def clc_spectra(self, ux=capi.F_UX, uy=capi.F_UY, uz=capi.F_UZ, pressure=capi.F_PR, scalar=None):
+        """One sample of clcStat's energy spectra (src/m_FlowCase.f90:430-805)."""
+        if scalar is None:
+            scalar = capi.F_SCALAR if self.case.scalar else -1
+        check(self.L.ptc_clc_spectra(self.h, ux, uy, uz, pressure, scalar), "clcStat spectra")
+
+    def get_spectra(self, reset: bool = False):
+        """(EnergySpecX, EnergySpecZ, EnergySpec2D, nSpectime): the accumulated sums,
+        shapes (nx/2+1, ny, NE1), (nz/2+1, ny, NE1), (nx, ny2d, nz, NE2) in Fortran
+        order (None where that part is off)."""
+        on1, on2, ny2 = getattr(self, "_spectra", (False, False, 0))
+        ne1, ne2 = (9, 6) if self.case.scalar else (8, 5)
+        sx = np.zeros((self.nx // 2 + 1, self.ny, ne1), order="F") if on1 else None
+        sz = np.zeros((self.nz // 2 + 1, self.ny, ne1), order="F") if on1 else None
+        s2 = np.zeros((self.nx, ny2, self.nz, ne2), order="F") if on2 else None
+        n = C.c_int()
+        check(self.L.ptc_get_spectra(self.h, _dp(sx) if on1 else None, _dp(sz) if on1 else None,
+                                     _dp(s2) if on2 else None, C.byref(n), 1 if reset else 0), "ptc_get_spectra")
+        return sx, sz, s2, n.value
+
     def visu_field(self, which: int, skip=(1, 1, 1), real_bytes: int = 8) -> np.ndarray:
         """dump_visu's cell-centre field `which` (capi.VISU_*) at the kept points,
         shape (ceil(nx/iskip), ceil(ny/jskip), ceil(nz/kskip)) in Fortran order,

@@ -31,7 +31,8 @@ def test_fortran_host_matches_oracle(tmp_path):
         f" nx={case.nx}, ny={case.ny}, nz={case.nz}, xlx={case.xlx!r}, yly={case.yly!r}, zlz={case.zlz!r},\n"
         f" xnu={case.xnu!r}, ubulk={case.ubulk!r}, dtmax={case.dtmax!r}, nsteps={nsteps},\n"
         f" flow_type=1, is_ux_const=1, ischeme=3,\n"
-        f" yp_file='{tmp_path / 'yp.bin'}', restart_in='{rin}', restart_out='{rout}'\n/\n")
+        f" yp_file='{tmp_path / 'yp.bin'}', restart_in='{rin}', restart_out='{rout}',\n"
+        f" spec_out='{tmp_path / 'final_'}', jforlcs=3\n/\n")
     r = subprocess.run([EXE, str(tmp_path / "run.nml")], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
@@ -47,3 +48,10 @@ def test_fortran_host_matches_oracle(tmp_path):
     for name, ref in (("ux", st.ux), ("uy", st.uy), ("uz", st.uz)):
         assert rel_err(got[name], inner(ref)) <= 1e-12, name
     assert rel_err(got["p"], inner(st.p)) <= 1e-10
+    # clcStat's 1-D spectra of the final fields, written by the host
+    sx, sz = orc.spectra_1d(st.ux, st.uy, st.uz, st.p, jlcs=3)
+    fx = np.fromfile(tmp_path / "final_SpecX").reshape(sx.shape, order="F")
+    fz = np.fromfile(tmp_path / "final_SpecZ").reshape(sz.shape, order="F")
+    for m in range(8):
+        assert rel_err(fx[..., m], sx[..., m]) <= 1e-9, m
+        assert rel_err(fz[..., m], sz[..., m]) <= 1e-9, m
