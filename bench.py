@@ -245,6 +245,75 @@ def bench_spectra(r, ev, samples=5):
             "alg_bytes_per_point": alg, "hbm_gbs_alg": round(alg * case.npts / (ms * 1e-3) / 1e9, 1)}
 
 
+def bench_large(ev, solves=10, rk_steps=5):
+    """The large single case: the Poisson solve at 9216x140x1400 (gamma 2.2,
+    right-hand side as the 1024x256x512 one with seed 4), FP64 and FP32
+    storage with one FP64 correction sweep; the RK3 substep of the gravity
+    current at 4608x140x700 (gamma 1.6, Re 5000, Ri 1, ws 0.02, dt 1e-3)."""
+    from paratc_amd import capi, case as CS, ic
+    from paratc_amd.solver import Solver
+
+    case = CS.case_poisson(9216, 140, 1400, 2.2)
+    n = case.npts
+    f = ic.poisson_rhs(case, seed=4)
+    progress("9216x140x1400 right-hand side generated")
+    s = Solver(case)
+    s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+    s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+    del f
+    plan64 = s.poisson_get_config()
+    stream = s.stream()
+
+    def timed():
+        s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        s.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream)
+        for _ in range(solves):
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+        ev.record(e1, stream)
+        return ev.elapsed_ms(e0, e1) / solves, s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+
+    ms64, r64 = timed()
+    progress(f"9216x140x1400 FP64: {ms64:.3f} ms per solve")
+    s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 1)
+    ms32, r32 = timed()
+    progress(f"9216x140x1400 FP32 + 1 sweep: {ms32:.3f} ms per solve")
+    s.close()
+    out = {"poisson_9216x140x1400": {
+        "fp64_ms_per_solve": round(ms64, 3), "fp64_mpts_per_s": round(n / (ms64 * 1e-3) / 1e6, 1),
+        "fp64_hbm_gbs_alg": round(80.0 * n / (ms64 * 1e-3) / 1e9, 1),
+        "fp64_frac_of_peak": round(80.0 * n / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fp32_refine_1_sweep_ms_per_solve": round(ms32, 3), "fp32_refine_speed_vs_fp64": round(ms64 / ms32, 3),
+        "residual_fp64": r64, "residual_fp32_refine": r32, "plan_fp64": plan64}}
+    case = CS.case_gravity_current(4608, 140, 700, gamma=1.6)
+    ux, uy, uz, sc = ic.gravity_current_fields(case, seed=2, amp=1e-3)
+    s = Solver(case)
+    for fid, a in ((capi.F_UX, ux), (capi.F_UY, uy), (capi.F_UZ, uz), (capi.F_SCALAR, sc)):
+        s.upload(fid, a)
+    del ux, uy, uz, sc
+    s.SetBC_and_UpdateHalo(capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_UY_YM)
+    s.SetBC_and_UpdateHalo_pr(capi.F_PR)
+    s.SetBC_and_UpdateHalo_scalar(capi.F_SCALAR)
+    s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+    s.L.ptc_channel_iterate(s.h, 1, None)
+    s.synchronize()
+    stream = s.stream()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    rc = s.L.ptc_channel_iterate(s.h, rk_steps, None)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1) / (3 * rk_steps)
+    assert rc == 0, s.L.ptc_last_error()
+    mon = s.ChannelIterate(1)
+    s.close()
+    progress(f"4608x140x700 gravity current: {ms:.3f} ms per RK substep")
+    out["rk_substep_4608x140x700"] = {
+        "workload": "gravity current 4608x140x700, RK3, dt 1e-3, FP64, scalar", "ms_per_substep": round(ms, 3),
+        "mpts_per_s": round(case.npts / (ms * 1e-3) / 1e6, 1), "divmax_after": float(mon[1])}
+    return out
+
+
 def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
     from oracle.oracle import Oracle
 
@@ -352,6 +421,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage + refinement leg")
     ap.add_argument("--no-spectra", action="store_true", help="skip the statistics-spectra leg")
+    ap.add_argument("--large", action="store_true",
+                    help="add the 9216x140x1400 Poisson and 4608x140x700 RK legs (several minutes)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -367,6 +438,7 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
     ens = None if args.no_ensemble else bench_ensemble()
+    large = bench_large(ev) if args.large else None
     n = case.npts
     kernel = stage_kernels(r["plan"], case.nx, case.nz)[r["dom"]]
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
@@ -406,6 +478,7 @@ def main():
         "rk_substep": rk,
         "spectra": spec,
         "ensemble": ens,
+        "large": large,
     }
     print(json.dumps(line), flush=True)
 

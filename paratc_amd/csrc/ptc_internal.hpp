@@ -136,6 +136,7 @@ struct SpectraState {
   double2 *d_fa = nullptr, *d_fb = nullptr;    // FP64 2-D spectra pitch * ny * nz
   double *d_cell = nullptr;                    // cell-centre field, field layout
   double2 *d_px = nullptr, *d_pz = nullptr;    // partial sums
+  double2 *d_hx = nullptr;                     // symbol of the cell-centre interpolation of ux
 };
 
 }  // namespace ptc

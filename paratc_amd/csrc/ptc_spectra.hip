@@ -19,10 +19,15 @@
 //     (F(kx, kz) - conj F(kx, -kz)) / 2i.
 // The linear-coherence spectra are the real and imaginary parts of the same
 // sums with b the reference row.  The reference's uCRF shift of ux only moves
-// F(0, 0) by nx nz uCRF, which the partial sums add.
+// F(0, 0) by nx nz uCRF, which the partial sums add.  The cell-centre ux
+// (the 4-point interpolation ic1 ux(i-1) + ic2 ux(i) + ic3 ux(i+1) + ic4 ux(i+2)
+// + uCRF) is not transformed separately: its spectrum is H(kx) F_ux(kx, kz)
+// with H = ic1 w + ic2 + ic3 conj(w) + ic4 conj(w)^2, w = exp(-2 pi i kx / nx),
+// plus the same shift.
 #include "ptc_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 
 namespace ptc {
 
@@ -31,90 +36,107 @@ namespace {
 constexpr int kSpecThreads = 256;  // partial-sum workgroup
 constexpr int kSpecRows = 16;      // kz rows per partial-sum workgroup
 
-// ---- cell-centre fields: ux (+uCRF) by the 4-point interpolation (:433-444),
-// uy by the two-face mean (:576-587), written into rows 1..ny of a field
-__global__ void __launch_bounds__(64) k_spec_cell(Geo g, int mode, const double *__restrict__ u, double ucrf,
-                                                  double *__restrict__ out) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y, k = blockIdx.z;
+// ---- the cell-centre uy (:576-587): the two-face mean into rows 1..ny of a field
+__global__ void __launch_bounds__(256) k_spec_cell(Geo g, const double *__restrict__ u, double *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = 1 + blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
-  const long long row = (long long)g.nx * j + g.plane * k;
-  if (mode == 0) {
-    const int nx = g.nx;
-    const int im = i > 0 ? i - 1 : nx - 1, ip = i + 1 < nx ? i + 1 : i + 1 - nx, iu = i + 2 < nx ? i + 2 : i + 2 - nx;
-    out[row + i] = g.ic1 * u[row + im] + g.ic2 * u[row + i] + g.ic3 * u[row + ip] + g.ic4 * u[row + iu] + ucrf;
-  } else {
-    out[row + i] = 0.5 * (u[row + i] + u[row + g.nx + i]);
-  }
+  const long long o = (long long)g.nx * j + g.plane * k + i;
+  out[o] = 0.5 * (u[o] + u[o + g.nx]);
 }
 
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
 // One pair of spectra to multiply, P = Fa conj Fb.  b's row is the same row,
-// or (lcs) the reference row: jr1 below row jsplit, jr2 from it on.
+// or (lcs) the reference row: jr1 below row jsplit, jr2 from it on.  ha: a is
+// first multiplied by hx[kx] (the cell-centre interpolation's symbol).
 struct SpecPair {
   const double2 *a, *b;
-  double sa, sb;  // added to Re F(0, 0) of a and b
+  double sa, sb;  // added to Re F(0, 0) of a and b (after the multiplication)
   int lcs, jr1, jr2, jsplit;
+  const double2 *ha;
 };
 
 // Partial sums of one row j and kSpecRows kz rows (grid: x = kz chunk, y = j):
 //   px[(j * nch + chunk) * (h+1) + kx] = sum over the chunk's kz of P(kx, kz)
 //   pz[2 * (j * nz + kz)]     = sum over kx = 0..h of P(kx, kz)
 //   pz[2 * (j * nz + kz) + 1] = sum over kx = 1..h-1 of P(kx, kz)
-// Thread t owns kx = t + 256 s, s < NS.
+// Thread t owns kx = t + 256 s, s < NS.  Rows go in batches of spec_batch
+// (fewer for more kx per thread, to stay within the registers): all their
+// loads are issued before the products and the wavefront sums.
+template <int NS>
+constexpr int spec_batch() { return NS == 1 ? 4 : NS <= 3 ? 2 : 1; }
+
 template <int NS, bool SAME>
 __global__ void __launch_bounds__(kSpecThreads) k_spec_part(SpecPair P, int ny, int nz, int h, int pitch,
                                                             double2 *__restrict__ px, double2 *__restrict__ pz) {
-  constexpr int NW = kSpecThreads / 64;
+  constexpr int NW = kSpecThreads / 64, RB = spec_batch<NS>();
   __shared__ double red[kSpecRows][NW][4];
   const int j = blockIdx.y, ch = blockIdx.x, nch = gridDim.x;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int k0 = ch * kSpecRows, k1 = min(nz, k0 + kSpecRows);
   const int jb = P.lcs ? (j < P.jsplit ? P.jr1 : P.jr2) : j;
   double sxr[NS], sxi[NS];
+  double2 hs[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) sxr[s] = sxi[s] = 0.0;
-  for (int kz = k0; kz < k1; ++kz) {
-    const double2 *ra = P.a + ((long long)kz * ny + j) * pitch;
-    const double2 *rb = P.b + ((long long)kz * ny + jb) * pitch;
-    double2 va[NS], vb[NS];
+  for (int s = 0; s < NS; ++s) {
+    const int kx = t + kSpecThreads * s;
+    sxr[s] = sxi[s] = 0.0;
+    hs[s] = (P.ha && kx <= h) ? P.ha[kx] : make_double2(1.0, 0.0);
+  }
+  for (int kb = k0; kb < k1; kb += RB) {
+    double2 va[RB][NS], vb[RB][NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int kx = t + kSpecThreads * s;
-      va[s] = kx <= h ? ra[kx] : make_double2(0.0, 0.0);
-      if (!SAME) vb[s] = kx <= h ? rb[kx] : make_double2(0.0, 0.0);
-    }
-    double rr = 0.0, ri = 0.0, ir = 0.0, ii = 0.0;
+    for (int r = 0; r < RB; ++r) {
+      const bool live = kb + r < k1;
+      const long long kz = live ? kb + r : k0;
+      const double2 *ra = P.a + (kz * ny + j) * pitch;
+      const double2 *rb = P.b + (kz * ny + jb) * pitch;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int kx = t + kSpecThreads * s;
-      double2 a = va[s];
-      if (kx == 0 && kz == 0) a.x += P.sa;
-      double2 b = a;
-      if (!SAME) {
-        b = vb[s];
-        if (kx == 0 && kz == 0) b.x += P.sb;
-      }
-      const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
-      sxr[s] += pr;
-      sxi[s] += pi;
-      rr += pr;
-      ri += pi;
-      if (kx > 0 && kx < h) {
-        ir += pr;
-        ii += pi;
+      for (int s = 0; s < NS; ++s) {
+        const int kx = t + kSpecThreads * s;
+        const bool in = live && kx <= h;
+        va[r][s] = in ? ra[kx] : make_double2(0.0, 0.0);
+        if (!SAME) vb[r][s] = in ? rb[kx] : make_double2(0.0, 0.0);
       }
     }
+    double q[RB][4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      rr += __shfl_xor(rr, off);
-      ri += __shfl_xor(ri, off);
-      ir += __shfl_xor(ir, off);
-      ii += __shfl_xor(ii, off);
+    for (int r = 0; r < RB; ++r) {
+      q[r][0] = q[r][1] = q[r][2] = q[r][3] = 0.0;
+      const bool origin = kb + r == 0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int kx = t + kSpecThreads * s;
+        double2 a = P.ha ? cmul(va[r][s], hs[s]) : va[r][s];
+        if (origin && kx == 0) a.x += P.sa;
+        double2 b = a;
+        if (!SAME) {
+          b = vb[r][s];
+          if (origin && kx == 0) b.x += P.sb;
+        }
+        const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
+        sxr[s] += pr;
+        sxi[s] += pi;
+        q[r][0] += pr;
+        q[r][1] += pi;
+        if (kx > 0 && kx < h) {
+          q[r][2] += pr;
+          q[r][3] += pi;
+        }
+      }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[r][e] += __shfl_xor(q[r][e], off);
     if (lane == 0) {
-      red[kz - k0][w][0] = rr;
-      red[kz - k0][w][1] = ri;
-      red[kz - k0][w][2] = ir;
-      red[kz - k0][w][3] = ii;
+#pragma unroll
+      for (int r = 0; r < RB; ++r)
+        if (kb + r < k1)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) red[kb + r - k0][w][e] = q[r][e];
     }
   }
   double2 *ox = px + ((long long)j * nch + ch) * (h + 1);
@@ -127,9 +149,9 @@ __global__ void __launch_bounds__(kSpecThreads) k_spec_part(SpecPair P, int ny, 
   if (t < k1 - k0) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < NW; ++q)
+    for (int qq = 0; qq < NW; ++qq)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] += red[t][q][e];
+      for (int e = 0; e < 4; ++e) v[e] += red[t][qq][e];
     double2 *oz = pz + 2 * ((long long)j * nz + k0 + t);
     oz[0] = make_double2(v[0], v[1]);
     oz[1] = make_double2(v[2], v[3]);
@@ -184,12 +206,21 @@ __global__ void __launch_bounds__(256) k_spec_fin(const double2 *__restrict__ px
 // The value at half-complex position (p, q) of the R2HC along x then z of
 // spectral row `row` (0-based): p <= h is the real part of the x transform at
 // kx = p, p > h its imaginary part at kx = nx - p; q likewise along z.
-__device__ inline double hc2d_value(const double2 *__restrict__ F, int ny, int nx, int nz, int pitch, int row, int p,
-                                    int q) {
+// hx (may be null) multiplies F first and shift is then added to Re F(0, 0).
+__device__ inline double hc2d_value(const double2 *__restrict__ F, const double2 *__restrict__ hx, double shift,
+                                    int ny, int nx, int nz, int pitch, int row, int p, int q) {
   const int h = nx / 2, nzh = nz / 2;
   const int kx = p <= h ? p : nx - p, kz = q <= nzh ? q : nz - q, km = kz == 0 ? 0 : nz - kz;
-  const double2 f1 = F[((long long)kz * ny + row) * pitch + kx];
-  const double2 f2 = F[((long long)km * ny + row) * pitch + kx];
+  double2 f1 = F[((long long)kz * ny + row) * pitch + kx];
+  double2 f2 = F[((long long)km * ny + row) * pitch + kx];
+  if (hx) {
+    f1 = cmul(f1, hx[kx]);
+    f2 = cmul(f2, hx[kx]);
+  }
+  if (kx == 0 && kz == 0) {
+    f1.x += shift;
+    f2.x += shift;
+  }
   if (p <= h) return q <= nzh ? 0.5 * (f1.x + f2.x) : 0.5 * (f1.y - f2.y);
   return q <= nzh ? 0.5 * (f1.y + f2.y) : 0.5 * (f2.x - f1.x);
 }
@@ -197,16 +228,19 @@ __device__ inline double hc2d_value(const double2 *__restrict__ F, int ny, int n
 // Adds one sample to the 2-D accumulator of quantity m (grid: x = p / 64,
 // y = chosen row, z = q).  mirror: 0 none (half channel), 1 the row ny+1-jct,
 // 2 the face row ny+2-jct (uy); rcoe weighs the mirror's product.
-__global__ void __launch_bounds__(64) k_spec_2d(const double2 *__restrict__ a, const double2 *__restrict__ b, int ny,
-                                                int nx, int nz, int pitch, const int *__restrict__ iy, int ny2d,
-                                                int mirror, double rcoe, double norm, double *__restrict__ out) {
+__global__ void __launch_bounds__(64) k_spec_2d(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                const double2 *__restrict__ ha, double sa, int ny, int nx, int nz,
+                                                int pitch, const int *__restrict__ iy, int ny2d, int mirror,
+                                                double rcoe, double norm, double *__restrict__ out) {
   const int p = blockIdx.x * 64 + threadIdx.x, jc = blockIdx.y, q = blockIdx.z;
   if (p >= nx) return;
   const int jct = iy[jc];
-  double v = hc2d_value(a, ny, nx, nz, pitch, jct - 1, p, q) * hc2d_value(b, ny, nx, nz, pitch, jct - 1, p, q);
+  double v = hc2d_value(a, ha, sa, ny, nx, nz, pitch, jct - 1, p, q) *
+             hc2d_value(b, nullptr, 0.0, ny, nx, nz, pitch, jct - 1, p, q);
   if (mirror) {
     const int jcs = (mirror == 2 ? ny + 2 : ny + 1) - jct;
-    v += rcoe * (hc2d_value(a, ny, nx, nz, pitch, jcs - 1, p, q) * hc2d_value(b, ny, nx, nz, pitch, jcs - 1, p, q));
+    v += rcoe * (hc2d_value(a, ha, sa, ny, nx, nz, pitch, jcs - 1, p, q) *
+                 hc2d_value(b, nullptr, 0.0, ny, nx, nz, pitch, jcs - 1, p, q));
   }
   out[p + (long long)nx * (jc + (long long)ny2d * q)] += norm * v;
 }
@@ -257,7 +291,8 @@ int spec_1d(ptc_ctx *c, const SpecPair &P, bool same, int kind, int m0, int m1) 
   return PTC_OK;
 }
 
-int spec_2d(ptc_ctx *c, const double2 *a, const double2 *b, int m, int mirror, double rcoe) {
+int spec_2d(ptc_ctx *c, const double2 *a, const double2 *b, int m, int mirror, double rcoe,
+            const double2 *ha = nullptr, double sa = 0.0) {
   SpectraState &S = c->spec;
   const int nx = c->nx, nz = c->nz;
   // normEgy of clcEnergySpectra2D / clcCosSpectra2D (:1253-1255)
@@ -266,15 +301,15 @@ int spec_2d(ptc_ctx *c, const double2 *a, const double2 *b, int m, int mirror, d
   if (c->prm.flow_type == 1) norm = norm * 0.5;
   double *out = S.d_2d + (long long)nx * S.ny2d * nz * (m - 1);
   const dim3 grid((unsigned)((nx + 63) / 64), (unsigned)S.ny2d, (unsigned)nz);
-  hipLaunchKernelGGL(k_spec_2d, grid, dim3(64), 0, c->stream, a, b, c->ny, nx, nz, S.pitch, S.d_iy2d, S.ny2d,
+  hipLaunchKernelGGL(k_spec_2d, grid, dim3(64), 0, c->stream, a, b, ha, sa, c->ny, nx, nz, S.pitch, S.d_iy2d, S.ny2d,
                      c->prm.flow_type == 1 ? mirror : 0, rcoe, norm, out);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
 
-int cell_field(ptc_ctx *c, int mode, const double *u) {
-  const dim3 grid((unsigned)((c->nx + 63) / 64), (unsigned)c->ny, (unsigned)c->nz);
-  hipLaunchKernelGGL(k_spec_cell, grid, dim3(64), 0, c->stream, c->geo, mode, u, c->prm.ucrf, c->spec.d_cell);
+int cell_uy(ptc_ctx *c, const double *uy) {
+  const dim3 grid((unsigned)((c->nx + 255) / 256), (unsigned)c->ny, (unsigned)c->nz);
+  hipLaunchKernelGGL(k_spec_cell, grid, dim3(256), 0, c->stream, c->geo, uy, c->spec.d_cell);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -310,6 +345,7 @@ int ensure_work(ptc_ctx *c) {
   release(S.d_cell);
   release(S.d_px);
   release(S.d_pz);
+  release(S.d_hx);
   S.pitch = c->pp.pitch;
   S.nch = (c->nz + kSpecRows - 1) / kSpecRows;
   const size_t nspec = (size_t)S.pitch * c->ny * c->nz;
@@ -318,6 +354,18 @@ int ensure_work(ptc_ctx *c) {
   PTC_TRY(alloc(&S.d_cell, (size_t)c->nx * (c->ny + 2) * c->nz));
   PTC_TRY(alloc(&S.d_px, (size_t)c->ny * S.nch * (c->nx / 2 + 1)));
   PTC_TRY(alloc(&S.d_pz, (size_t)2 * c->ny * c->nz));
+  // the symbol of the cell-centre interpolation of ux, kx = 0..nx/2
+  const int h = c->nx / 2;
+  const double ic[4] = {c->geo.ic1, c->geo.ic2, c->geo.ic3, c->geo.ic4};
+  std::vector<double2> hx(h + 1);
+  for (int kx = 0; kx <= h; ++kx) {
+    const double th = 2.0 * M_PI * (double)kx / (double)c->nx;
+    const double cs = std::cos(th), sn = std::sin(th), c2 = std::cos(2.0 * th), s2 = std::sin(2.0 * th);
+    // ic1 e^{-i th} + ic2 + ic3 e^{i th} + ic4 e^{2 i th}
+    hx[kx] = make_double2(ic[0] * cs + ic[1] + ic[2] * cs + ic[3] * c2, -ic[0] * sn + ic[2] * sn + ic[3] * s2);
+  }
+  PTC_TRY(alloc(&S.d_hx, hx.size()));
+  PTC_HIP(hipMemcpy(S.d_hx, hx.data(), sizeof(double2) * hx.size(), hipMemcpyHostToDevice));
   return PTC_OK;
 }
 
@@ -334,6 +382,7 @@ void spectra_free(ptc_ctx *c) {
   release(S.d_cell);
   release(S.d_px);
   release(S.d_pz);
+  release(S.d_hx);
   S = SpectraState{};
 }
 
@@ -392,29 +441,28 @@ int spectra_sample(ptc_ctx *c, const double *ux, const double *uy, const double 
   const double crf = (double)c->nx * (double)c->nz * c->prm.ucrf;  // ux + uCRF moves F(0, 0) only
   SpecPair P{};
   // cell-centre uy -> fb
-  PTC_TRY(cell_field(c, 1, uy));
+  PTC_TRY(cell_uy(c, uy));
   PTC_TRY(forward(c, S.d_cell, S.d_fb));
-  // ux -> fa: energy (1), linear coherence (7, 8), co-spectrum with the cell-centre uy (5); 2-D ux (1)
+  // ux -> fa: energy (1), linear coherence (7, 8), co-spectra with the
+  // cell-centre uy of ux + uCRF (5) and of the cell-centre ux (6, H(kx) fa);
+  // 2-D: ux (1) and the cell-centre co-spectrum (5, IsSymmetry = .false.: rCoe = -1)
   PTC_TRY(forward(c, ux, S.d_fa));
   if (S.on1d) {
-    P = SpecPair{S.d_fa, S.d_fa, crf, crf, 0, 0, 0, 0};
+    P = SpecPair{S.d_fa, S.d_fa, crf, crf, 0, 0, 0, 0, nullptr};
     PTC_TRY(spec_1d(c, P, true, 0, 1, 1));
     // :473-489: rows 1..nyc/2 against jForLCS, a channel's upper rows against nyc+1-jForLCS
     const bool ch = c->prm.flow_type == 1;
-    P = SpecPair{S.d_fa, S.d_fa, crf, crf, 1, S.jlcs - 1, ch ? ny - S.jlcs : S.jlcs - 1, ch ? ny / 2 : ny};
+    P = SpecPair{S.d_fa, S.d_fa, crf, crf, 1, S.jlcs - 1, ch ? ny - S.jlcs : S.jlcs - 1, ch ? ny / 2 : ny, nullptr};
     PTC_TRY(spec_1d(c, P, false, 1, 7, 8));
-    P = SpecPair{S.d_fa, S.d_fb, crf, 0.0, 0, 0, 0, 0};
+    P = SpecPair{S.d_fa, S.d_fb, crf, 0.0, 0, 0, 0, 0, nullptr};
     PTC_TRY(spec_1d(c, P, false, 0, 5, 5));
-  }
-  if (S.on2d) PTC_TRY(spec_2d(c, S.d_fa, S.d_fa, 1, 1, 1.0));
-  // cell-centre ux (+uCRF) -> fa: co-spectrum 6; 2-D co-spectrum 5 (IsSymmetry = .false., rCoe = -1)
-  PTC_TRY(cell_field(c, 0, ux));
-  PTC_TRY(forward(c, S.d_cell, S.d_fa));
-  if (S.on1d) {
-    P = SpecPair{S.d_fa, S.d_fb, 0.0, 0.0, 0, 0, 0, 0};
+    P = SpecPair{S.d_fa, S.d_fb, crf, 0.0, 0, 0, 0, 0, S.d_hx};
     PTC_TRY(spec_1d(c, P, false, 0, 6, 6));
   }
-  if (S.on2d) PTC_TRY(spec_2d(c, S.d_fa, S.d_fb, 5, 1, -1.0));
+  if (S.on2d) {
+    PTC_TRY(spec_2d(c, S.d_fa, S.d_fa, 1, 1, 1.0));
+    PTC_TRY(spec_2d(c, S.d_fa, S.d_fb, 5, 1, -1.0, S.d_hx, crf));
+  }
   // uy (2-D: the face mirror), uz, p, scalar
   const double *rest[4] = {uy, uz, p, sc};
   const int m1[4] = {2, 3, 4, 9}, m2[4] = {2, 3, 4, 6};
@@ -422,7 +470,7 @@ int spectra_sample(ptc_ctx *c, const double *ux, const double *uy, const double 
     if (q == 3 && !c->prm.scalar) break;
     PTC_TRY(forward(c, rest[q], S.d_fa));
     if (S.on1d) {
-      P = SpecPair{S.d_fa, S.d_fa, 0.0, 0.0, 0, 0, 0, 0};
+      P = SpecPair{S.d_fa, S.d_fa, 0.0, 0.0, 0, 0, 0, 0, nullptr};
       PTC_TRY(spec_1d(c, P, true, 0, m1[q], m1[q]));
     }
     if (S.on2d) PTC_TRY(spec_2d(c, S.d_fa, S.d_fa, m2[q], q == 0 ? 2 : 1, 1.0));
