@@ -493,15 +493,34 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
   f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id (w -> XCD w % 8), and
+// each XCD has its own L2.  xcd_group maps w to a work index so that GRP
+// consecutive work indices run on one XCD as consecutive workgroups there;
+// the last (nwg mod 8 GRP) ids keep their own index.
+template <int GRP>
+__device__ inline int xcd_group(int w, int nwg) {
+  if (GRP == 1) return w;
+  constexpr int span = 8 * GRP;
+  if (w >= (nwg / span) * span) return w;
+  const int x = w % 8, s = w / 8;
+  return GRP * (x + 8 * (s / GRP)) + s % GRP;
+}
+
 // ---- z pass: T columns x N rows, thread id = g * T + col ----------------
+// 1-D grid of ntile * nj workgroups: column tile and plane from the
+// XCD-grouped index, so that the tiles sharing a 128-byte row segment (narrow
+// tiles of the long lengths) are read and written through one L2.
 template <int N, int T, bool INV, typename S>
 __global__ void __launch_bounds__(f2_threads<N>() * T) k2_fft_z(S *__restrict__ spec, const double2 *__restrict__ tw, int ny,
-                                                 int pitch, int j0) {
+                                                 int pitch, int j0, int ntile) {
   constexpr int G = f2_threads<N>();
+  constexpr int GRP = (int)(128 / (T * sizeof(S))) > 1 ? (int)(128 / (T * sizeof(S))) : 1;
   __shared__ double2 sh[N * T];
   const int col = threadIdx.x % T, g = threadIdx.x / T;
   const long long zs = (long long)pitch * ny;
-  S *base = spec + (long long)(j0 + blockIdx.y) * pitch + (long long)blockIdx.x * T + col;
+  const int w = xcd_group<GRP>((int)blockIdx.x, (int)gridDim.x);
+  const int tile = w % ntile, jj = w / ntile;
+  S *base = spec + (long long)(j0 + jj) * pitch + (long long)tile * T + col;
   auto load0 = [&](int m) { return f2::ld2(base[m * zs]); };
   auto lds = [&](int m) -> double2 & { return sh[m * T + col]; };
   auto store_out = [&](int m, double2 v) { f2::st2(base[m * zs], v); };
@@ -564,23 +583,24 @@ static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int 
 template <int N, int T>
 static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64) {
   constexpr int G = f2_threads<N>();
-  dim3 grid((unsigned)(c->pp.pitch / T), (unsigned)nj);
+  const int ntile = c->pp.pitch / T;
+  dim3 grid((unsigned)(ntile * nj));
   const auto *tw = c->pp.fz.tw;
   const int ny = c->ny, pitch = c->pp.pitch;
   if (!f64 && c->pp.prec == PTC_PREC_FP32_REFINE) {
     if (inverse)
       hipLaunchKernelGGL((k2_fft_z<N, T, true, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny, pitch,
-                         j0);
+                         j0, ntile);
     else
       hipLaunchKernelGGL((k2_fft_z<N, T, false, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny,
-                         pitch, j0);
+                         pitch, j0, ntile);
   } else {
     if (inverse)
       hipLaunchKernelGGL((k2_fft_z<N, T, true, double2>), grid, dim3(G * T), 0, c->stream, (double2 *)spec, tw, ny,
-                         pitch, j0);
+                         pitch, j0, ntile);
     else
       hipLaunchKernelGGL((k2_fft_z<N, T, false, double2>), grid, dim3(G * T), 0, c->stream, (double2 *)spec, tw, ny,
-                         pitch, j0);
+                         pitch, j0, ntile);
   }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
@@ -621,8 +641,9 @@ int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, 
 #define PTC_Z_SIZES(Z) \
   Z(16, 16) Z(32, 16) Z(64, 16) Z(96, 16) Z(128, 16) Z(192, 16) Z(256, 16) Z(512, 8) Z(700, 4) Z(1400, 2)
 // wider alternatives for the autotuner: 256-byte row segments, one 1024-thread
-// workgroup of 128 KB LDS per CU
-#define PTC_Z_WIDE(Z) Z(256, 32) Z(512, 16)
+// workgroup of 128 KB LDS per CU; for the long lengths twice the default
+// tile, one workgroup of 90 KB LDS per CU
+#define PTC_Z_WIDE(Z) Z(256, 32) Z(512, 16) Z(700, 8) Z(1400, 4)
 
 // req = 0: the default tile; else that tile if compiled for nz
 bool fft2_has_z(int nz, int req, int *tile) {

@@ -80,196 +80,25 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return sh[0];
 }
 
-// ---- clcRhsXYZ -------------------------------------------------------------
-// block (64, 4): i = bx*64 + tx, j = by*4 + ty + 1, k = bz
-__global__ void __launch_bounds__(256) k_rhs_xyz(Geo g, RhsCoef cf, const double *__restrict__ ux,
-                                                const double *__restrict__ uy, const double *__restrict__ uz,
-                                                const double *__restrict__ sc, const double *__restrict__ p,
-                                                double *__restrict__ rhsx, double *__restrict__ rhsy,
-                                                double *__restrict__ rhsz, double *__restrict__ rhsc,
-                                                double *__restrict__ hx, double *__restrict__ hy,
-                                                double *__restrict__ hz, double *__restrict__ hc,
-                                                double *__restrict__ s3part) {
-  __shared__ double sh[256];
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y + 1;
-  const int k = blockIdx.z;
-  double s3 = 0.0;
-  if (i < g.nx && j <= g.ny) {
-    const Ngh N(g, i, k);
-    const int jm = j - 1, jp = j + 1;
-    const int nyp = g.ny + 1;
-    const double *M = g.met;
-    const int W = g.nyh;
-    const double rdyp = M[M_RDYP * W + j], rdyc = M[M_RDYC * W + j];
-    const double IY1 = M[M_YINT * W + j], IY2 = 1.0 - IY1;
-    const double IY3 = M[M_YINT * W + jp], IY4 = 1.0 - IY3;
-    const long long o = N.ii[3] + N.nxl * j + N.kk[3];
-    const double c1 = g.ic1, c2 = g.ic2, c3 = g.ic3, c4 = g.ic4;
-    // interpolations (src/m_TScheme.f90:148-154) evaluated where needed
-    auto uxhx = [&](int di) {  // at (i+di, j, k)
-      return c1 * N(ux, di - 1, j, 0) + c2 * N(ux, di, j, 0) + c3 * N(ux, di + 1, j, 0) + c4 * N(ux, di + 2, j, 0);
-    };
-    auto uyhx = [&](int di, int jj) {  // at (i+di, jj, k); zero on row nyp (:157-160)
-      if (jj == nyp) return 0.0;
-      return c1 * N(uy, di - 2, jj, 0) + c2 * N(uy, di - 1, jj, 0) + c3 * N(uy, di, jj, 0) + c4 * N(uy, di + 1, jj, 0);
-    };
-    auto uzhx = [&](int di, int dk) {
-      return c1 * N(uz, di - 2, j, dk) + c2 * N(uz, di - 1, j, dk) + c3 * N(uz, di, j, dk) + c4 * N(uz, di + 1, j, dk);
-    };
-    auto uxhz = [&](int di, int dk) {
-      return c1 * N(ux, di, j, dk - 2) + c2 * N(ux, di, j, dk - 1) + c3 * N(ux, di, j, dk) + c4 * N(ux, di, j, dk + 1);
-    };
-    auto uyhz = [&](int dk, int jj) {
-      if (jj == nyp) return 0.0;
-      return c1 * N(uy, 0, jj, dk - 2) + c2 * N(uy, 0, jj, dk - 1) + c3 * N(uy, 0, jj, dk) + c4 * N(uy, 0, jj, dk + 1);
-    };
-    auto uzhz = [&](int dk) {
-      return c1 * N(uz, 0, j, dk - 1) + c2 * N(uz, 0, j, dk) + c3 * N(uz, 0, j, dk + 1) + c4 * N(uz, 0, j, dk + 2);
-    };
-    const double ucrf = cf.ucrf;
-
-    // ---- RhsX (:169-204)
-    double rx, cx, d22q1, dcq13x;
-    {
-      const double uxhyc = IY1 * N(ux, 0, jm, 0) + IY2 * N(ux, 0, j, 0);
-      const double uxhyp = IY3 * N(ux, 0, j, 0) + IY4 * N(ux, 0, jp, 0);
-      const double a0 = uxhx(-2), a1 = uxhx(-1), a2 = uxhx(0), a3 = uxhx(1);
-      const double h11 = g.dxc1 * a0 * (a0 + ucrf) + g.dxc2 * a1 * (a1 + ucrf) + g.dxc3 * a2 * (a2 + ucrf) +
-                         g.dxc4 * a3 * (a3 + ucrf);
-      const double h12 = (uyhx(0, jp) * uxhyp - uyhx(0, j) * uxhyc) * rdyp;
-      const double h13 = g.dzc1 * uxhz(0, -1) * uzhx(0, -1) + g.dzc2 * uxhz(0, 0) * uzhx(0, 0) +
-                         g.dzc3 * uxhz(0, 1) * uzhx(0, 1) + g.dzc4 * uxhz(0, 2) * uzhx(0, 2);
-      const double d11 = g.dxxc1 * N(ux, -2, j, 0) + g.dxxc2 * N(ux, -1, j, 0) + g.dxxc3 * N(ux, 0, j, 0) +
-                         g.dxxc4 * N(ux, 1, j, 0) + g.dxxc5 * N(ux, 2, j, 0);
-      d22q1 = M[M_AP2C * W + j] * N(ux, 0, jp, 0) + M[M_AC2C * W + j] * N(ux, 0, j, 0) +
-              M[M_AM2C * W + j] * N(ux, 0, jm, 0);
-      const double d33 = g.dzzc1 * N(ux, 0, j, -2) + g.dzzc2 * N(ux, 0, j, -1) + g.dzzc3 * N(ux, 0, j, 0) +
-                         g.dzzc4 * N(ux, 0, j, 1) + g.dzzc5 * N(ux, 0, j, 2);
-      dcq13x = d11 + d33;
-      cx = -h11 - h12 - h13 + cf.xnu * dcq13x + cf.gx;
-      const double gp = g.dxc1 * N(p, -2, j, 0) + g.dxc2 * N(p, -1, j, 0) + g.dxc3 * N(p, 0, j, 0) +
-                        g.dxc4 * N(p, 1, j, 0);
-      rx = cf.gamma * cx + cf.theta * hx[o] - cf.alpha * gp + 2.0 * cf.beta * d22q1;
-    }
-    if (cf.s3_mode == 1) s3 = (dcq13x + d22q1) * M[M_DYP * W + j];
-    else if (cf.s3_mode == 2) s3 = N(ux, 0, j, 0) * M[M_DYP * W + j];
-
-    // ---- RhsY (:206-236)
-    double ry, cy;
-    {
-      const double qs = 0.25 * rdyc;
-      const double uxhym = IY1 * N(ux, -1, jm, 0) + IY2 * N(ux, -1, j, 0) + ucrf;
-      const double uxhyc = IY1 * N(ux, 0, jm, 0) + IY2 * N(ux, 0, j, 0) + ucrf;
-      const double uxhyp = IY1 * N(ux, 1, jm, 0) + IY2 * N(ux, 1, j, 0) + ucrf;
-      const double uxhyu = IY1 * N(ux, 2, jm, 0) + IY2 * N(ux, 2, j, 0) + ucrf;
-      const double uzhym = IY1 * N(uz, 0, jm, -1) + IY2 * N(uz, 0, j, -1);
-      const double uzhyc = IY1 * N(uz, 0, jm, 0) + IY2 * N(uz, 0, j, 0);
-      const double uzhyp = IY1 * N(uz, 0, jm, 1) + IY2 * N(uz, 0, j, 1);
-      const double uzhyu = IY1 * N(uz, 0, jm, 2) + IY2 * N(uz, 0, j, 2);
-      const double h21 = g.dxc1 * uyhx(-1, j) * uxhym + g.dxc2 * uyhx(0, j) * uxhyc + g.dxc3 * uyhx(1, j) * uxhyp +
-                         g.dxc4 * uyhx(2, j) * uxhyu;
-      const double s1 = N(uy, 0, jp, 0) + N(uy, 0, j, 0);
-      const double s2 = N(uy, 0, j, 0) + N(uy, 0, jm, 0);
-      const double h22 = (s1 * s1 - s2 * s2) * qs;
-      const double h23 = g.dzc1 * uyhz(-1, j) * uzhym + g.dzc2 * uyhz(0, j) * uzhyc + g.dzc3 * uyhz(1, j) * uzhyp +
-                         g.dzc4 * uyhz(2, j) * uzhyu;
-      const double d11 = g.dxxc1 * N(uy, -2, j, 0) + g.dxxc2 * N(uy, -1, j, 0) + g.dxxc3 * N(uy, 0, j, 0) +
-                         g.dxxc4 * N(uy, 1, j, 0) + g.dxxc5 * N(uy, 2, j, 0);
-      const double d22 = M[M_AP2P * W + j] * N(uy, 0, jp, 0) + M[M_AC2P * W + j] * N(uy, 0, j, 0) +
-                         M[M_AM2P * W + j] * N(uy, 0, jm, 0);
-      const double d33 = g.dzzc1 * N(uy, 0, j, -2) + g.dzzc2 * N(uy, 0, j, -1) + g.dzzc3 * N(uy, 0, j, 0) +
-                         g.dzzc4 * N(uy, 0, j, 1) + g.dzzc5 * N(uy, 0, j, 2);
-      cy = -h21 - h22 - h23 + cf.xnu * (d11 + d33) + cf.gy;
-      const double gp = (N(p, 0, j, 0) - N(p, 0, jm, 0)) * rdyc;
-      ry = cf.gamma * cy + cf.theta * hy[o] - cf.alpha * gp + 2.0 * cf.beta * d22;
-    }
-
-    // ---- RhsZ (:238-267)
-    double rz, cz;
-    {
-      const double uzhyc = IY1 * N(uz, 0, jm, 0) + IY2 * N(uz, 0, j, 0);
-      const double uzhyp = IY3 * N(uz, 0, j, 0) + IY4 * N(uz, 0, jp, 0);
-      const double h31 = g.dxc1 * uzhx(-1, 0) * (uxhz(-1, 0) + ucrf) + g.dxc2 * uzhx(0, 0) * (uxhz(0, 0) + ucrf) +
-                         g.dxc3 * uzhx(1, 0) * (uxhz(1, 0) + ucrf) + g.dxc4 * uzhx(2, 0) * (uxhz(2, 0) + ucrf);
-      const double h32 = (uyhz(0, jp) * uzhyp - uyhz(0, j) * uzhyc) * rdyp;
-      const double b0 = uzhz(-2), b1 = uzhz(-1), b2 = uzhz(0), b3 = uzhz(1);
-      const double h33 = g.dzc1 * b0 * b0 + g.dzc2 * b1 * b1 + g.dzc3 * b2 * b2 + g.dzc4 * b3 * b3;
-      const double d11 = g.dxxc1 * N(uz, -2, j, 0) + g.dxxc2 * N(uz, -1, j, 0) + g.dxxc3 * N(uz, 0, j, 0) +
-                         g.dxxc4 * N(uz, 1, j, 0) + g.dxxc5 * N(uz, 2, j, 0);
-      const double d22 = M[M_AP2C * W + j] * N(uz, 0, jp, 0) + M[M_AC2C * W + j] * N(uz, 0, j, 0) +
-                         M[M_AM2C * W + j] * N(uz, 0, jm, 0);
-      const double d33 = g.dzzc1 * N(uz, 0, j, -2) + g.dzzc2 * N(uz, 0, j, -1) + g.dzzc3 * N(uz, 0, j, 0) +
-                         g.dzzc4 * N(uz, 0, j, 1) + g.dzzc5 * N(uz, 0, j, 2);
-      cz = -h31 - h32 - h33 + cf.xnu * (d11 + d33) + cf.gz;
-      const double gp = g.dzc1 * N(p, 0, j, -2) + g.dzc2 * N(p, 0, j, -1) + g.dzc3 * N(p, 0, j, 0) +
-                        g.dzc4 * N(p, 0, j, 1);
-      rz = cf.gamma * cz + cf.theta * hz[o] - cf.alpha * gp + 2.0 * cf.beta * d22;
-    }
-
-    if (cf.scalar) {
-      // ---- scalar transport (:302-357)
-      auto shx = [&](int di) {
-        return c1 * N(sc, di - 2, j, 0) + c2 * N(sc, di - 1, j, 0) + c3 * N(sc, di, j, 0) + c4 * N(sc, di + 1, j, 0);
-      };
-      auto shz = [&](int dk) {
-        return c1 * N(sc, 0, j, dk - 2) + c2 * N(sc, 0, j, dk - 1) + c3 * N(sc, 0, j, dk) + c4 * N(sc, 0, j, dk + 1);
-      };
-      const double shyc = IY1 * N(sc, 0, jm, 0) + IY2 * N(sc, 0, j, 0);
-      const double shyp = IY3 * N(sc, 0, j, 0) + IY4 * N(sc, 0, jp, 0);
-      const double x0 = shx(0), z0 = shz(0);
-      const double e1 = cf.gev[0] * x0, e2 = cf.gev[1] * shyc, e3 = cf.gev[2] * z0;
-      rx = rx + cf.gamma * e1;
-      ry = ry + cf.gamma * e2;
-      rz = rz + cf.gamma * e3;
-      cx = cx + e1;
-      cy = cy + e2;
-      cz = cz + e3;
-      const double f0 = cf.fvv[0], f1 = cf.fvv[1], f2 = cf.fvv[2];
-      const double hc1 = g.dxc1 * (N(ux, -1, j, 0) + f0) * shx(-1) + g.dxc2 * (N(ux, 0, j, 0) + f0) * x0 +
-                         g.dxc3 * (N(ux, 1, j, 0) + f0) * shx(1) + g.dxc4 * (N(ux, 2, j, 0) + f0) * shx(2);
-      const double hc2 = ((N(uy, 0, jp, 0) + f1) * shyp - (N(uy, 0, j, 0) + f1) * shyc) * rdyp;
-      const double hc3 = g.dzc1 * (N(uz, 0, j, -1) + f2) * shz(-1) + g.dzc2 * (N(uz, 0, j, 0) + f2) * z0 +
-                         g.dzc3 * (N(uz, 0, j, 1) + f2) * shz(1) + g.dzc4 * (N(uz, 0, j, 2) + f2) * shz(2);
-      const double d11 = g.dxxc1 * N(sc, -2, j, 0) + g.dxxc2 * N(sc, -1, j, 0) + g.dxxc3 * N(sc, 0, j, 0) +
-                         g.dxxc4 * N(sc, 1, j, 0) + g.dxxc5 * N(sc, 2, j, 0);
-      const double d22 = M[M_AP2CSC * W + j] * N(sc, 0, jp, 0) + M[M_AC2CSC * W + j] * N(sc, 0, j, 0) +
-                         M[M_AM2CSC * W + j] * N(sc, 0, jm, 0);
-      const double d33 = g.dzzc1 * N(sc, 0, j, -2) + g.dzzc2 * N(sc, 0, j, -1) + g.dzzc3 * N(sc, 0, j, 0) +
-                         g.dzzc4 * N(sc, 0, j, 1) + g.dzzc5 * N(sc, 0, j, 2);
-      const double cc = -hc1 - hc2 - hc3 + cf.sdiff * (d11 + d33);
-      rhsc[o] = cf.gamma * cc + cf.theta * hc[o] + 2.0 * cf.beta_sc * d22;
-      hc[o] = cc;
-    }
-    rhsx[o] = rx;
-    rhsy[o] = ry;
-    rhsz[o] = rz;
-    hx[o] = cx;
-    hy[o] = cy;
-    hz[o] = cz;
-  }
-  if (cf.s3_mode) {
-    const double t = block_sum(s3, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-      s3part[b] = t;
-    }
-  }
-}
-
-// ---- clcRhsXYZ, row-marching form -----------------------------------------
-// The same right-hand sides as k_rhs_xyz, for rows of at most 512 points: a
-// workgroup owns one whole x row (one thread per column) at fixed j and
+// ---- clcRhsXYZ (src/m_TScheme.f90:113-360), row-marching form -------------
+// The interpolations of Update_VelInterp (:143-167) are formed where they are
+// used, from the neighbouring velocities, so the six haloed temporaries never
+// reach HBM.  A workgroup owns one x row (one thread per column) at fixed j and
 // marches along k over planes [k0, k1).  Each thread keeps its column's values
 // in register windows along z (ux, uy, uz, scalar: k-3..k+3; p and uy at j+1:
 // k-2..k+1; uz at j-1: k-1..k+2), so a value is loaded once per plane instead
 // of once per use, and the next plane's loads are in flight while a plane is
 // computed.  The x neighbours are the other threads' values of the same plane,
-// exchanged through LDS rows; x wraps inside the row.  The two products whose
+// exchanged through LDS rows; x wraps inside the row.  Rows longer than a
+// workgroup go in segments (seg > 0): the workgroup of lw = seg + 6 threads
+// owns seg columns from blockIdx.x * seg and loads three wrapped columns on
+// either side; those halo threads take part in the exchange (their products
+// two planes ahead included) but store nothing.  The two products whose
 // interpolations cross x and z, uxhz * uzhx in h13 (a z difference) and
 // uzhx * (uxhz + ucrf) in h31 (an x difference), are formed two planes ahead
-// and kept in short queues.  Operand order follows k_rhs_xyz except that
-// those products are formed before their difference weight is applied.
+// and kept in short queues.  Operand order follows the reference's
+// expressions except that those products are formed before their difference
+// weight is applied.
 enum {
   X_UX = 0,  // ux(., j, k)
   X_UY,      // uy(., j, k)
@@ -306,12 +135,16 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
                                                   double *__restrict__ rhsz, double *__restrict__ rhsc,
                                                   double *__restrict__ hx, double *__restrict__ hy,
                                                   double *__restrict__ hz, double *__restrict__ hc,
-                                                  double *__restrict__ s3part) {
-  extern __shared__ double xr[];  // X_NROWS rows of nx values
+                                                  double *__restrict__ s3part, int seg, int lw) {
+  extern __shared__ double xr[];  // X_NROWS rows of lw values
   const int nx = g.nx, nz = g.nz;
   const int i = threadIdx.x;
-  const bool act = i < nx;
-  const int ic = act ? i : 0;
+  // whole row: thread = column; segment: thread t = column blockIdx.x * seg - 3 + t (wrapped)
+  int gcol = seg ? (int)blockIdx.x * seg - 3 + i : i;
+  gcol = gcol < 0 ? gcol + nx : (gcol >= nx ? gcol - nx : gcol);
+  const bool act = seg ? (i >= 3 && i < 3 + seg && (int)blockIdx.x * seg + i - 3 < nx) : i < nx;
+  const bool lds_in = seg ? true : act;  // writes its values to the LDS rows
+  const int ic = seg ? gcol : (act ? i : 0);
   const int j = blockIdx.y + 1, jp = j + 1;
   const int nyp = g.ny + 1;
   const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, nz);
@@ -324,15 +157,18 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
   const double c1 = g.ic1, c2 = g.ic2, c3 = g.ic3, c4 = g.ic4;
   const double ucrf = cf.ucrf;
   const bool top = jp == nyp;  // uy interpolations on row ny+1 are zero (src/m_TScheme.f90:157-160)
-  // wrapped x neighbours i-3..i+3
+  // LDS slots of the x neighbours i-3..i+3: wrapped in a whole row; in a
+  // segment the halo holds them (clamped for halo threads, whose results at
+  // those slots are never read)
   int nb[7];
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
-    int q = ic + s - 3;
-    q = q < 0 ? q + nx : (q >= nx ? q - nx : q);
+    int q = (seg ? i : ic) + s - 3;
+    if (seg) q = q < 0 ? 0 : (q >= lw ? lw - 1 : q);
+    else q = q < 0 ? q + nx : (q >= nx ? q - nx : q);
     nb[s] = q;
   }
-  auto X = [&](int row, int d) { return xr[row * nx + nb[d + 3]]; };
+  auto X = [&](int row, int d) { return xr[row * lw + nb[d + 3]]; };
   auto wz = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
   auto at = [&](int kk) { return (long long)wz(kk) * pl; };
   // byte offsets of this column at rows j, j-1, j+1 inside a plane: loads and
@@ -373,10 +209,10 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
   auto uxhz_at = [&](int s) {  // uxhz at plane k-3+s+2 from the ux window entries s..s+3
     return c1 * wux[s] + c2 * wux[s + 1] + c3 * wux[s + 2] + c4 * wux[s + 3];
   };
-  if (act) {
-    xr[0 * nx + i] = wuz[2];
-    xr[1 * nx + i] = wuz[3];
-    xr[2 * nx + i] = wuz[4];
+  if (lds_in) {
+    xr[0 * lw + i] = wuz[2];
+    xr[1 * lw + i] = wuz[3];
+    xr[2 * lw + i] = wuz[4];
   }
   __syncthreads();
   {
@@ -397,16 +233,16 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
                  hzo = bld(plane_rsrc(hz + kp, pbytes), bj);
     const double hco = SC ? bld(plane_rsrc(hc + kp, pbytes), bj) : 0.0;
     __syncthreads();  // the previous plane's reads are done
-    if (act) {
-      xr[X_UX * nx + i] = wux[3];
-      xr[X_UY * nx + i] = wuy[3];
-      xr[X_UZ * nx + i] = wuz[3];
-      xr[X_UZ2 * nx + i] = wuz[5];
-      xr[X_P * nx + i] = wp[2];
-      xr[X_UXM * nx + i] = ux_m;
-      xr[X_UYP * nx + i] = wuyp[2];
-      xr[X_R * nx + i] = R[0];
-      if (SC) xr[X_SC * nx + i] = wsc[3];
+    if (lds_in) {
+      xr[X_UX * lw + i] = wux[3];
+      xr[X_UY * lw + i] = wuy[3];
+      xr[X_UZ * lw + i] = wuz[3];
+      xr[X_UZ2 * lw + i] = wuz[5];
+      xr[X_P * lw + i] = wp[2];
+      xr[X_UXM * lw + i] = ux_m;
+      xr[X_UYP * lw + i] = wuyp[2];
+      xr[X_R * lw + i] = R[0];
+      if (SC) xr[X_SC * lw + i] = wsc[3];
     }
     __syncthreads();
     // the products two planes ahead
@@ -609,7 +445,7 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       __syncthreads();
       n = h;
     }
-    if (i == 0) s3part[blockIdx.y + gridDim.y * blockIdx.z] = xr[0];
+    if (i == 0) s3part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = xr[0];
   }
 }
 
@@ -890,20 +726,24 @@ static int pt_nblocks(const ptc_ctx *c) {
   return (int)(gr.x * gr.y * gr.z);
 }
 
-// row-marching RHS (rows of at most 512 points: a 1024-thread workgroup would
-// spill): one workgroup per (j, k chunk); the chunks are at least 16 planes
-// long and many enough for about 24 wavefronts per CU over the launch
+// row-marching RHS: one workgroup per (row segment, j, k chunk).  Rows of at
+// most 512 points are one segment (a 1024-thread workgroup would spill);
+// longer rows go in segments of kRhsSeg columns with a 3-column halo either
+// side (512 threads).  The k chunks are at least 16 planes long and many
+// enough for about 24 wavefronts per CU over the launch.
 constexpr int kRhsRowsMaxNx = 512;
-static bool rhs_rows(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx; }
+constexpr int kRhsSeg = kRhsRowsMaxNx - 6;
+static int rhs_seg(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx ? 0 : kRhsSeg; }
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
-  const int wpb = (c->nx + 63) / 64;
-  const int want = (24 * c->n_cu + c->ny * wpb - 1) / (c->ny * wpb);
+  const int seg = rhs_seg(c);
+  const int nseg = seg ? (c->nx + seg - 1) / seg : 1;
+  const int wpb = seg ? kRhsRowsMaxNx / 64 : (c->nx + 63) / 64;
+  const int want = (24 * c->n_cu + c->ny * nseg * wpb - 1) / (c->ny * nseg * wpb);
   const int nch = std::max(1, std::min(want, c->nz / 16));
   *kc = (c->nz + nch - 1) / nch;
-  return dim3(1u, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
+  return dim3((unsigned)nseg, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
 }
 static int rhs_nblocks(const ptc_ctx *c) {
-  if (!rhs_rows(c)) return pt_nblocks(c);
   int kc = 0;
   const dim3 gr = rhs_rows_grid(c, &kc);
   return (int)(gr.x * gr.y * gr.z);
@@ -942,26 +782,23 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   cf.s3_mode = c->prm.is_ux_const ? (c->prm.scalar ? 2 : 1) : 0;
   const int nb = rhs_nblocks(c);
   PTC_TRY(ensure_red(c, nb));
-  if (rhs_rows(c)) {
-    int kc = 0;
-    const dim3 grid = rhs_rows_grid(c, &kc);
-    const int nt = (c->nx + 63) / 64 * 64;
-    const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * c->nx, nt);
-#define PTC_ROWS(SCAL, MAXT)                                                                                    \
+  int kc = 0;
+  const dim3 grid = rhs_rows_grid(c, &kc);
+  const int seg = rhs_seg(c);
+  const int nt = seg ? kRhsRowsMaxNx : (c->nx + 63) / 64 * 64;
+  const int lw = seg ? kRhsRowsMaxNx : c->nx;  // LDS row width
+  const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * lw, nt);
+#define PTC_ROWS(SCAL, MAXT)                                                                                  \
   hipLaunchKernelGGL((k_rhs_rows<SCAL, MAXT>), grid, dim3(nt), lds, c->stream, c->geo, cf, kc, ux, uy, uz, sc, p, \
-                     rhsx, rhsy, rhsz, rhsc, hx, hy, hz, hc, c->d_red)
-    if (nt <= 256) {
-      if (cf.scalar) PTC_ROWS(true, 256);
-      else PTC_ROWS(false, 256);
-    } else {
-      if (cf.scalar) PTC_ROWS(true, 512);
-      else PTC_ROWS(false, 512);
-    }
-#undef PTC_ROWS
+                     rhsx, rhsy, rhsz, rhsc, hx, hy, hz, hc, c->d_red, seg, lw)
+  if (nt <= 256) {
+    if (cf.scalar) PTC_ROWS(true, 256);
+    else PTC_ROWS(false, 256);
   } else {
-    hipLaunchKernelGGL(k_rhs_xyz, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, cf, ux, uy, uz, sc, p, rhsx, rhsy,
-                       rhsz, rhsc, hx, hy, hz, hc, c->d_red);
+    if (cf.scalar) PTC_ROWS(true, 512);
+    else PTC_ROWS(false, 512);
   }
+#undef PTC_ROWS
   PTC_HIP(hipGetLastError());
   if (cf.s3_mode) {
     hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,

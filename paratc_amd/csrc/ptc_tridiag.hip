@@ -554,7 +554,7 @@ struct TriShape {
 };
 static const TriShape kPoissonShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {12, 64, 1024}, {16, 64, 1024},
                                           {8, 32, 1024},  {12, 32, 1024}, {16, 32, 512},  {16, 16, 256}};
-static const TriShape kCnShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {8, 32, 1024}, {16, 32, 512}};
+static const TriShape kCnShapes[] = {{4, 64, 1024}, {8, 64, 1024}, {12, 64, 1024}, {8, 32, 1024}, {16, 32, 512}};
 
 static int tri_choose(int ny, size_t lds_limit, int mm_req, int lpb_req, const TriShape *shapes, int nshapes,
                       int *P_out, int *mm_out, int *lpb_out) {
@@ -628,7 +628,7 @@ int tri_plan_cplx(int ny, size_t lds_limit, int cl, int mm_req, int *P_out, int 
 int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int lpb_req, int *P_out, int *mm_out,
              int *lpb_out) {
   return poisson ? tri_choose(ny, lds_limit, mm_req, lpb_req, kPoissonShapes, 8, P_out, mm_out, lpb_out)
-                 : tri_choose(ny, lds_limit, mm_req, lpb_req, kCnShapes, 4, P_out, mm_out, lpb_out);
+                 : tri_choose(ny, lds_limit, mm_req, lpb_req, kCnShapes, 5, P_out, mm_out, lpb_out);
 }
 
 int launch_tri_poisson(ptc_ctx *c, void *spec) {
@@ -764,6 +764,7 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
   }
   PTC_CN(4, 64)
   PTC_CN(8, 64)
+  PTC_CN(12, 64)
   PTC_CN(8, 32)
   PTC_CN(16, 32)
 #undef PTC_CN

@@ -180,7 +180,8 @@ def test_lds_staged_y_solve_matches_oracle(lines, ny, rows):
     assert res <= 1e-13 * np.abs(out).max()
 
 
-@pytest.mark.parametrize("grid,ztile", [((64, 8, 512), 16), ((128, 12, 256), 32)])
+@pytest.mark.parametrize("grid,ztile", [((64, 8, 512), 16), ((128, 12, 256), 32), ((32, 8, 700), 8),
+                                        ((16, 6, 1400), 4)])
 def test_wide_z_tiles_match_oracle(grid, ztile):
     nx, ny, nz = grid
     case = CS.case_poisson(nx, ny, nz, 2.0)

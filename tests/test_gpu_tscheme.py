@@ -68,10 +68,11 @@ def test_clc_rhs_xyz(kind, ns):
 
 
 # rows of 8 and 96 points (part-filled wavefronts), 256 and 400 (several
-# wavefronts per row, both workgroup bounds) and 640 (beyond the
-# row-marching kernel: the per-point kernel)
+# wavefronts per row, both workgroup bounds); 640, 1024 and 1536 go in 2, 3
+# and 4 segments of up to 506 columns with wrapped halos, the last one short
 @pytest.mark.parametrize("kind", ["channel", "gravity"])
-@pytest.mark.parametrize("grid", [(8, 8, 8), (96, 20, 24), (256, 10, 12), (400, 8, 8), (640, 8, 8)],
+@pytest.mark.parametrize("grid", [(8, 8, 8), (96, 20, 24), (256, 10, 12), (400, 8, 8), (640, 8, 8),
+                                  (1024, 6, 20), (1536, 6, 8)],
                          ids=lambda g: "x".join(map(str, g)))
 def test_clc_rhs_xyz_row_lengths(kind, grid):
     rhs_against_oracle(kind, 2, grid)
