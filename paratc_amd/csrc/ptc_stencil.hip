@@ -729,15 +729,17 @@ static int pt_nblocks(const ptc_ctx *c) {
 // row-marching RHS: one workgroup per (row segment, j, k chunk).  Rows of at
 // most 512 points are one segment (a 1024-thread workgroup would spill);
 // longer rows go in segments of kRhsSeg columns with a 3-column halo either
-// side (512 threads).  The k chunks are at least 16 planes long and many
-// enough for about 24 wavefronts per CU over the launch.
+// side (kRhsSegThreads threads: at ~200 VGPRs two such workgroups share a CU
+// and overlap each other's per-plane barriers).  The k chunks are at least 16
+// planes long and many enough for about 24 wavefronts per CU over the launch.
 constexpr int kRhsRowsMaxNx = 512;
-constexpr int kRhsSeg = kRhsRowsMaxNx - 6;
+constexpr int kRhsSegThreads = 256;
+constexpr int kRhsSeg = kRhsSegThreads - 6;
 static int rhs_seg(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx ? 0 : kRhsSeg; }
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
   const int seg = rhs_seg(c);
   const int nseg = seg ? (c->nx + seg - 1) / seg : 1;
-  const int wpb = seg ? kRhsRowsMaxNx / 64 : (c->nx + 63) / 64;
+  const int wpb = seg ? kRhsSegThreads / 64 : (c->nx + 63) / 64;
   const int want = (24 * c->n_cu + c->ny * nseg * wpb - 1) / (c->ny * nseg * wpb);
   const int nch = std::max(1, std::min(want, c->nz / 16));
   *kc = (c->nz + nch - 1) / nch;
@@ -785,8 +787,8 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   int kc = 0;
   const dim3 grid = rhs_rows_grid(c, &kc);
   const int seg = rhs_seg(c);
-  const int nt = seg ? kRhsRowsMaxNx : (c->nx + 63) / 64 * 64;
-  const int lw = seg ? kRhsRowsMaxNx : c->nx;  // LDS row width
+  const int nt = seg ? kRhsSegThreads : (c->nx + 63) / 64 * 64;
+  const int lw = seg ? kRhsSegThreads : c->nx;  // LDS row width
   const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * lw, nt);
 #define PTC_ROWS(SCAL, MAXT)                                                                                  \
   hipLaunchKernelGGL((k_rhs_rows<SCAL, MAXT>), grid, dim3(nt), lds, c->stream, c->geo, cf, kc, ux, uy, uz, sc, p, \

@@ -68,8 +68,8 @@ def test_clc_rhs_xyz(kind, ns):
 
 
 # rows of 8 and 96 points (part-filled wavefronts), 256 and 400 (several
-# wavefronts per row, both workgroup bounds); 640, 1024 and 1536 go in 2, 3
-# and 4 segments of up to 506 columns with wrapped halos, the last one short
+# wavefronts per row, both workgroup bounds); 640, 1024 and 1536 go in 3, 5
+# and 7 segments of up to 250 columns with wrapped halos, the last one short
 @pytest.mark.parametrize("kind", ["channel", "gravity"])
 @pytest.mark.parametrize("grid", [(8, 8, 8), (96, 20, 24), (256, 10, 12), (400, 8, 8), (640, 8, 8),
                                   (1024, 6, 20), (1536, 6, 8)],
