@@ -133,7 +133,10 @@ void orc_spectra_2d(const orc_ctx *c, const double *ux, const double *uy, const 
 /* One ChannelIterate time step (m_ChannelSystem.f90:80-145).  Buffers follow
  * the reference's aliasing: arr1 = RealArr1, arr2 = RealArr2, halo = RealHalo,
  * arrc = RealArrC.  mon receives [divmax1 (last substep), divmax2, vmax x3,
- * cfl*dt, dt].  poisson_choice selects the clcPPE variant. */
+ * cfl*dt, dt].  poisson_choice selects the clcPPE variant.
+ * Each procedure it calls is pinned on its own; the sequence of many steps is
+ * parity unpinned beyond its invariants (post-projection divergence at
+ * round-off, the constant flow rate held). */
 void orc_channel_iterate(orc_ctx *c, int poisson_choice,
                          double *ux, double *uy, double *uz, double *p, double *scalar,
                          double *histx, double *histy, double *histz, double *histc,

@@ -276,3 +276,43 @@ def test_scalar_build_flow_rate_relaxation():
     assert abs(shift.mean() - alphac * force) < 1e-12 * abs(alphac * force)
     assert np.abs(shift - shift.mean()).max() < 1e-14
     assert abs(coe[8] - force * alphac / dt) < 1e-12 * abs(force * alphac / dt)
+
+
+def test_scalar_z_advection_and_explicit_diffusion():
+    """hc3 = D_z[(w + fvv_z) I_z(c)] with fvv_z = ws e_g,z (:341-345) and the z
+    body force gamma g_e e_g,z I_z(c) at the w points; the explicit scalar
+    diffusion kappa (D_xx + D_zz) c (:346-350) by the symbol of the 5-point
+    4th-order second difference, (-2 cos 2w + 32 cos w - 30) / (12 ds^2)."""
+    case = scalar_case()
+    case.gravity_dir = (0.0, -0.6, 0.8)
+    o = O.Oracle(case)
+    g = Grid(case, o)
+    gev, fvv = case.scalar_vectors()
+    assert abs(fvv[2] - 0.03 * 0.8) < 1e-15
+    A, W, kz = 0.5, -0.35, 3 * (2 * math.pi / case.zlz)
+    si, lam = si_lam(kz, g.dz)
+    r = run(case, g.zero(), g.zero(), g.field() * W, sc=g.field(fz=A * np.sin(kz * g.zc)))
+    hc3 = (W + fvv[2]) * A * si * lam * np.cos(kz * g.zc[2:-2])
+    assert rel_err(r["rc"], -r["gam"] * np.broadcast_to(hc3[None, None, :], r["rc"].shape)) < 1e-12
+    body = r["gam"] * gev[2] * A * si * np.sin(kz * g.zf[2:-2])
+    assert rel_err(r["rz"], np.broadcast_to(body[None, None, :], r["rz"].shape)) < 1e-12
+
+    case = scalar_case()
+    case.xnu = 0.02  # Sc = 1: kappa = nu
+    case.gravity_dir = (0.0, -1.0, 0.0)  # settling along y only: no x/z advection of c
+    o = O.Oracle(case)
+    g = Grid(case, o)
+    kappa = case.xnu / case.schmidt
+    Bx, Bz, kx, kz = 0.3, 0.7, 2 * (2 * math.pi / case.xlx), 2 * (2 * math.pi / case.zlz)
+
+    def d2(k, ds):
+        w = k * ds
+        return (-2 * math.cos(2 * w) + 32 * math.cos(w) - 30) / (12 * ds * ds)
+
+    sc = g.field(fx=Bx * np.cos(kx * g.xc)) + g.field(fz=Bz * np.cos(kz * g.zc))
+    r = run(case, g.zero(), g.zero(), g.zero(), sc=np.asfortranarray(sc))
+    lap = (Bx * d2(kx, g.dx) * np.cos(kx * g.xc[2:-2]))[:, None, None] + \
+          (Bz * d2(kz, g.dz) * np.cos(kz * g.zc[2:-2]))[None, None, :]
+    want = np.broadcast_to(kappa * lap, r["hc"].shape)
+    assert rel_err(r["hc"], want) < 1e-12
+    assert rel_err(r["rc"], r["gam"] * want) < 1e-12
