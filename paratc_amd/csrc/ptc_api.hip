@@ -181,15 +181,16 @@ int channel_step(ptc_ctx *c, double *mon) {
   }
   for (int ns = 1; ns <= c->iadvance; ++ns) {
     PTC_TRY(pm_coe_update(c, ns, dt));
-    PTC_TRY(launch_uy_ym(c, uym, duym));
+    // Update_uy_ym (:105) runs inside the v solve below: nothing in between reads uy_ym or duy_ym
     PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p));
     {  // clcScalar, clcU1Hat, clcU2Hat, clcU3Hat (:111,116-118): independent solves, one launch
       CnJobs cn{};
       if (scal) cn.j[cn.n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
       cn.j[cn.n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
-      // clcU2Hat with SetBC_and_UpdateHaloForPrSrc's v wall rows (:122)
+      // clcU2Hat with Update_uy_ym and SetBC_and_UpdateHaloForPrSrc's v wall rows (:122)
       CnJob v = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
-      v.wall = uym;
+      v.uym_upd = uym;
+      v.duym_upd = duym;
       v.top = c->prm.flow_type == 1 ? c->prm.uybc[3] : 0.0;
       cn.j[cn.n++] = v;
       cn.j[cn.n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
@@ -199,8 +200,9 @@ int channel_step(ptc_ctx *c, double *mon) {
     // last substep's, reported by the monitors)
     PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr, ns == c->iadvance));
     PTC_TRY(poisson_solve(c, a1, hl, true));  // with SetBC_and_UpdateHalo_pr(prphiHalo) (:126)
-    PTC_TRY(launch_vel_p_update(c, hl, ux, uy, uz, p));
-    PTC_TRY(launch_bc_vel_pr(c, ux, uy, uz, uym, p));
+    // FluidVelUpdate, PressureUpdate, then SetBC_and_UpdateHalo and
+    // SetBC_and_UpdateHalo_pr (:129-136) in one launch
+    PTC_TRY(launch_vel_p_update_bc(c, hl, ux, uy, uz, p, uym));
     if (scal) PTC_TRY(launch_bc_scalar(c, sc));
   }
   if (mon) {

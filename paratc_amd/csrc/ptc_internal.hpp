@@ -245,6 +245,10 @@ struct CnJob {
   const double *rhs, *duy_ym, *shift, *wall;
   double beta, top;
   int kind, rA, rB, rC;
+  // v job with Update_uy_ym folded in: the thread of a line's wall row reads
+  // the old uy_ym, uses -old as the wall du, writes uy_ym = 0 and
+  // duy_ym = -old, and fills the wall row with the new uy_ym
+  double *uym_upd, *duym_upd;
 };
 constexpr int kCnMaxJobs = 4;
 struct CnJobs {
@@ -267,7 +271,10 @@ int launch_pressure_update(ptc_ctx *c, double *p, const double *phi);
 int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p);
 int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, bool prsrc_only);
 int launch_bc_pr(ptc_ctx *c, double *p);
-int launch_bc_vel_pr(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, double *p);
+// the velocity/pressure update with the wall and Neumann ghost rows written
+// by the same threads (SetBC_and_UpdateHalo + SetBC_and_UpdateHalo_pr)
+int launch_vel_p_update_bc(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p,
+                           const double *uy_ym);
 int launch_bc_scalar(ptc_ctx *c, double *s);
 int launch_visu(ptc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *p,
                 const double *sc, int iskip, int jskip, int kskip, int real_bytes, void *dev_out);

@@ -520,10 +520,21 @@ __global__ void __launch_bounds__(kReduceThreads) k_final_sum(const double *__re
 }
 
 // ---- FluidVelUpdate (:82-107) and PressureUpdate (:512-532) ---------------
+// wall values of the velocity boundary conditions (m_BC_and_Halo)
+struct BcArgs {
+  int mode, flow_type;
+  double uxb_m, uxb_p, uyb_p, uzb_m, uzb_p;
+  double r1m, r2m, r1p, r2p;
+};
+
+// bc: the threads of rows 1 and ny then write the ghost rows of
+// SetBC_and_UpdateHalo_pr (p, src/m_BC_and_Halo.f90:115-120) and
+// SetBC_and_UpdateHalo (velocities, :32-58, uy's bottom face from uy_ym) from
+// the values they have just formed: the boundary pass of the substep folded in.
 __global__ void __launch_bounds__(256) k_vel_p_update(Geo g, const double *__restrict__ phi, double *__restrict__ ux,
                                                      double *__restrict__ uy, double *__restrict__ uz,
                                                      double *__restrict__ p, double alpha, double beta, int do_vel,
-                                                     int do_p) {
+                                                     int do_p, int bc, BcArgs b, const double *__restrict__ uy_ym) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y + 1;
   const int k = blockIdx.z;
@@ -548,15 +559,31 @@ __global__ void __launch_bounds__(256) k_vel_p_update(Geo g, const double *__res
     const double bm = -beta * g.met[M_AM2PH * W + j];
     p[o] = p[o] + bp * N(phi, 0, j + 1, 0) + bc * N(phi, 0, j, 0) + bm * N(phi, 0, j - 1, 0);
   }
+  if (bc) {
+    const long long dn = (long long)g.nx;  // one row
+    if (j == 1) {
+      p[o - dn] = p[o];
+      ux[o - dn] = b.uxb_m * 2.0 - ux[o];
+      uy[o] = uy_ym[i + (long long)g.nx * k];
+      uz[o - dn] = b.uzb_m * 2.0 - uz[o];
+    }
+    if (j == g.ny) {
+      p[o + dn] = p[o];
+      if (b.flow_type == 1) {
+        ux[o + dn] = b.uxb_p * 2.0 - ux[o];
+        uy[o + dn] = b.uyb_p;
+        uz[o + dn] = b.uzb_p * 2.0 - uz[o];
+      } else {
+        ux[o + dn] = ux[o];
+        uy[o + dn] = 0.0;
+        uz[o + dn] = uz[o];
+      }
+    }
+  }
 }
 
 // ---- wall boundary conditions (m_BC_and_Halo) on the (i, k) plane ---------
 // mode 0: SetBC_and_UpdateHalo, 1: ...ForPrSrc, 2: _pr, 3: _scalar, 4: Update_uy_ym
-struct BcArgs {
-  int mode, flow_type;
-  double uxb_m, uxb_p, uyb_p, uzb_m, uzb_p;
-  double r1m, r2m, r1p, r2p;
-};
 
 __global__ void k_bc(Geo g, BcArgs b, double *__restrict__ a0, double *__restrict__ a1, double *__restrict__ a2,
                      double *__restrict__ pl0, double *__restrict__ pl1) {
@@ -844,10 +871,30 @@ int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *u
   return PTC_OK;
 }
 
+static BcArgs vel_bc_args(const ptc_ctx *c, int mode) {
+  BcArgs b{};
+  b.mode = mode;
+  b.flow_type = c->prm.flow_type;
+  b.uxb_m = c->prm.uxbc[2];
+  b.uxb_p = c->prm.uxbc[3];
+  b.uyb_p = c->prm.uybc[3];
+  b.uzb_m = c->prm.uzbc[2];
+  b.uzb_p = c->prm.uzbc[3];
+  return b;
+}
+
 int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p) {
   const int dv = (ux && uy && uz) ? 1 : 0, dp = p ? 1 : 0;
   hipLaunchKernelGGL(k_vel_p_update, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, phi, ux, uy, uz, p, c->pmAlpha,
-                     c->pmBeta, dv, dp);
+                     c->pmBeta, dv, dp, 0, BcArgs{}, nullptr);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
+int launch_vel_p_update_bc(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p,
+                           const double *uy_ym) {
+  hipLaunchKernelGGL(k_vel_p_update, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, phi, ux, uy, uz, p, c->pmAlpha,
+                     c->pmBeta, 1, 1, 1, vel_bc_args(c, 5), uy_ym);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -883,17 +930,6 @@ int launch_bc_vel(ptc_ctx *c, double *ux, double *uy, double *uz, const double *
 
 // the velocity wall fills and the pressure's Neumann ghosts in one launch
 // (the end of a substep, src/m_ChannelSystem.f90:135-136)
-int launch_bc_vel_pr(ptc_ctx *c, double *ux, double *uy, double *uz, const double *uy_ym, double *p) {
-  BcArgs b{};
-  b.mode = 5;
-  b.flow_type = c->prm.flow_type;
-  b.uxb_m = c->prm.uxbc[2];
-  b.uxb_p = c->prm.uxbc[3];
-  b.uyb_p = c->prm.uybc[3];
-  b.uzb_m = c->prm.uzbc[2];
-  b.uzb_p = c->prm.uzbc[3];
-  return launch_bc(c, b, ux, uy, uz, const_cast<double *>(uy_ym), p);
-}
 int launch_bc_pr(ptc_ctx *c, double *p) {
   BcArgs b{};
   b.mode = 2;

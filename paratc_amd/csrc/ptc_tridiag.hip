@@ -524,7 +524,21 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
     const double v = rhs[col + (long long)nx * (min(s + r, ny - 1) + 1)];
     f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   }
-  if (kind == 1 && s == 0) f[0] = valid ? J.duy_ym[i + (long long)nx * k] : 0.0;  // du at the wall row
+  // du at the wall row: duy_ym, or with Update_uy_ym folded in (src/m_FlowCase.f90:112-122)
+  // the new uy_ym (zero) minus the old one, both planes written by this thread
+  double wall_new = 0.0;
+  if (kind == 1 && s == 0 && valid) {
+    const long long q = i + (long long)nx * k;
+    if (J.uym_upd) {
+      const double old = J.uym_upd[q];
+      J.uym_upd[q] = 0.0;
+      J.duym_upd[q] = wall_new - old;
+      f[0] = wall_new - old;
+    } else {
+      f[0] = J.duy_ym[q];
+    }
+  }
+  if (kind == 1 && s == 0 && !valid) f[0] = 0.0;
   CnRows getc{met + J.rA * nyh, met + J.rB * nyh, met + J.rC * nyh, J.beta, ny, kind};
   double A[MM], C[MM];
   double aS, cS, aE, cE, fS, fE, xs, xe;
@@ -532,12 +546,12 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
   solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
   chunk_backsub<MM>(f, A, C, xs, xe);
   if (valid) {
-    const bool wall = kind == 1 && J.wall != nullptr;
+    const bool wall = kind == 1 && (J.wall != nullptr || J.uym_upd != nullptr);
 #pragma unroll
     for (int r = 0; r < MM; ++r)
       if (s + r < ny) {
         const long long o = col + (long long)nx * (s + r + 1);
-        u[o] = (wall && s + r == 0) ? J.wall[i + (long long)nx * k] : u[o] + f[r];
+        u[o] = (wall && s + r == 0) ? (J.uym_upd ? wall_new : J.wall[i + (long long)nx * k]) : u[o] + f[r];
       }
     if (wall && s <= ny - 1 && ny - 1 < s + MM) u[col + (long long)nx * (ny + 1)] = J.top;
   }
@@ -736,6 +750,8 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
   CnJob J{};
   J.wall = nullptr;
   J.top = 0.0;
+  J.uym_upd = nullptr;
+  J.duym_upd = nullptr;
   J.u = u;
   J.rhs = rhs;
   J.duy_ym = duy_ym;
