@@ -196,10 +196,19 @@ int channel_step(ptc_ctx *c, double *mon) {
       cn.j[cn.n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
       PTC_TRY(launch_tri_cn_jobs(c, cn));
     }
-    // clcPrSrc; its divergence maximum is reduced only where it is read (the
-    // last substep's, reported by the monitors)
-    PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr, ns == c->iadvance));
-    PTC_TRY(poisson_solve(c, a1, hl, true));  // with SetBC_and_UpdateHalo_pr(prphiHalo) (:126)
+    // clcPrSrc and clcPPE (:124-126), with SetBC_and_UpdateHalo_pr(prphiHalo):
+    // the divergence is formed in the Poisson x pass where the plan allows,
+    // else written to RealArr1 first.  Its maximum is reduced only where it is
+    // read (the last substep's, reported by the monitors).
+    if (poisson_div_fusable(c)) {
+      PTC_TRY(stencil_prepare(c));  // partials for the x pass's workgroups (already sized under capture)
+      const DivSrc dv{ux, uy, uz, 1.0 / c->pmAlpha, c->d_red};
+      PTC_TRY(poisson_solve_div(c, dv, hl, true));
+      if (ns == c->iadvance) PTC_TRY(launch_divmax(c, (int)fft2_x_fwd_blocks(c, c->ny)));
+    } else {
+      PTC_TRY(launch_prsrc(c, ux, uy, uz, a1, nullptr, ns == c->iadvance));
+      PTC_TRY(poisson_solve(c, a1, hl, true));
+    }
     // FluidVelUpdate, PressureUpdate, then SetBC_and_UpdateHalo and
     // SetBC_and_UpdateHalo_pr (:129-136) in one launch
     PTC_TRY(launch_vel_p_update_bc(c, hl, ux, uy, uz, p, uym));

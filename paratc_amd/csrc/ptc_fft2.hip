@@ -388,10 +388,16 @@ constexpr int f2_xblock() {
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
-template <int N, typename S, bool STREAM>
+// DIV: the rows are not read from src but formed from the velocities, as
+// clcPrSrc does (src/m_TScheme.f90:48-77): sudtal times the divergence of
+// (ux, uy, uz) with the 4th-order staggered x and z differences and the 2nd-
+// order y difference, in k_prsrc's operand order.  Each workgroup then writes
+// the maximum |divergence| over its rows to dv.part[blockIdx.x].
+template <int N, typename S, bool STREAM, bool DIV>
 __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
-                                                    int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
+                                                    int ny, int nz, int j0, int nj, int pitch, long long rowlen,
+                                                    DivSrc dv, Geo geo) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = f2::padded(N);
@@ -406,24 +412,70 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
   const long long sq = jl + (long long)ny * k;  // spectral row S[k][jl]
   const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   double2 *ls = sh + (l < L ? l : 0) * PN;
-  auto load0 = [&](int m) { return live ? f2::ld2x<STREAM>(row[m]) : make_double2(0.0, 0.0); };
+  double dmax = 0.0;
+  auto load_div = [&](int m) {
+    // the pair of points (2m, 2m+1) of row j = jl + 1, plane k
+    const long long rj = rowlen * (jl + 1 + (long long)(ny + 2) * k);
+    const double2 *uxr = reinterpret_cast<const double2 *>(dv.ux + rj);
+    const double2 *uyr = reinterpret_cast<const double2 *>(dv.uy + rj);
+    const int mm = m > 0 ? m - 1 : N - 1, mp = m + 1 < N ? m + 1 : 0;
+    const double2 a = uxr[mm], b = uxr[m], cc = uxr[mp];
+    const double2 y0 = uyr[m], y1 = uyr[m + N];
+    const long long pl = rowlen * (ny + 2);
+    double2 z[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      long long kk = k - 1 + s;
+      kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+      z[s] = reinterpret_cast<const double2 *>(dv.uz + rj + (kk - k) * pl)[m];
+    }
+    const double rdyp = geo.met[M_RDYP * geo.nyh + jl + 1];
+    const double dudx0 = geo.dxc1 * a.y + geo.dxc2 * b.x + geo.dxc3 * b.y + geo.dxc4 * cc.x;
+    const double dudx1 = geo.dxc1 * b.x + geo.dxc2 * b.y + geo.dxc3 * cc.x + geo.dxc4 * cc.y;
+    const double dvdy0 = (y1.x - y0.x) * rdyp, dvdy1 = (y1.y - y0.y) * rdyp;
+    const double dwdz0 = geo.dzc1 * z[0].x + geo.dzc2 * z[1].x + geo.dzc3 * z[2].x + geo.dzc4 * z[3].x;
+    const double dwdz1 = geo.dzc1 * z[0].y + geo.dzc2 * z[1].y + geo.dzc3 * z[2].y + geo.dzc4 * z[3].y;
+    const double r0 = dudx0 + dvdy0 + dwdz0, r1 = dudx1 + dvdy1 + dwdz1;
+    dmax = fmax(dmax, fmax(fabs(r0), fabs(r1)));
+    return make_double2(dv.sudtal * r0, dv.sudtal * r1);
+  };
+  auto load0 = [&](int m) {
+    if (!live) return make_double2(0.0, 0.0);
+    if constexpr (DIV) return load_div(m);
+    else return f2::ld2x<STREAM>(row[m]);
+  };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
   __syncthreads();
-  if (!live) return;
-  S *out = spec + sq * pitch;
-  for (int kx = g; kx < pitch; kx += G) {
-    double2 X = make_double2(0.0, 0.0);
-    if (kx <= N) {
-      const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
-      double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
-      zc.y = -zc.y;
-      const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
-      const double2 wd = f2::cmul(wx[kx], d);
-      X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
+  if (live) {
+    S *out = spec + sq * pitch;
+    for (int kx = g; kx < pitch; kx += G) {
+      double2 X = make_double2(0.0, 0.0);
+      if (kx <= N) {
+        const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
+        double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
+        zc.y = -zc.y;
+        const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
+        const double2 wd = f2::cmul(wx[kx], d);
+        X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
+      }
+      f2::st2x<STREAM>(out[kx], X);
     }
-    f2::st2x<STREAM>(out[kx], X);
+  }
+  if constexpr (DIV) {
+    // fixed-order tree over the workgroup, in the FFT's LDS once its reads are done
+    double *red = reinterpret_cast<double *>(sh);
+    __syncthreads();
+    red[threadIdx.x] = dmax;
+    __syncthreads();
+    for (int n = blockDim.x; n > 1;) {
+      const int h = (n + 1) / 2;
+      if ((int)threadIdx.x < n - h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
+      __syncthreads();
+      n = h;
+    }
+    if (threadIdx.x == 0) dv.part[blockIdx.x] = red[0];
   }
 }
 
@@ -528,23 +580,39 @@ __global__ void __launch_bounds__(f2_threads<N>() * T) k2_fft_z(S *__restrict__ 
 }
 
 // ---- dispatch ----------------------------------------------------------------
-template <int N, typename S, bool STREAM>
-static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj) {
+template <int N>
+static long long x_fwd_blocks(long long nlines) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  const long long nlines = (long long)nj * c->nz;
-  const unsigned nblk = (unsigned)((nlines + L - 1) / L);
-  hipLaunchKernelGGL((k2_fft_x_fwd<N, S, STREAM>), dim3(nblk), dim3(L * G), 0, c->stream, src, spec, c->pp.fx.tw,
-                     c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx);
+  return (nlines + L - 1) / L;
+}
+
+template <int N, typename S, bool STREAM, bool DIV>
+static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj, const DivSrc &dv) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  const unsigned nblk = (unsigned)x_fwd_blocks<N>((long long)nj * c->nz);
+  hipLaunchKernelGGL((k2_fft_x_fwd<N, S, STREAM, DIV>), dim3(nblk), dim3(L * G), 0, c->stream, src, spec,
+                     c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, dv, c->geo);
 }
 
 template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64) {
+static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div) {
   const bool f32 = !f64 && c->pp.prec == PTC_PREC_FP32_REFINE;
-  if (f32 && c->pp.stream) x_fwd_launch<N, float2, true>(c, src, (float2 *)spec, j0, nj);
-  else if (f32) x_fwd_launch<N, float2, false>(c, src, (float2 *)spec, j0, nj);
-  else if (c->pp.stream) x_fwd_launch<N, double2, true>(c, src, (double2 *)spec, j0, nj);
-  else x_fwd_launch<N, double2, false>(c, src, (double2 *)spec, j0, nj);
+  const DivSrc none{};
+  if (div) {
+    PTC_CHECK(!f32, PTC_E_STATE, "the divergence-forming x pass needs FP64 spectra");
+    if (c->pp.stream) x_fwd_launch<N, double2, true, true>(c, src, (double2 *)spec, j0, nj, *div);
+    else x_fwd_launch<N, double2, false, true>(c, src, (double2 *)spec, j0, nj, *div);
+  } else if (f32 && c->pp.stream) {
+    x_fwd_launch<N, float2, true, false>(c, src, (float2 *)spec, j0, nj, none);
+  } else if (f32) {
+    x_fwd_launch<N, float2, false, false>(c, src, (float2 *)spec, j0, nj, none);
+  } else if (c->pp.stream) {
+    x_fwd_launch<N, double2, true, false>(c, src, (double2 *)spec, j0, nj, none);
+  } else {
+    x_fwd_launch<N, double2, false, false>(c, src, (double2 *)spec, j0, nj, none);
+  }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -618,12 +686,21 @@ bool fft2_has_x(int h) {
   return false;
 }
 
-int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64) {
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj, f64);
+#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj, f64, div);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;
+  }
+}
+
+long long fft2_x_fwd_blocks(const ptc_ctx *c, int nj) {
+  switch (c->pp.h) {
+#define PTC_CASE(N) case N: return x_fwd_blocks<N>((long long)nj * c->nz);
+    PTC_X_SIZES(PTC_CASE)
+#undef PTC_CASE
+    default: return 0;
   }
 }
 

@@ -185,11 +185,24 @@ namespace ptc {
 int field(ptc_ctx *c, int id, double **out);
 int check_field(ptc_ctx *c, int id);
 
+// The divergence of clcPrSrc formed in the x forward pass (src/m_TScheme.f90:48-77):
+// the pass transforms sudtal * div(ux, uy, uz) and writes each workgroup's
+// maximum |div| to part[workgroup].
+struct DivSrc {
+  const double *ux, *uy, *uz;
+  double sudtal;
+  double *part;
+};
+
 // Poisson
 int poisson_build_plan(ptc_ctx *c);
 int poisson_free_plan(ptc_ctx *c);
 // ghosts: also fill dst's Neumann ghost rows j = 0, ny+1 (SetBC_and_UpdateHalo_pr)
 int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts = false);
+// clcPrSrc + clcPPE with the source formed in the x pass (FP64 plans with the
+// specialised x kernel and no slabs: poisson_div_fusable)
+bool poisson_div_fusable(ptc_ctx *c);
+int poisson_solve_div(ptc_ctx *c, const DivSrc &div, double *dst, bool ghosts);
 int poisson_prepare(ptc_ctx *c);
 int stencil_prepare(ptc_ctx *c);
 // one ChannelIterate step (monitors into mon[7] when mon != NULL)
@@ -228,8 +241,12 @@ bool fft2_has_z(int nz, int req, int *tile);
 // specialised kernels over the planes j0 .. j0+nj-1 (0-based) of the spectrum;
 // the spectrum holds double2 or, in the FP32-storage form, float2 values;
 // acc: the x-inverse adds its result to dst
-// f64: FP64 spectra whatever the plan's storage precision (the statistics spectra)
-int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64 = false);
+// f64: FP64 spectra whatever the plan's storage precision (the statistics spectra);
+// div: form the rows from the velocities instead of reading src
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64 = false,
+               const DivSrc *div = nullptr);
+// workgroups of one specialised x forward launch over nj planes (0: no such kernel)
+long long fft2_x_fwd_blocks(const ptc_ctx *c, int nj);
 int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost);
 int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64 = false);
 
@@ -266,6 +283,8 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
                double *hx, double *hy, double *hz, double *hc, const double *p);
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host, bool reduce = true);
+// the divergence maximum over n partials into its device slot
+int launch_divmax(ptc_ctx *c, int n);
 int launch_vel_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz);
 int launch_pressure_update(ptc_ctx *c, double *p, const double *phi);
 int launch_vel_p_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz, double *p);

@@ -219,10 +219,15 @@ int poisson_prepare(ptc_ctx *c) {
 
 // One pass sequence src -> spectrum -> dst (acc: dst += result); ev (may be
 // NULL) receives six events around the five passes.
-static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, bool ghosts, hipEvent_t *ev) {
+static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, bool ghosts, hipEvent_t *ev,
+                          const DivSrc *div = nullptr) {
   PoissonPlan &P = c->pp;
   if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
-  if (P.slab == 0) {
+  if (div) {
+    PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, 0, c->ny, false, div));
+    if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
+    PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  } else if (P.slab == 0) {
     PTC_TRY(launch_fft_x_fwd(c, src, P.d_spec));
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
     PTC_TRY(launch_fft_z(c, P.d_spec, false));
@@ -283,6 +288,32 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts) {
     }
   }
   return PTC_OK;
+}
+
+bool poisson_div_fusable(ptc_ctx *c) {
+  if (poisson_build_plan(c) != PTC_OK) return false;
+  const PoissonPlan &P = c->pp;
+  return P.x_fast && P.slab == 0 && P.prec == PTC_PREC_FP64;
+}
+
+int poisson_solve_div(ptc_ctx *c, const DivSrc &div, double *dst, bool ghosts) {
+  PTC_TRY(poisson_build_plan(c));
+  PoissonPlan &P = c->pp;
+  PTC_CHECK(P.x_fast && P.slab == 0 && P.prec == PTC_PREC_FP64, PTC_E_STATE,
+            "the divergence-forming solve needs an FP64 plan with the specialised x pass and no slabs");
+  PTC_TRY(ensure_thomas_scratch(c));
+  hipEvent_t *ev = nullptr;
+  if (P.profile) {
+    const size_t need = (size_t)(P.prof_used + 1) * 6;
+    while (P.prof_ev.size() < need) {
+      hipEvent_t e;
+      PTC_HIP(hipEventCreate(&e));
+      P.prof_ev.push_back(e);
+    }
+    ev = P.prof_ev.data() + (size_t)P.prof_used * 6;
+    ++P.prof_used;
+  }
+  return solve_sequence(c, nullptr, dst, false, ghosts, ev, &div);
 }
 
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves) {

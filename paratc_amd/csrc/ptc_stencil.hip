@@ -848,7 +848,16 @@ const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
 
 // reduction partials of the RHS and PrSrc kernels, allocated ahead of a
 // captured step
-int stencil_prepare(ptc_ctx *c) { return ensure_red(c, std::max(pt_nblocks(c), rhs_nblocks(c))); }
+int stencil_prepare(ptc_ctx *c) {
+  const int nx = (int)fft2_x_fwd_blocks(c, c->ny);
+  return ensure_red(c, std::max({pt_nblocks(c), rhs_nblocks(c), nx}));
+}
+
+int launch_divmax(ptc_ctx *c, int n) {
+  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, n, c->d_scal + S_DIVMAX);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
 
 // reduce: form the divergence maximum in the device slot (always when it is
 // read back on the host)
