@@ -370,7 +370,9 @@ def bench_rk_substep(ev, steps=30, warmup=3):
         "workload": "closed channel 128x96x128, RK3, dt 2e-3, FP64",
         "mpts_per_s": round(case.npts / (t_sub * 1e-3) / 1e6, 2),
         "ms_per_substep": round(t_sub, 4),
-        "hbm_gbs_alg": round(368.0 * case.npts / (t_sub * 1e-3) / 1e9, 1),
+        # 344 B per point: RHS 104, three CN solves 72, Poisson with the
+        # divergence formed in its x pass 96, velocity/pressure update 72
+        "hbm_gbs_alg": round(344.0 * case.npts / (t_sub * 1e-3) / 1e9, 1),
         "divmax_after": float(mon[1]),
     }
 
