@@ -763,11 +763,16 @@ constexpr int kRhsRowsMaxNx = 512;
 constexpr int kRhsSegThreads = 256;
 constexpr int kRhsSeg = kRhsSegThreads - 6;
 static int rhs_seg(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx ? 0 : kRhsSeg; }
+// z chunks: enough that the grid holds about 48 waves per CU (at ~200 VGPRs
+// two workgroups are resident per CU, so a grid of only a few rounds of them
+// leaves a long partial last round: at 4608x140x700 one 700-plane chunk per
+// row ran 54.1 ms per substep, two 350-plane chunks 52.8 ms), at least 16
+// planes per chunk (each chunk re-reads a 6-plane window prologue)
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
   const int seg = rhs_seg(c);
   const int nseg = seg ? (c->nx + seg - 1) / seg : 1;
   const int wpb = seg ? kRhsSegThreads / 64 : (c->nx + 63) / 64;
-  const int want = (24 * c->n_cu + c->ny * nseg * wpb - 1) / (c->ny * nseg * wpb);
+  const int want = (48 * c->n_cu + c->ny * nseg * wpb - 1) / (c->ny * nseg * wpb);
   const int nch = std::max(1, std::min(want, c->nz / 16));
   *kc = (c->nz + nch - 1) / nch;
   return dim3((unsigned)nseg, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
