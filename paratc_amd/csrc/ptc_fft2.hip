@@ -375,6 +375,19 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
 
 }  // namespace f2
 
+// Workgroups go to the 8 XCDs round-robin by linear id (w -> XCD w % 8), and
+// each XCD has its own L2.  xcd_group maps w to a work index so that GRP
+// consecutive work indices run on one XCD as consecutive workgroups there;
+// the last (nwg mod 8 GRP) ids keep their own index.
+template <int GRP>
+__device__ inline int xcd_group(int w, int nwg) {
+  if (GRP == 1) return w;
+  constexpr int span = 8 * GRP;
+  if (w >= (nwg / span) * span) return w;
+  const int x = w % 8, s = w / 8;
+  return GRP * (x + 8 * (s / GRP)) + s % GRP;
+}
+
 // threads per line: the largest-radix stage has one butterfly per thread
 template <int N>
 constexpr int f2_threads() {
@@ -403,12 +416,19 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
   constexpr int PN = f2::padded(N);
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
-  // lines (j, k) of the planes j0 .. j0+nj-1 (0-based), all k
+  // lines (j, k) of the planes j0 .. j0+nj-1 (0-based), all k.  Plain rows go
+  // j fastest.  DIV reads the uz rows of planes k-1 .. k+2 and the uy rows j,
+  // j+1 of each line: its lines go k fastest, runs of consecutive k on one XCD
+  // (xcd_group), so the neighbouring lines that re-read a uz row do so through
+  // the same L2.
   const long long nlines = (long long)nj * nz;
-  const long long q = (long long)blockIdx.x * L + l;
+  const long long wb = DIV ? (long long)xcd_group<(L == 1 ? 32 : 4)>((int)blockIdx.x, (int)gridDim.x)
+                           : (long long)blockIdx.x;
+  const long long q = wb * L + l;
   const bool live = l < L && q < nlines;
   const long long qq = live ? q : 0;
-  const long long jl = j0 + qq % nj, k = qq / nj;
+  const long long jl = DIV ? j0 + qq / nz : j0 + qq % nj;
+  const long long k = DIV ? qq % nz : qq / nj;
   const long long sq = jl + (long long)ny * k;  // spectral row S[k][jl]
   const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   double2 *ls = sh + (l < L ? l : 0) * PN;
@@ -543,19 +563,6 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
     if (grow) f2::st2x<STREAM>(grow[m], x);
   };
   f2::run_fft<N, G, true, true>(g, tw, load0, lds, store_out);
-}
-
-// Workgroups go to the 8 XCDs round-robin by linear id (w -> XCD w % 8), and
-// each XCD has its own L2.  xcd_group maps w to a work index so that GRP
-// consecutive work indices run on one XCD as consecutive workgroups there;
-// the last (nwg mod 8 GRP) ids keep their own index.
-template <int GRP>
-__device__ inline int xcd_group(int w, int nwg) {
-  if (GRP == 1) return w;
-  constexpr int span = 8 * GRP;
-  if (w >= (nwg / span) * span) return w;
-  const int x = w % 8, s = w / 8;
-  return GRP * (x + 8 * (s / GRP)) + s % GRP;
 }
 
 // ---- z pass: T columns x N rows, thread id = g * T + col ----------------
