@@ -307,7 +307,10 @@ int ptc_get_spectra(ptc_ctx *ctx, double *specx, double *specz, double *spec2d, 
  * point from the first, as decomp_2d_write_every keeps them with from1
  * (src/m_Decomp2d.f90:1232-1389), x fastest, as real_bytes = 4 or 8 reals
  * (mytype_save) into host_out of count = ceil(nx/iskip) ceil(ny/jskip)
- * ceil(nz/kskip) values.  ux carries uCRF; PTC_VISU_WMAG is |vorticity|. */
+ * ceil(nz/kskip) values.  ux carries uCRF; PTC_VISU_WMAG is |vorticity|.
+ * PTC_VISU_Q is Clc_Q_vor (src/m_Tools.f90:307-342), PTC_VISU_LAMBDA2
+ * Clc_lamda2 with CubicRoot (:347-442, :494-570); where the reference aborts
+ * with "lamda2 wrong" (a complex root pair) this returns PTC_E_LIMIT. */
 enum {
   PTC_VISU_UX = 0,
   PTC_VISU_UY = 1,
@@ -317,7 +320,9 @@ enum {
   PTC_VISU_WY = 5,
   PTC_VISU_WZ = 6,
   PTC_VISU_WMAG = 7,
-  PTC_VISU_SCALAR = 8
+  PTC_VISU_SCALAR = 8,
+  PTC_VISU_Q = 9,
+  PTC_VISU_LAMBDA2 = 10
 };
 int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
                    long long count);

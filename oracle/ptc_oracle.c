@@ -1358,6 +1358,44 @@ void orc_channel_iterate(orc_ctx *c, int poisson_choice,
  * from1 = .true. (src/m_Decomp2d.f90:1232-1389): i = 1, 1+iskip, ... (and the
  * same in j and k), written x fastest.  The velocity, pressure and scalar
  * arrays are haloed (x/z halos current, y ghost rows set). */
+/* CubicRoot (src/m_Tools.f90:494-570): the roots of x^3 + b x^2 + c x + d.
+ * Returns 1 with root[] in the reference's order when they are all real (for
+ * three distinct roots root[1] is the middle one), 0 for a complex pair. */
+static int cubic_root(double b, double c, double d, double root[3]) {
+  const double SMALL = 1.0e-12, third = 0.3333333333333333333333, pi = 3.141592653589793238462643383279502884;
+  const double p = c - third * b * b;
+  const double q = (2.0 * b * b * b - 9.0 * b * c + 27.0 * d) / 27.0;
+  double t = q * q / 4.0 + p * p * p / 27.0;
+  if (t < 0.0) {
+    t = -third * b;
+    double rho = pow(-p, 1.5) / sqrt(27.0);
+    const double eta = acos(0.5 * q / rho);
+    rho = -2.0 * pow(rho, third);
+    const double r1 = cos(third * eta) * rho + t;
+    const double r2 = cos(third * (2.0 * pi + eta)) * rho + t;
+    const double r3 = cos(third * (4.0 * pi + eta)) * rho + t;
+    double o[3];
+    if (r1 < r2 && r1 < r3) {
+      o[0] = r1; o[1] = r2 < r3 ? r2 : r3; o[2] = r2 < r3 ? r3 : r2;
+    } else if (r1 > r2 && r1 > r3) {
+      o[0] = r2 < r3 ? r2 : r3; o[1] = r2 < r3 ? r3 : r2; o[2] = r1;
+    } else {
+      o[0] = r2 < r3 ? r2 : r3; o[1] = r1; o[2] = r2 < r3 ? r3 : r2;
+    }
+    root[0] = o[0]; root[1] = o[1]; root[2] = o[2];
+    return 1;
+  }
+  if (t < SMALL) {
+    double rho = 0.5 * q;
+    rho = rho < 0.0 ? pow(-rho, third) : -pow(rho, third);
+    root[0] = 2.0 * rho - third * b;
+    root[1] = -rho - third * b;
+    root[2] = root[1];
+    return 1;
+  }
+  return 0;
+}
+
 static double visu_point(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz,
                          const double *pr, const double *sc, int ic, int jc, int kc) {
   const double half = 0.5, quarter = 0.25;
@@ -1384,6 +1422,32 @@ static double visu_point(const orc_ctx *c, int which, const double *ux, const do
                        (H(c, uz, ic, jc, kp) + H(c, uz, ic, jc, kc)) * cac12 -
                        (H(c, uz, ic, jm, kp) + H(c, uz, ic, jm, kc)) * cac1) * quarter;
   const double vor_x = dwdy - dvdz, vor_y = dudz - dwdx, vor_z = dvdx - dudy;
+  if (which == 9 || which == 10) {
+    const double dudx = (H(c, ux, ip, jc, kc) - H(c, ux, ic, jc, kc)) * rdx;
+    const double dvdy = (H(c, uy, ic, jp, kc) - H(c, uy, ic, jc, kc)) * c->rdyp[jc];
+    const double dwdz = (H(c, uz, ic, jc, kp) - H(c, uz, ic, jc, kc)) * rdz;
+    if (which == 9) { /* Clc_Q_vor, src/m_Tools.f90:307-342 */
+      const double udiv = dudx + dvdy + dwdz;
+      return half * (udiv * udiv - dudx * dudx - dvdy * dvdy - dwdz * dwdz) - dudy * dvdx - dudz * dwdx - dvdz * dwdy;
+    }
+    /* Clc_lamda2, src/m_Tools.f90:347-442: the tensors carry a factor 2, so
+     * the root is scaled by a quarter; a complex pair (the reference aborts
+     * with "lamda2 wrong") gives NaN here */
+    const double o12 = dvdx - dudy, o23 = dwdy - dvdz, o31 = dudz - dwdx;
+    const double e11 = dudx + dudx, e12 = dudy + dvdx, e13 = dudz + dwdx;
+    const double e22 = dvdy + dvdy, e23 = dvdz + dwdy, e33 = dwdz + dwdz;
+    const double l12 = e11 * e12 + e12 * e22 + e13 * e23 + o31 * o23;
+    const double l13 = e11 * e13 + e12 * e23 + e13 * e33 + o12 * o23;
+    const double l23 = e12 * e13 + e22 * e23 + e23 * e33 + o12 * o31;
+    const double l11 = e11 * e11 + e12 * e12 + e13 * e13 - o12 * o12 - o31 * o31;
+    const double l22 = e12 * e12 + e22 * e22 + e23 * e23 - o12 * o12 - o23 * o23;
+    const double l33 = e13 * e13 + e23 * e23 + e33 * e33 - o31 * o31 - o23 * o23;
+    const double rb = -(l11 + l22 + l33);
+    const double rc = l11 * l22 + l22 * l33 + l33 * l11 - l12 * l12 - l13 * l13 - l23 * l23;
+    const double rd = l12 * l12 * l33 + l13 * l13 * l22 + l23 * l23 * l11 - l11 * l22 * l33 - 2.0 * l12 * l23 * l13;
+    double root[3];
+    return cubic_root(rb, rc, rd, root) ? quarter * root[1] : NAN;
+  }
   switch (which) {
     case 4: return vor_x;
     case 5: return vor_y;

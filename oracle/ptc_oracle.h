@@ -110,7 +110,9 @@ void orc_clc_stat(const orc_ctx *c, const double *ux, const double *uy, const do
 /* dump_visu's cell-centre fields (src/m_IOAndVisu.f90:182-385) on the points
  * decomp_2d_write_every keeps (every iskip/jskip/kskip-th, from the first),
  * x fastest: which = 0 ux (+uCRF), 1 uy, 2 uz, 3 pr, 4 wx, 5 wy, 6 wz,
- * 7 |w|, 8 scalar.  orc_visu_count: the number of points written. */
+ * 7 |w|, 8 scalar, 9 Q (Clc_Q_vor, src/m_Tools.f90:307-342), 10 lambda2
+ * (Clc_lamda2 with CubicRoot, :347-442, :494-570; NaN where the reference
+ * aborts on a complex root pair).  orc_visu_count: the number of points. */
 long long orc_visu_count(const orc_ctx *c, int iskip, int jskip, int kskip);
 void orc_dump_visu(const orc_ctx *c, int which, const double *ux, const double *uy, const double *uz, const double *pr,
                    const double *sc, int iskip, int jskip, int kskip, double *out);

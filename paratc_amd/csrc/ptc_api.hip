@@ -816,7 +816,7 @@ int ptc_get_spectra(ptc_ctx *ctx, double *specx, double *specz, double *spec2d, 
 int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
                    long long count) {
   PTC_CTX(ctx);
-  PTC_CHECK(which >= PTC_VISU_UX && which <= PTC_VISU_SCALAR, PTC_E_ARG, "unknown visualisation field");
+  PTC_CHECK(which >= PTC_VISU_UX && which <= PTC_VISU_LAMBDA2, PTC_E_ARG, "unknown visualisation field");
   PTC_CHECK(iskip >= 1 && jskip >= 1 && kskip >= 1, PTC_E_ARG, "skips must be >= 1");
   PTC_CHECK(real_bytes == 4 || real_bytes == 8, PTC_E_ARG, "visualisation reals are 4 or 8 bytes");
   PTC_CHECK(host_out != nullptr, PTC_E_ARG, "null output");
@@ -839,9 +839,15 @@ int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int
     PTC_HIP(hipMalloc(&ctx->d_visu, bytes));
     ctx->visu_bytes = bytes;
   }
+  if (which == PTC_VISU_LAMBDA2) PTC_TRY(fill_device(ctx, ctx->d_scal + kSlotVisuErr, 1, 0.0));
   PTC_TRY(launch_visu(ctx, which, ux, uy, uz, p, sc, iskip, jskip, kskip, real_bytes, ctx->d_visu));
   PTC_HIP(hipMemcpyAsync(host_out, ctx->d_visu, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (which == PTC_VISU_LAMBDA2)
+    PTC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_scal + kSlotVisuErr, sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
   PTC_HIP(hipStreamSynchronize(ctx->stream));
+  // Clc_lamda2 aborts when a point's cubic has a complex root pair (src/m_Tools.f90:432-436)
+  PTC_CHECK(which != PTC_VISU_LAMBDA2 || ctx->h_pinned[0] == 0.0, PTC_E_LIMIT, "Clc_lamda2: lamda2 wrong");
   return PTC_OK;
 }
 

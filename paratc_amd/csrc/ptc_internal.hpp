@@ -210,6 +210,8 @@ int channel_step(ptc_ctx *c, double *mon);
 // device scalar holding the x-z mean of phi's first row before a correction
 // (the refinement's x-inverse subtracts it, keeping the (0,0) pin)
 constexpr int kSlotPinShift = 4;
+// set to 1 by the visualisation kernel where lambda2's cubic has a complex root pair
+constexpr int kSlotVisuErr = 32;
 
 // fill n doubles of device memory with v on the context's stream
 int fill_device(ptc_ctx *c, double *a, long long n, double v);

@@ -641,12 +641,43 @@ __global__ void k_bc(Geo g, BcArgs b, double *__restrict__ a0, double *__restric
 // 2 uz, 3 p, 4-6 the vorticity components, 7 its magnitude, 8 the scalar.
 // One thread per written point: x from the block's x index, the kept y row and
 // z plane from the grid's y and z.
+// CubicRoot (src/m_Tools.f90:494-570) for x^3 + b x^2 + c x + d, returning its
+// Root(2): the middle root of three distinct real roots (the reference's
+// ordering branches reduce to that), the repeated root when two coincide
+// (0 <= t < 1e-12).  false: a complex pair.
+__device__ bool lambda2_middle_root(double b, double c, double d, double &mid) {
+  const double SMALL = 1.0e-12, third = 0.3333333333333333333333, pi = 3.141592653589793238462643383279502884;
+  const double p = c - third * b * b;
+  const double q = (2.0 * b * b * b - 9.0 * b * c + 27.0 * d) / 27.0;
+  const double t = q * q / 4.0 + p * p * p / 27.0;
+  if (t < 0.0) {
+    const double sh = -third * b;
+    double rho = pow(-p, 1.5) / sqrt(27.0);
+    const double eta = acos(0.5 * q / rho);
+    rho = -2.0 * pow(rho, third);
+    const double r1 = cos(third * eta) * rho + sh;
+    const double r2 = cos(third * (2.0 * pi + eta)) * rho + sh;
+    const double r3 = cos(third * (4.0 * pi + eta)) * rho + sh;
+    if (r1 < r2 && r1 < r3) mid = r2 < r3 ? r2 : r3;
+    else if (r1 > r2 && r1 > r3) mid = r2 < r3 ? r3 : r2;
+    else mid = r1;
+    return true;
+  }
+  if (t < SMALL) {
+    double rho = 0.5 * q;
+    rho = rho < 0.0 ? pow(-rho, third) : -pow(rho, third);
+    mid = -rho - third * b;
+    return true;
+  }
+  return false;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(64) k_visu(Geo g, int which, const double *__restrict__ ux,
                                              const double *__restrict__ uy, const double *__restrict__ uz,
                                              const double *__restrict__ p, const double *__restrict__ sc,
                                              double ucrf, int iskip, int jskip, int kskip, int nxo,
-                                             R *__restrict__ out) {
+                                             R *__restrict__ out, double *__restrict__ err) {
   const int io = blockIdx.x * 64 + threadIdx.x, jo = blockIdx.y, ko = blockIdx.z;
   if (io >= nxo) return;
   const int nx = g.nx, nz = g.nz;
@@ -681,7 +712,37 @@ __global__ void __launch_bounds__(64) k_visu(Geo g, int which, const double *__r
     const double dwdy = ((at(uz, i, jp, kp) + at(uz, i, jp, k)) * cac2 + (at(uz, i, j, kp) + at(uz, i, j, k)) * cac12 -
                          (at(uz, i, jm, kp) + at(uz, i, jm, k)) * cac1) * quarter;
     const double vx = dwdy - dvdz, vy = dudz - dwdx, vz = dvdx - dudy;
-    v = which == 4 ? vx : which == 5 ? vy : which == 6 ? vz : sqrt(vx * vx + vy * vy + vz * vz);
+    if (which <= 7) {
+      v = which == 4 ? vx : which == 5 ? vy : which == 6 ? vz : sqrt(vx * vx + vy * vy + vz * vz);
+    } else {
+      const double dudx = (at(ux, ip, j, k) - at(ux, i, j, k)) * rdx;
+      const double dvdy = (at(uy, i, jp, k) - at(uy, i, j, k)) * M[M_RDYP * W + j];
+      const double dwdz = (at(uz, i, j, kp) - at(uz, i, j, k)) * rdz;
+      if (which == 9) {  // Clc_Q_vor (src/m_Tools.f90:307-342)
+        const double udiv = dudx + dvdy + dwdz;
+        v = half * (udiv * udiv - dudx * dudx - dvdy * dvdy - dwdz * dwdz) - dudy * dvdx - dudz * dwdx - dvdz * dwdy;
+      } else {  // Clc_lamda2 (:347-442): strain and rotation tensors carry a factor 2
+        const double o12 = dvdx - dudy, o23 = dwdy - dvdz, o31 = dudz - dwdx;
+        const double e11 = dudx + dudx, e12 = dudy + dvdx, e13 = dudz + dwdx;
+        const double e22 = dvdy + dvdy, e23 = dvdz + dwdy, e33 = dwdz + dwdz;
+        const double l12 = e11 * e12 + e12 * e22 + e13 * e23 + o31 * o23;
+        const double l13 = e11 * e13 + e12 * e23 + e13 * e33 + o12 * o23;
+        const double l23 = e12 * e13 + e22 * e23 + e23 * e33 + o12 * o31;
+        const double l11 = e11 * e11 + e12 * e12 + e13 * e13 - o12 * o12 - o31 * o31;
+        const double l22 = e12 * e12 + e22 * e22 + e23 * e23 - o12 * o12 - o23 * o23;
+        const double l33 = e13 * e13 + e23 * e23 + e33 * e33 - o31 * o31 - o23 * o23;
+        const double rb = -(l11 + l22 + l33);
+        const double rc = l11 * l22 + l22 * l33 + l33 * l11 - l12 * l12 - l13 * l13 - l23 * l23;
+        const double rd = l12 * l12 * l33 + l13 * l13 * l22 + l23 * l23 * l11 - l11 * l22 * l33 - 2.0 * l12 * l23 * l13;
+        double mid;
+        if (lambda2_middle_root(rb, rc, rd, mid)) {
+          v = quarter * mid;
+        } else {
+          v = __builtin_nan("");
+          err[0] = 1.0;  // the reference aborts ("lamda2 wrong"); the host reports it
+        }
+      }
+    }
   }
   out[io + (long long)nxo * (jo + (long long)gridDim.y * ko)] = (R)v;
 }
@@ -692,10 +753,10 @@ int launch_visu(ptc_ctx *c, int which, const double *ux, const double *uy, const
   const dim3 grid((unsigned)((nxo + 63) / 64), (unsigned)nyo, (unsigned)nzo);
   if (real_bytes == 8)
     hipLaunchKernelGGL(k_visu<double>, grid, dim3(64), 0, c->stream, c->geo, which, ux, uy, uz, p, sc, c->prm.ucrf,
-                       iskip, jskip, kskip, nxo, (double *)dev_out);
+                       iskip, jskip, kskip, nxo, (double *)dev_out, c->d_scal + kSlotVisuErr);
   else
     hipLaunchKernelGGL(k_visu<float>, grid, dim3(64), 0, c->stream, c->geo, which, ux, uy, uz, p, sc, c->prm.ucrf,
-                       iskip, jskip, kskip, nxo, (float *)dev_out);
+                       iskip, jskip, kskip, nxo, (float *)dev_out, c->d_scal + kSlotVisuErr);
   PTC_HIP(hipGetLastError());
   return PTC_OK;
 }
@@ -758,7 +819,7 @@ static int pt_nblocks(const ptc_ctx *c) {
 // longer rows go in segments of kRhsSeg columns with a 3-column halo either
 // side (kRhsSegThreads threads: at ~200 VGPRs two such workgroups share a CU
 // and overlap each other's per-plane barriers).  The k chunks are at least 16
-// planes long and many enough for about 24 wavefronts per CU over the launch.
+// planes long and many enough for about 48 wavefronts per CU over the launch.
 constexpr int kRhsRowsMaxNx = 512;
 constexpr int kRhsSegThreads = 256;
 constexpr int kRhsSeg = kRhsSegThreads - 6;

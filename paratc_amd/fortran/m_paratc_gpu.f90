@@ -52,7 +52,7 @@ module m_paratc_gpu
   public :: ptc_init_spectra, ptc_clc_spectra, ptc_get_spectra
   integer(c_int), parameter, public :: PTC_VISU_UX = 0, PTC_VISU_UY = 1, PTC_VISU_UZ = 2, PTC_VISU_PR = 3
   integer(c_int), parameter, public :: PTC_VISU_WX = 4, PTC_VISU_WY = 5, PTC_VISU_WZ = 6, PTC_VISU_WMAG = 7
-  integer(c_int), parameter, public :: PTC_VISU_SCALAR = 8
+  integer(c_int), parameter, public :: PTC_VISU_SCALAR = 8, PTC_VISU_Q = 9, PTC_VISU_LAMBDA2 = 10
   public :: gpu_check, gpu_put, gpu_get, gpu_clcPPE
 
   integer(c_int), parameter, public :: PTC_PREC_FP64 = 0, PTC_PREC_FP32_REFINE = 1

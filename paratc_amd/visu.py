@@ -7,7 +7,8 @@ precision (src/m_IOAndVisu.f90:182-385 with decomp_2d_write_every,
 src/m_Decomp2d.f90:1232-1389).  The fields are computed on the device
 (``ptc_visu_field``).  ``write_xdmf`` writes the ``VisuFor<RunName>.xmf``
 index of InitVisu (src/m_IOAndVisu.f90:30-155) that ParaView or VisIt read.
-The vortex criteria Q and lambda2 are not offered.
+The vortex criteria are ``Q`` (Clc_Q_vor) and ``lamda2`` (Clc_lamda2), as the
+reference names their files.
 """
 from __future__ import annotations
 
@@ -21,9 +22,9 @@ from . import capi
 # attribute name -> device field id, in dump_visu's write order
 FIELDS = {"ux": capi.VISU_UX, "uy": capi.VISU_UY, "uz": capi.VISU_UZ, "pr": capi.VISU_PR,
           "scalar": capi.VISU_SCALAR, "wx": capi.VISU_WX, "wy": capi.VISU_WY, "wz": capi.VISU_WZ,
-          "wMag": capi.VISU_WMAG}
+          "wMag": capi.VISU_WMAG, "Q": capi.VISU_Q, "lamda2": capi.VISU_LAMBDA2}
 # InitVisu's attribute order in the index (:136-148)
-XDMF_ORDER = ("ux", "uy", "uz", "pr", "wx", "wy", "wz", "wMag", "scalar")
+XDMF_ORDER = ("ux", "uy", "uz", "pr", "wx", "wy", "wz", "wMag", "Q", "lamda2", "scalar")
 
 
 def visu_file(run_name: str, attr: str, ntime: int) -> str:

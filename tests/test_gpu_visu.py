@@ -1,6 +1,7 @@
 """GPU parity of the dump_visu fields (src/m_IOAndVisu.f90:182-385) with the
 oracle: cell-centre velocities (ux with uCRF), pressure, scalar, vorticity
-components and magnitude, with and without subsampling, FP64 and FP32."""
+components and magnitude, the vortex criteria Q and lambda2, with and without
+subsampling, FP64 and FP32."""
 import numpy as np
 import pytest
 
@@ -22,13 +23,15 @@ def test_visu_fields_match_oracle(kind, skip):
     nf = 9 if case.scalar else 8
     with Solver(case) as s:
         upload_state(s, ux, uy, uz, p, sc)
-        for which in range(nf):
+        for which in list(range(nf)) + [capi.VISU_Q, capi.VISU_LAMBDA2]:
             ref = orc.dump_visu(which, ux, uy, uz, p, sc, skip=skip)
             got = s.visu_field(which, skip, 8)
             assert got.shape == ref.shape
             e = rel_err(got, ref)
             print(kind, skip, which, f"{e:.2e}", flush=True)
-            assert e <= 1e-13, which
+            # lambda2 goes through acos, cos and cube roots: device and host
+            # libm differ by ulps, amplified where two eigenvalues nearly meet
+            assert e <= (1e-10 if which == capi.VISU_LAMBDA2 else 1e-13), which
             got4 = s.visu_field(which, skip, 4)
             assert got4.dtype == np.float32
             assert np.allclose(got4, ref.astype(np.float32), rtol=2e-7, atol=1e-7 * np.abs(ref).max()), which

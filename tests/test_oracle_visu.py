@@ -133,3 +133,63 @@ def test_xdmf_index_and_raw_files(tmp_path):
     data = np.fromfile(out["wz"], dtype=np.float32)
     assert data.size == 4 * 6 * 4 and np.all(data == capi.VISU_WZ)
     assert os.path.basename(out["ux"]) == "VisuForrun_ux_0000000010"
+
+
+def linear_velocity(g, G):
+    """ux, uy, uz linear in (x, y, z) with gradient G[i][j] = d u_i / d x_j,
+    each at its staggered position, halos included: every difference and
+    average of the visualisation stencils is then exact."""
+    def lin(a, b, c, fx, fy, fz):
+        return g.field(fx=a * fx) + g.field(fy=b * fy) + g.field(fz=c * fz)
+    ux = lin(*G[0], g.xf, g.yc, g.zc)
+    uy = lin(*G[1], g.xc, g.yf, g.zc)
+    uz = lin(*G[2], g.xc, g.yc, g.zf)
+    return [np.asfortranarray(a) for a in (ux, uy, uz)]
+
+
+def test_vortex_criteria_on_linear_fields():
+    """Clc_Q_vor and Clc_lamda2 (src/m_Tools.f90:307-442) on uniform velocity
+    gradients G on a stretched mesh, against closed forms computed
+    independently: Q is the second invariant (1/2)((tr G)^2 - tr(G G)), and
+    lambda2 the middle eigenvalue of S^2 + W^2 (S, W the symmetric and
+    antisymmetric parts of G) from numpy's symmetric eigensolver, which
+    also pins CubicRoot's three-real-roots branch and its ordering."""
+    case = CS.case_channel(12, 14, 10, 2.0)
+    o = O.Oracle(case)
+    g = Grid(case, o)
+    rng = np.random.default_rng(7)
+    z = g.zero()
+    for trial in range(6):
+        G = rng.standard_normal((3, 3))
+        ux, uy, uz = linear_velocity(g, G)
+        q_want = 0.5 * (np.trace(G) ** 2 - np.trace(G @ G))
+        S, W = 0.5 * (G + G.T), 0.5 * (G - G.T)
+        lam = np.linalg.eigvalsh(S @ S + W @ W)
+        q = o.dump_visu(capi.VISU_Q, ux, uy, uz, z)
+        l2 = o.dump_visu(capi.VISU_LAMBDA2, ux, uy, uz, z)
+        scale = np.abs(G).max() ** 2
+        assert np.abs(q - q_want).max() <= 1e-11 * scale, trial
+        assert np.abs(l2 - lam[1]).max() <= 1e-10 * scale, (trial, lam)
+        # vortex criteria of a skipped subset are the same points
+        assert np.array_equal(o.dump_visu(capi.VISU_Q, ux, uy, uz, z, skip=(2, 3, 2)), q[::2, ::3, ::2])
+
+
+def test_vortex_criteria_of_pure_shear_and_rotation():
+    """Pure shear u = a y: S^2 + W^2 = 0, so Q = 0 and lambda2 = 0 through
+    CubicRoot's repeated-root branch (t = 0).  Solid rotation u = -w z,
+    w = w x about y: S = 0, Q = w^2 > 0 and the eigenvalues of W^2 are
+    -w^2, -w^2, 0, so lambda2 = -w^2 (a repeated middle root)."""
+    case = CS.case_channel(10, 12, 8, 1.8)
+    o = O.Oracle(case)
+    g = Grid(case, o)
+    z = g.zero()
+    a = 0.9
+    ux, uy, uz = linear_velocity(g, [[0, a, 0], [0, 0, 0], [0, 0, 0]])
+    assert np.abs(o.dump_visu(capi.VISU_Q, ux, uy, uz, z)).max() <= 1e-12
+    assert np.abs(o.dump_visu(capi.VISU_LAMBDA2, ux, uy, uz, z)).max() <= 1e-12
+    w = 0.6
+    ux, uy, uz = linear_velocity(g, [[0, 0, -w], [0, 0, 0], [w, 0, 0]])
+    assert np.allclose(o.dump_visu(capi.VISU_Q, ux, uy, uz, z), w * w, rtol=0, atol=1e-12)
+    l2 = o.dump_visu(capi.VISU_LAMBDA2, ux, uy, uz, z)
+    # a repeated root of a cubic moves by O(sqrt(eps)) under round-off
+    assert np.allclose(l2, -w * w, rtol=0, atol=1e-7), (l2.min(), l2.max())
