@@ -560,9 +560,13 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
 // ---- host launchers -------------------------------------------------------
 
 // Launch shape of the partitioned solver: chunk length MM (compile-time), lines
-// per block LPB and chunk count P = ceil(ny / MM).  The candidates are the
-// instantiations that run without private-memory spills; the first one with
+// per block LPB and chunk count P = ceil(ny / MM).  The first candidate with
 // at most 1/8 padded rows whose block (LPB * P threads) and LDS fit wins.
+// Under the 1024-thread bound the Crank-Nicolson kernels of 8- and 12-row
+// chunks spill registers (24 and 91 VGPRs); spill-free 32-lane shapes with
+// tighter bounds measured slower (4608x140x700 substep 60.0 against 53.4 ms,
+// 128x96x128 0.179 against 0.166 ms): the 512-byte rows of 64 lanes matter
+// more, and the spills stay in cache.
 struct TriShape {
   int mm, lpb, cap;  // cap: the instantiation's launch bound
 };
