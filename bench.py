@@ -108,23 +108,23 @@ def host_cores() -> int:
 
 
 def bench_poisson(args, ev):
-    from paratc_amd import capi, case as CS, ic
+    from paratc_amd import capi, case as CS
     from paratc_amd.solver import Solver
 
     case = CS.case_poisson(1024, 256, 512, 2.0)
     n = case.npts
     s = Solver(case)
-    t0 = time.time()
-    f1 = ic.poisson_rhs(case, seed=1, stream=0)
-    f2 = ic.poisson_rhs(case, seed=1, stream=1)
-    t_gen = time.time() - t0
-    progress(f"generated two 1024x256x512 right-hand sides in {t_gen:.1f} s")
     # InitPoissonSolver times the variants (tile shape, Thomas vs partitioned)
-    # and zeroes both arrays, so it runs before the inputs are uploaded.
+    # and zeroes both arrays, so it runs before the inputs are made.
     s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
-    s.upload(capi.F_ARR1, f1, capi.L_INTERIOR)
-    s.upload(capi.F_WORK1, f2, capi.L_INTERIOR)
-    progress(f"autotuned plan {s.poisson_get_config()}, inputs resident")
+    t0 = time.time()
+    # the two right-hand sides generated in HBM from the host generator's
+    # stream (ic.poisson_rhs(case, seed=1, stream=0 / 1) on the host)
+    s.seed_normal(capi.F_ARR1, seed=1, stream=0, compatible=True)
+    s.seed_normal(capi.F_WORK1, seed=1, stream=1, compatible=True)
+    f1 = s.download(capi.F_ARR1, capi.L_INTERIOR)  # the CPU baseline's input
+    t_gen = time.time() - t0
+    progress(f"autotuned plan {s.poisson_get_config()}, inputs seeded in HBM ({t_gen:.1f} s with the download)")
     srcs = [capi.F_ARR1, capi.F_WORK1]
     if args.calib_copy:
         # a plain device copy of one field (8 B read + 8 B write per point) to
@@ -246,76 +246,173 @@ def bench_spectra(r, ev, samples=5):
             "alg_bytes_per_point": alg, "hbm_gbs_alg": round(alg * case.npts / (ms * 1e-3) / 1e9, 1)}
 
 
-def bench_large(ev, solves=10, rk_steps=5):
-    """The large single case: the Poisson solve at 9216x140x1400 (gamma 2.2,
-    right-hand side as the 1024x256x512 one with seed 4), FP64 and FP32
-    storage with one FP64 correction sweep; the RK3 substep of the gravity
-    current at 4608x140x700 (gamma 1.6, Re 5000, Ri 1, ws 0.02, dt 1e-3)."""
-    from paratc_amd import capi, case as CS, ic
+def bench_c4a(ev, solves=10):
+    """The large single case's Poisson solve at 9216 x 140 x 1400 (gamma 2.2):
+    the right-hand side N(0,1) per cell from seed 4 minus its dyp-weighted
+    mean, generated in HBM (ptc_seed_normal, the host generator's stream);
+    FP64 with per-pass times for the roofline of its dominant pass, then FP32
+    storage with one FP64 correction sweep."""
+    from paratc_amd import capi, case as CS
     from paratc_amd.solver import Solver
 
     case = CS.case_poisson(9216, 140, 1400, 2.2)
     n = case.npts
-    f = ic.poisson_rhs(case, seed=4)
-    progress("9216x140x1400 right-hand side generated")
     s = Solver(case)
+    t0 = time.time()
     s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
-    s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
-    del f
+    t_tune = time.time() - t0
+    s.seed_normal(capi.F_ARR1, seed=4, stream=0, compatible=True)
     plan64 = s.poisson_get_config()
+    progress(f"9216x140x1400: autotuned in {t_tune:.1f} s, plan {plan64}; right-hand side seeded in HBM")
     stream = s.stream()
 
-    def timed():
+    def timed(stages=False):
         s.clcPPE(capi.F_ARR1, capi.F_HALO)
         s.synchronize()
+        if stages:
+            s.poisson_stage_times(1)
         e0, e1 = ev.create(), ev.create()
         ev.record(e0, stream)
         for _ in range(solves):
             s.clcPPE(capi.F_ARR1, capi.F_HALO)
         ev.record(e1, stream)
-        return ev.elapsed_ms(e0, e1) / solves, s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+        ms = ev.elapsed_ms(e0, e1) / solves
+        st = None
+        if stages:
+            st, k = s.poisson_stage_times(0)
+            st = [v / k for v in st]
+        return ms, st, s.poisson_residual(capi.F_ARR1, capi.F_HALO)
 
-    ms64, r64 = timed()
-    progress(f"9216x140x1400 FP64: {ms64:.3f} ms per solve")
+    ms64, st64, r64 = timed(stages=True)
+    ms_plain, _, _ = timed()
+    progress(f"9216x140x1400 FP64: {ms64:.3f} ms per solve (with pass events), {ms_plain:.3f} ms plain")
+    p64 = s.download(capi.F_HALO, capi.L_INTERIOR)
+    scale = float(np.abs(p64).max())
     s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 1)
-    ms32, r32 = timed()
-    progress(f"9216x140x1400 FP32 + 1 sweep: {ms32:.3f} ms per solve")
+    ms32, _, r32 = timed()
+    err32 = float(np.abs(s.download(capi.F_HALO, capi.L_INTERIOR) - p64).max()) / scale
+    del p64
+    progress(f"9216x140x1400 FP32 + 1 sweep: {ms32:.3f} ms per solve, field error {err32:.2e} vs FP64")
     s.close()
-    out = {"poisson_9216x140x1400": {
-        "fp64_ms_per_solve": round(ms64, 3), "fp64_mpts_per_s": round(n / (ms64 * 1e-3) / 1e6, 1),
-        "fp64_hbm_gbs_alg": round(80.0 * n / (ms64 * 1e-3) / 1e9, 1),
-        "fp64_frac_of_peak": round(80.0 * n / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "fp32_refine_1_sweep_ms_per_solve": round(ms32, 3), "fp32_refine_speed_vs_fp64": round(ms64 / ms32, 3),
-        "residual_fp64": r64, "residual_fp32_refine": r32, "plan_fp64": plan64}}
-    case = CS.case_gravity_current(4608, 140, 700, gamma=1.6)
-    ux, uy, uz, sc = ic.gravity_current_fields(case, seed=2, amp=1e-3)
-    s = Solver(case)
-    for fid, a in ((capi.F_UX, ux), (capi.F_UY, uy), (capi.F_UZ, uz), (capi.F_SCALAR, sc)):
-        s.upload(fid, a)
-    del ux, uy, uz, sc
-    s.SetBC_and_UpdateHalo(capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_UY_YM)
-    s.SetBC_and_UpdateHalo_pr(capi.F_PR)
-    s.SetBC_and_UpdateHalo_scalar(capi.F_SCALAR)
-    s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
-    s.L.ptc_channel_iterate(s.h, 1, None)
+    ms = ms_plain
+    dom = int(np.argmax(st64)) if not plan64["slab_planes"] else 2
+    names = stage_kernels(plan64, case.nx, case.nz)
+    return {
+        "workload": "Poisson 9216x140x1400, gamma 2.2, N(0,1) seed 4 minus dyp-weighted mean (seeded in HBM)",
+        "fp64_ms_per_solve": round(ms, 3), "fp64_mpts_per_s": round(n / (ms * 1e-3) / 1e6, 1),
+        "fp64_solves_per_s": round(1e3 / ms, 2),
+        "fp64_hbm_gbs_alg": round(80.0 * n / (ms * 1e-3) / 1e9, 1),
+        "fp64_frac_of_peak": round(80.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": {k: round(v, 4) for k, v in zip(STAGES, st64)},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "launch_ms": round(st64[dom], 4),
+                     "bytes_per_launch": 16 * n,
+                     "achieved": round(16.0 * n / (st64[dom] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(16.0 * n / (st64[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "fp32_refine_1_sweep_ms_per_solve": round(ms32, 3), "fp32_refine_speed_vs_fp64": round(ms / ms32, 3),
+        "fp32_refine_field_error_vs_fp64": err32,
+        "residual_fp64": r64, "residual_fp32_refine": r32, "plan_fp64": plan64,
+    }
+
+
+def bench_gravity_substep(ev, grid, gamma=1.6, steps=5, fp32=True):
+    """RK3 substeps of the gravity current (Re 5000, Sc 1, Ri 1, ws 0.02,
+    dt 1e-3) at `grid`, the initial front and noise seeded in HBM
+    (ptc_seed_fields, seed 2), the Poisson variant chosen by
+    InitPoissonSolver; then the same with FP32 Poisson storage and one FP64
+    correction sweep, its field error after the same steps stated against
+    the FP64 run."""
+    from paratc_amd import capi, case as CS
+    from paratc_amd.solver import Solver
+
+    nx, ny, nz = grid
+    case = CS.case_gravity_current(nx, ny, nz, gamma=gamma)
+
+    def run(prec):
+        s = Solver(case)
+        s.seed_fields(capi.SEED_FRONT, seed=2, amp=1e-3)
+        s.SetBC_and_UpdateHalo(capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_UY_YM)
+        s.SetBC_and_UpdateHalo_pr(capi.F_PR)
+        s.SetBC_and_UpdateHalo_scalar(capi.F_SCALAR)
+        s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+        if prec == "fp32":
+            cfg = s.poisson_get_config()
+            s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 1)
+            if cfg["slab_planes"]:
+                s.poisson_slab_planes(cfg["slab_planes"])
+        s.L.ptc_channel_iterate(s.h, 1, None)
+        s.synchronize()
+        stream = s.stream()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream)
+        rc = s.L.ptc_channel_iterate(s.h, steps, None)
+        ev.record(e1, stream)
+        ms = ev.elapsed_ms(e0, e1) / (3 * steps)
+        assert rc == 0, s.L.ptc_last_error()
+        mon = s.ChannelIterate(1)
+        fields = [s.download(f, capi.L_INTERIOR) for f in (capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_SCALAR)]
+        plan = s.poisson_get_config()
+        s.close()
+        return ms, mon, fields, plan
+
+    ms, mon, f64, plan = run("fp64")
+    progress(f"{nx}x{ny}x{nz} gravity current: {ms:.3f} ms per RK substep")
+    out = {"workload": f"gravity current {nx}x{ny}x{nz}, RK3, dt 1e-3, FP64, scalar (seeded in HBM)",
+           "ms_per_substep": round(ms, 3), "mpts_per_s": round(case.npts / (ms * 1e-3) / 1e6, 1),
+           # 432 B per point: RHS with the scalar 136, four CN solves 96, Poisson
+           # with the divergence formed in its x pass 96, the update with the
+           # scalar ghosts 72 + 32
+           "hbm_gbs_alg": round(432.0 * case.npts / (ms * 1e-3) / 1e9, 1),
+           "divmax_after": float(mon[1]), "plan": plan}
+    if fp32:
+        ms32, mon32, f32, _ = run("fp32")
+        errs = {name: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                for name, a, b in zip(("ux", "uy", "uz", "scalar"), f32, f64)}
+        progress(f"{nx}x{ny}x{nz} gravity current, FP32 Poisson + 1 sweep: {ms32:.3f} ms per substep, "
+                 f"field errors {errs}")
+        out["fp32_refine_1_sweep"] = {"ms_per_substep": round(ms32, 3), "speed_vs_fp64": round(ms / ms32, 3),
+                                      "steps": steps + 2, "field_error_vs_fp64": errs,
+                                      "divmax_after": float(mon32[1])}
+    del f64
+    return out
+
+
+def copy_ceiling(r, ev, reps=20):
+    """Measured copy bandwidth on this part: the library's plain streaming
+    copy (k_copy, 16-byte lanes) of one 1024 x 258 x 512 field, read once and
+    written once."""
+    from paratc_amd import capi
+
+    s = r["solver"]
+    s.copy(capi.F_WORK2, capi.F_ARR1)
     s.synchronize()
     stream = s.stream()
     e0, e1 = ev.create(), ev.create()
     ev.record(e0, stream)
-    rc = s.L.ptc_channel_iterate(s.h, rk_steps, None)
+    for _ in range(reps):
+        s.copy(capi.F_WORK2, capi.F_ARR1)
     ev.record(e1, stream)
-    ms = ev.elapsed_ms(e0, e1) / (3 * rk_steps)
-    assert rc == 0, s.L.ptc_last_error()
-    mon = s.ChannelIterate(1)
-    s.close()
-    progress(f"4608x140x700 gravity current: {ms:.3f} ms per RK substep")
-    out["rk_substep_4608x140x700"] = {
-        "workload": "gravity current 4608x140x700, RK3, dt 1e-3, FP64, scalar", "ms_per_substep": round(ms, 3),
-        "mpts_per_s": round(case.npts / (ms * 1e-3) / 1e6, 1), "divmax_after": float(mon[1])}
-    return out
+    ms = ev.elapsed_ms(e0, e1) / reps
+    nbytes = 2 * 8 * 1024 * 258 * 512
+    return {"kernel": "k_copy", "bytes_per_launch": nbytes, "ms": round(ms, 4),
+            "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, f, budget_s=10.0, max_solves=20):
+    """The oracle's clcPPE_x1 (a plain-C restatement of the reference's
+    Choice-1 solve, FFTs by its own mixed-radix code) on the same right-hand
+    side: up to budget_s of solves on all host cores this process may use
+    (at most 16, the box's CPU share), then one solve on one core."""
     from oracle.oracle import Oracle
 
     cores = host_cores()
@@ -330,6 +427,13 @@ def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
         times.append(time.time() - t0)
         progress(f"cpu baseline solve {len(times)}: {times[-1]:.2f} s on {cores} cores")
     t = float(np.mean(times))
+    orc1 = Oracle(case, threads=1)
+    src = f.copy(order="F")
+    t0 = time.time()
+    orc1.clcPPE(src, out, 1)
+    t1 = time.time() - t0
+    progress(f"cpu baseline, one core: {t1:.2f} s per solve")
+    orc.L.orc_set_threads(cores)
     return {
         "value": round(case.npts / t / 1e6, 3),
         "unit": "Mpts/s",
@@ -337,6 +441,8 @@ def cpu_baseline(case, f, budget_s=12.0, max_solves=20):
         "kind": "port",
         "sample": f"{len(times)} full FP64 Poisson solves (clcPPE_x1 restatement, OpenMP) at "
                   f"{case.nx}x{case.ny}x{case.nz}, mean {t:.3f} s per solve",
+        "one_core_mpts_per_s": round(case.npts / t1 / 1e6, 3),
+        "host": {"nproc": os.cpu_count(), "usable_cores": cores, "model": cpu_model()},
     }
 
 
@@ -424,8 +530,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage + refinement leg")
     ap.add_argument("--no-spectra", action="store_true", help="skip the statistics-spectra leg")
-    ap.add_argument("--large", action="store_true",
-                    help="add the 9216x140x1400 Poisson and 4608x140x700 RK legs (several minutes)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 9216x140x1400 Poisson and the 4608x140x700 and 1536x192x256 RK legs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -440,8 +546,15 @@ def main():
     spec = None if args.no_spectra else bench_spectra(r, ev)
     cpu = None if args.no_cpu_baseline else cpu_baseline(case, r["f1"])
     rk = None if args.no_rk else bench_rk_substep(ev)
+    copy = copy_ceiling(r, ev)
+    progress(f"copy kernel: {copy['gbs']} GB/s")
     ens = None if args.no_ensemble else bench_ensemble()
-    large = bench_large(ev) if args.large else None
+    large = None
+    if not args.no_large:
+        r["solver"].close()  # free the 1024x256x512 fields before the large case
+        large = {"poisson_9216x140x1400": bench_c4a(ev),
+                 "rk_substep_4608x140x700": bench_gravity_substep(ev, (4608, 140, 700), steps=3),
+                 "rk_substep_1536x192x256": bench_gravity_substep(ev, (1536, 192, 256), steps=5)}
     n = case.npts
     kernel = stage_kernels(r["plan"], case.nx, case.nz)[r["dom"]]
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
@@ -467,7 +580,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kernel, "bytes_per_launch": 16 * n,
-                     "launch_ms": round(r["per_stage"][r["dom"]], 4)},
+                     "launch_ms": round(r["per_stage"][r["dom"]], 4),
+                     "copy_ceiling": copy, "frac_of_copy": round(r["achieved"] / copy["gbs"], 4)},
         "cpu_baseline": cpu,
         "poisson": {"solves_per_s": round(1e3 / r["ms_per"], 2),
                     "hbm_gbs_alg": round(r["solve_gbs"], 1),

@@ -327,6 +327,29 @@ enum {
 int ptc_visu_field(ptc_ctx *ctx, int which, int iskip, int jskip, int kskip, int real_bytes, void *host_out,
                    long long count);
 
+/* ---- Seeded synthetic inputs, generated in HBM.  No counterpart in the
+ *      reference, whose initial perturbation is seeded from the wall clock
+ *      (InitVelocity, src/m_FlowCase.f90:49-107) and cannot be reproduced.
+ *  Values are a pure function of (seed, stream, cell): SplitMix64 on the
+ *  counter q = i + nx (j - 1 + ny k) of the interior cell in column-major
+ *  order (the host generator paratc_amd/rng.py gives the same values).
+ *  ptc_seed_normal: field interior = N(0,1) per cell (Box-Muller on uniform
+ *  pairs); compatible = 1 then subtracts the dyp-weighted mean, so the field
+ *  is a compatible right-hand side of the Neumann Poisson problem (the
+ *  Poisson workloads' sources).
+ *  ptc_seed_fields: the initial ux, uy, uz (and scalar) of a case: uniform
+ *  noise amp (2 U - 1) on u, v, w (streams 0, 1, 2; v = 0 on the bottom wall
+ *  face); PTC_SEED_CHANNEL adds the laminar profile 1.5 (1 - ((y - h)/h)^2)
+ *  ubulk to u (h = yly / 2); PTC_SEED_FRONT sets the scalar to the front
+ *  0.5 (1 - tanh((x - x0) / delta)) plus noise (stream 3), velocities noise
+ *  only.  Ghost rows are zeroed: call the boundary fills afterwards. */
+enum {
+  PTC_SEED_CHANNEL = 0,
+  PTC_SEED_FRONT = 1
+};
+int ptc_seed_normal(ptc_ctx *ctx, int field, int seed, int stream, int compatible);
+int ptc_seed_fields(ptc_ctx *ctx, int kind, int seed, double amp, double x0, double delta);
+
 /* ---- Ensembles: many independent cases on one GPU.  No counterpart in the
  *      reference, which runs one case per job; this is the serving layer
  *      around ChannelIterate (src/m_ChannelSystem.f90:80-145).

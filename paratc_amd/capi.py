@@ -35,6 +35,8 @@ TRI_AUTO, TRI_THOMAS, TRI_PARTITIONED = 0, 1, 2
 S_DIVMAX_PRSRC, S_PRGRADAVER, S_SIMTIME, S_DT = 0, 1, 2, 3
 
 # every symbol declared in include/paratc_gpu.h
+SEED_CHANNEL, SEED_FRONT = 0, 1  # ptc_seed_fields kinds
+
 EXPORTS = [
     "ptc_create", "ptc_destroy", "ptc_last_error", "ptc_abi_version", "ptc_upload", "ptc_download",
     "ptc_fill", "ptc_copy", "ptc_field_device_ptr", "ptc_synchronize", "ptc_get_stream", "ptc_get_metric",
@@ -49,7 +51,7 @@ EXPORTS = [
     "ptc_get_spectra", "ptc_visu_field", "ptc_step_graphs",
     "ptc_ensemble_create", "ptc_ensemble_destroy", "ptc_case_footprint", "ptc_ensemble_admit",
     "ptc_ensemble_step", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
-    "ptc_ensemble_latency",
+    "ptc_ensemble_latency", "ptc_seed_normal", "ptc_seed_fields",
 ]
 
 
@@ -140,6 +142,8 @@ def load(path: str = LIB_PATH):
     L.ptc_init_spectra.argtypes = [vp, ip, ip, ip, ip, C.POINTER(ip)]
     L.ptc_clc_spectra.argtypes = [vp, ip, ip, ip, ip, ip]
     L.ptc_get_spectra.argtypes = [vp, dp, dp, dp, C.POINTER(ip), ip]
+    L.ptc_seed_normal.argtypes = [vp, ip, ip, ip, ip]
+    L.ptc_seed_fields.argtypes = [vp, ip, ip, C.c_double, C.c_double, C.c_double]
     sz, ll = C.c_size_t, C.c_longlong
     L.ptc_ensemble_create.argtypes = [sz, C.POINTER(vp)]
     L.ptc_ensemble_destroy.argtypes = [vp]

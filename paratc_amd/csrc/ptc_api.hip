@@ -275,7 +275,11 @@ void drop_step_graph(ptc_ctx *c) {
 // the CFL of the fields before the step and the monitors after it are
 // computed around the launch, as ChannelIterate reports them.
 static int graph_step(ptc_ctx *c, double *mon) {
-  if (!c->step_exec) PTC_TRY(capture_step(c, &c->step_graph, &c->step_exec));
+  if (c->step_exec && c->step_gen != c->buf_gen) drop_step_graph(c);
+  if (!c->step_exec) {
+    PTC_TRY(capture_step(c, &c->step_graph, &c->step_exec));
+    c->step_gen = c->buf_gen;
+  }
   double cflmp = 0.0;
   double *ux, *uy, *uz;
   PTC_TRY(field(c, PTC_F_UX, &ux));

@@ -25,6 +25,7 @@ struct ptc_ensemble {
     ptc_ctx *ctx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned long long gen = 0;  // the context's buffer generation at capture
     size_t bytes = 0;
     long long steps = 0;
     bool live = false;
@@ -60,7 +61,21 @@ static int take_event(ptc_ensemble *e, hipEvent_t *ev) {
   return PTC_OK;
 }
 
+static void drop_case_graph(ptc_ensemble::Case &k) {
+  if (k.exec) (void)hipGraphExecDestroy(k.exec);
+  if (k.graph) (void)hipGraphDestroy(k.graph);
+  k.exec = nullptr;
+  k.graph = nullptr;
+}
+
 static int step_case(ptc_ensemble::Case &k) {
+  // a plan change, a freed plan or grown reduction partials (clcStat, a
+  // residual) since the capture: the graph holds stale addresses
+  if (k.exec && k.gen != k.ctx->buf_gen) drop_case_graph(k);
+  if (!k.exec && k.ctx->prm.icfl != 1 && k.ctx->pp.ready) {
+    PTC_TRY(capture_step(k.ctx, &k.graph, &k.exec));
+    k.gen = k.ctx->buf_gen;
+  }
   if (k.exec) {
     PTC_HIP(hipGraphLaunch(k.exec, k.ctx->stream));
     k.ctx->simtime += k.ctx->prm.dtmax;  // the RK3 alphas sum to one
@@ -95,8 +110,7 @@ int ptc_ensemble_destroy(ptc_ensemble *e) {
   if (!e) return PTC_OK;
   for (auto &k : e->cases) {
     if (k.ctx) (void)hipStreamSynchronize(k.ctx->stream);
-    if (k.exec) (void)hipGraphExecDestroy(k.exec);
-    if (k.graph) (void)hipGraphDestroy(k.graph);
+    drop_case_graph(k);
     if (k.ctx) ptc_destroy(k.ctx);
   }
   for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
@@ -127,7 +141,10 @@ int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id) {
   k.ctx = ctx;
   k.bytes = b;
   k.live = true;
-  if (ctx->prm.icfl != 1) PTC_TRY(capture_step(ctx, &k.graph, &k.exec));
+  if (ctx->prm.icfl != 1) {
+    PTC_TRY(capture_step(ctx, &k.graph, &k.exec));
+    k.gen = ctx->buf_gen;
+  }
   e->used += b;
   int id = -1;
   for (size_t q = 0; q < e->cases.size(); ++q)
@@ -196,8 +213,7 @@ int ptc_ensemble_release(ptc_ensemble *e, int case_id, ptc_ctx **ctx) {
             "unknown case");
   auto &k = e->cases[case_id];
   PTC_HIP(hipStreamSynchronize(k.ctx->stream));
-  if (k.exec) (void)hipGraphExecDestroy(k.exec);
-  if (k.graph) (void)hipGraphDestroy(k.graph);
+  drop_case_graph(k);
   e->used -= k.bytes;
   if (ctx) {
     *ctx = k.ctx;

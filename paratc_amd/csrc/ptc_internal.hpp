@@ -177,6 +177,12 @@ struct ptc_ctx {
   bool step_graphs = true;               // ptc_channel_iterate replays a captured step when dt is fixed
   hipGraph_t step_graph = nullptr;       // one fixed-dt ChannelIterate step, captured on first use
   hipGraphExec_t step_exec = nullptr;
+  // Buffer generation: bumped whenever a buffer that a captured step may
+  // hold (reduction partials, Poisson plan buffers) is freed or reallocated.
+  // A graph records the generation it was captured at and is re-captured
+  // before a replay when the two differ.
+  unsigned long long buf_gen = 1;
+  unsigned long long step_gen = 0;       // generation of step_graph
 };
 
 namespace ptc {
@@ -230,6 +236,9 @@ int prepare_step(ptc_ctx *c);
 int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec);
 // forget the captured step (its buffers are about to change)
 void drop_step_graph(ptc_ctx *c);
+// the reduction partials d_red with room for n values; growing them bumps the
+// buffer generation and drops the context's captured step
+int ensure_red(ptc_ctx *c, int n);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 

@@ -46,6 +46,7 @@ static void make_twiddles(int n, std::vector<double2> &out) {
 int poisson_free_plan(ptc_ctx *c) {
   PoissonPlan &P = c->pp;
   drop_step_graph(c);  // the captured step holds the plan's buffers
+  ++c->buf_gen;        // and so do the ensemble's captured steps
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
   if (P.d_res) (void)hipFree(P.d_res);
@@ -506,12 +507,7 @@ int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *r
   int kc = 0;
   const dim3 grid = residual_grid(c, &kc);
   const int nblk = (int)(grid.x * grid.y * grid.z);
-  if (c->red_cap < nblk) {
-    if (c->d_red) (void)hipFree(c->d_red);
-    c->d_red = nullptr;
-    PTC_HIP(hipMalloc(&c->d_red, sizeof(double) * nblk));
-    c->red_cap = nblk;
-  }
+  PTC_TRY(ensure_red(c, nblk));
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
   hipLaunchKernelGGL(k_residual<false>, grid, dim3(64, 4), 0, c->stream, src, dst, c->geo, rdx2, rdz2, kc, nullptr,
                      c->d_red, nullptr);

@@ -844,10 +844,14 @@ static int rhs_nblocks(const ptc_ctx *c) {
   return (int)(gr.x * gr.y * gr.z);
 }
 
-static int ensure_red(ptc_ctx *c, int n) {
+int ensure_red(ptc_ctx *c, int n) {
   if (c->red_cap < n) {
+    // a captured step (this context's or an ensemble's) holds the old address
+    drop_step_graph(c);
+    ++c->buf_gen;
     if (c->d_red) (void)hipFree(c->d_red);
     c->d_red = nullptr;
+    c->red_cap = 0;
     PTC_HIP(hipMalloc(&c->d_red, sizeof(double) * (size_t)n));
     c->red_cap = n;
   }

@@ -94,6 +94,19 @@ class Solver:
         check(self.L.ptc_get_metric(self.h, which, _dp(out)), "ptc_get_metric")
         return out
 
+    # ---- seeded inputs generated in HBM -----------------------------------------
+    def seed_normal(self, fid: int, seed: int, stream: int = 0, compatible: bool = True):
+        """Interior of fid = N(0,1) per cell from (seed, stream), as
+        ic.poisson_rhs draws it on the host; compatible: minus its dyp-weighted
+        mean (a compatible Neumann right-hand side)."""
+        check(self.L.ptc_seed_normal(self.h, fid, seed, stream, 1 if compatible else 0), "ptc_seed_normal")
+
+    def seed_fields(self, kind: int, seed: int, amp: float, x0: float = 2.0, delta: float = 0.05):
+        """Initial ux, uy, uz (and the scalar front) as ic.channel_fields
+        (kind capi.SEED_CHANNEL) or ic.gravity_current_fields
+        (capi.SEED_FRONT) draw them on the host; ghost rows zeroed."""
+        check(self.L.ptc_seed_fields(self.h, kind, seed, amp, x0, delta), "ptc_seed_fields")
+
     # ---- m_Parameters --------------------------------------------------------
     def PMcoeUpdate(self, ns: int, dt: float):
         check(self.L.ptc_pm_coe_update(self.h, ns, dt), "PMcoeUpdate")
@@ -220,6 +233,30 @@ class Solver:
 
     def Update_uy_ym(self, uy_ym, duy_ym):
         check(self.L.ptc_update_uy_ym(self.h, uy_ym, duy_ym), "Update_uy_ym")
+
+    def project_initial(self, ux=capi.F_UX, uy=capi.F_UY, uz=capi.F_UZ, uy_ym=capi.F_UY_YM,
+                        prsrc=capi.F_ARR1, phi=capi.F_HALO) -> float:
+        """Remove the divergence of an initial velocity field by one projection,
+        built from the reference's own procedures: SetBC_and_UpdateHaloForPrSrc,
+        clcPrSrc, clcPPE, SetBC_and_UpdateHalo_pr, FluidVelUpdate and
+        SetBC_and_UpdateHalo (src/m_ChannelSystem.f90:121-134) with alpha dt = 1,
+        so u <- u - G L^-1 D u.  The substep coefficients are restored
+        afterwards; prsrc and phi are work arrays.  Needs a Poisson plan.
+        Returns max|div u| after the projection (CheckDivergence)."""
+        save = self.coe()
+        c = save.copy()
+        c[2] = 1.0  # pmAlpha
+        self.set_coe(c)
+        try:
+            self.SetBC_and_UpdateHaloForPrSrc(ux, uy, uz, uy_ym)
+            self.clcPrSrc(ux, uy, uz, prsrc)
+            self.clcPPE(prsrc, phi)
+            self.SetBC_and_UpdateHalo_pr(phi)
+            self.FluidVelUpdate(phi, ux, uy, uz)
+            self.SetBC_and_UpdateHalo(ux, uy, uz, uy_ym)
+        finally:
+            self.set_coe(save)
+        return self.CheckDivergence(ux, uy, uz)
 
     # ---- m_Tools monitors ---------------------------------------------------
     def CalcMaxCFL(self, ux, uy, uz) -> float:

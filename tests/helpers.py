@@ -68,3 +68,23 @@ def apply_laplacian(o, case, p):
     pm = np.concatenate([p[:, :1], p, p[:, -1:]], axis=1)
     lp += am[None, :, None] * pm[:, :-2] + ac[None, :, None] * pm[:, 1:-1] + ap[None, :, None] * pm[:, 2:]
     return lp
+
+
+def oracle_project(o, st):
+    """The oracle's form of Solver.project_initial: one projection of the
+    velocities of `st` with alpha dt = 1 (clcPrSrc, clcPPE Choice-1,
+    FluidVelUpdate and the boundary fills), coefficients restored."""
+    save = o.coe()
+    c = save.copy()
+    c[2] = 1.0
+    o.set_coe(c)
+    o.SetBC_and_UpdateHaloForPrSrc(st.ux, st.uy, st.uz, st.uy_ym)
+    f = o.interior()
+    o.clcPrSrc(st.ux, st.uy, st.uz, f)
+    phi = o.halo()
+    o.clcPPE(f, phi, 1)
+    o.SetBC_and_UpdateHalo_pr(phi)
+    o.FluidVelUpdate(phi, st.ux, st.uy, st.uz)
+    o.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
+    o.set_coe(save)
+    return o.CheckDivergence(st.ux, st.uy, st.uz)

@@ -94,3 +94,82 @@ def test_many_cases_overlap_and_finish():
             assert np.array_equal(got.download(capi.F_UX), ref[q].download(capi.F_UX)), q
             got.close()
             ref[q].close()
+
+
+def test_ensemble_graph_recaptured_after_buffer_changes():
+    """An admitted case's captured step holds the context's reduction
+    partials and Poisson plan buffers.  clcStat on the admitted context
+    (growing the partials) and a Poisson plan rebuild (freeing the plan's
+    buffers) between ensemble steps make the next step re-capture; the
+    fields stay those of ordinary launches, bit for bit."""
+    case = CS.case_channel(64, 48, 64, 1.8, re_b=3000.0)
+    a, b = twins(case, seed=7)
+    b.step_graphs(False)
+    with E.Ensemble(1 << 34) as ens:
+        cid = ens.admit(a)
+        view = ens.view(cid)
+        ens.step(1)
+        b.ChannelIterate(1)
+        view.clcStat()
+        b.clcStat()
+        ens.step(2)
+        b.ChannelIterate(2)
+        for f in FIELDS:
+            assert np.array_equal(view.download(f), b.download(f)), f
+        view.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)  # frees and rebuilds the plan
+        b.poisson_config(capi.PREC_FP64, capi.TRI_AUTO, 0, 0)
+        ens.step(1)
+        b.ChannelIterate(1)
+        for f in FIELDS:
+            assert np.array_equal(view.download(f), b.download(f)), f
+        assert np.array_equal(view.get_stat()[0], b.get_stat()[0])
+    b.close()
+
+
+@pytest.mark.parametrize("nsteps,tol", [(1, 1e-12), (5, 1e-10)])
+def test_c3_draws_against_the_oracle(nsteps, tol):
+    """The first four draws of the 64-case workload (seed 3: 64x96x128,
+    64x128x96 with the settling scalar, 128x64x128, 64x96x192 with the
+    settling scalar), admitted together and stepped through the ensemble,
+    against the oracle's ChannelIterate on the same initial fields (noise
+    plus one projection, a uniform scalar of 1)."""
+    from helpers import inner_y, oracle_project, rel_err
+    from oracle.oracle import Oracle, State
+    from paratc_amd import ic
+
+    wl = E.c3_workload(ncases=4)
+    assert sum(c.scalar for c in wl.cases) == 2
+    seeds = [100 + q for q in range(4)]
+    solvers = [E.make_case(c, sd) for c, sd in zip(wl.cases, seeds)]
+    refs = []
+    for case, sd in zip(wl.cases, seeds):
+        o = Oracle(case)
+        st = State(o)
+        st.ux, st.uy, st.uz = ic.channel_fields(case, seed=sd, amp=0.1)
+        o.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
+        if case.scalar:
+            st.scalar[:] = 1.0
+            o.SetBC_and_UpdateHalo_scalar(st.scalar)
+        assert oracle_project(o, st) <= 1e-12
+        refs.append((o, st))
+    with E.Ensemble(1 << 36) as ens:
+        ids = [ens.admit(s) for s in solvers]
+        assert None not in ids
+        ens.step(nsteps)
+        for q, cid in enumerate(ids):
+            o, st = refs[q]
+            for _ in range(nsteps):
+                o.ChannelIterate(st, 1)
+            view = ens.view(cid)
+            pairs = [(capi.F_UX, st.ux), (capi.F_UY, st.uy), (capi.F_UZ, st.uz), (capi.F_PR, st.p)]
+            if wl.cases[q].scalar:
+                pairs.append((capi.F_SCALAR, st.scalar))
+            for fid, ref in pairs:
+                e = rel_err(inner_y(view.download(fid)), inner_y(ref))
+                # p: the ill-conditioned low modes, as in the single-case tests
+                bar = tol if fid != capi.F_PR else 10 * tol
+                print(f"C3 draw {q} ({wl.cases[q].nx}x{wl.cases[q].ny}x{wl.cases[q].nz}"
+                      f"{', scalar' if wl.cases[q].scalar else ''}) {nsteps} steps field {fid}: {e:.3e}",
+                      flush=True)
+                assert e <= bar, (q, fid, e)
+            assert view.CheckDivergence(capi.F_UX, capi.F_UY, capi.F_UZ) <= 1e-10
