@@ -98,6 +98,9 @@ __global__ void __launch_bounds__(256) k_seed_fields(Geo g, int kind, unsigned l
                                                      double ubulk, double half, double x0, double delta, double dx,
                                                      double *__restrict__ ux, double *__restrict__ uy,
                                                      double *__restrict__ uz, double *__restrict__ sc) {
+  // no fused multiply-adds: the profile and the noise round as the host
+  // generator's separate products and sums do, so the fields match it bit for bit
+#pragma clang fp contract(off)
   const int j = blockIdx.x, k = blockIdx.y;  // j = 0..ny+1
   const long long base = (long long)g.plane * k + (long long)g.nx * j;
   const bool interior = j >= 1 && j <= g.ny;

@@ -44,6 +44,7 @@ module m_paratc_gpu
   public :: ptc_set_bc_and_update_halo, ptc_set_bc_and_update_halo_pr
   public :: ptc_set_bc_and_update_halo_scalar, ptc_channel_iterate, ptc_calc_ux_aver
   public :: ptc_check_divergence, ptc_synchronize
+  public :: ptc_set_coe, ptc_get_coe
   public :: ptc_pm_coe_update, ptc_init_time_scheme, ptc_clc_rhs_xyz, ptc_clc_u1hat, ptc_clc_u2hat
   public :: ptc_clc_u3hat, ptc_clc_scalar, ptc_clc_prsrc, ptc_fluid_vel_update, ptc_pressure_update
   public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
@@ -148,6 +149,16 @@ module m_paratc_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: ns
       real(c_double), value :: dt
+    end function
+    integer(c_int) function ptc_set_coe(ctx, in9) bind(C, name="ptc_set_coe")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: in9(9)
+    end function
+    integer(c_int) function ptc_get_coe(ctx, out9) bind(C, name="ptc_get_coe")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out9(9)
     end function
     integer(c_int) function ptc_init_time_scheme(ctx) bind(C, name="ptc_init_time_scheme")
       import :: c_int, c_ptr
