@@ -193,7 +193,14 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *   PTC_POISSON_STREAMING   nontemporal (cache-bypassing) loads and stores in
  *                           the x passes and the complex-line y-solve: 1 on,
  *                           -1 off, 0 (automatic) on where the spectrum is
- *                           larger than the last-level cache. */
+ *                           larger than the last-level cache.
+ *   PTC_POISSON_PERSIST     1: the x and z passes of the lengths that have
+ *                           one (x: nx = 1024, 4608, 9216; z: nz = 512, 700,
+ *                           1400) run as persistent workgroups that load
+ *                           their next line or column tile while they
+ *                           transform the current one; 0 or -1: one
+ *                           workgroup per line or tile.  The search of
+ *                           InitPoissonSolver tries it. */
 enum {
   PTC_POISSON_CHUNK_ROWS = 1,
   PTC_POISSON_SLAB_PLANES = 2,
@@ -201,13 +208,14 @@ enum {
   PTC_POISSON_FAST_ZTILE = 4,
   PTC_POISSON_TRI_LANES = 5,
   PTC_POISSON_TRI_COMPLEX = 6,
-  PTC_POISSON_STREAMING = 7
+  PTC_POISSON_STREAMING = 7,
+  PTC_POISSON_PERSIST = 8
 };
 int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
 /* the value a plan knob has in the plan in use (chunk rows, slab planes,
  * LDS-staged lines, z tile, lanes per workgroup, complex lines per
- * workgroup, streaming on (1) or off (0); 0 where the knob's variant is not
- * in use) */
+ * workgroup, streaming on (1) or off (0), persistent passes (1 x, 2 z,
+ * 3 both); 0 where the knob's variant is not in use) */
 int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
 /* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per

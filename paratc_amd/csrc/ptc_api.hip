@@ -94,7 +94,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   // then the y-solve (partitioned: chunk length and workgroup width, LDS
   // staging or complex lines; or Thomas) with that FFT.
   struct Cand {
-    int tri, mm, tile, slab, ll, zf, lanes, cl;
+    int tri, mm, tile, slab, ll, zf, lanes, cl, ps;
   };
   auto apply = [&](const Cand &cd) {
     c->pp.tri_req = cd.tri;
@@ -105,6 +105,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     c->pp.zfast_req = cd.zf;
     c->pp.tri_lpb_req = cd.lanes;
     c->pp.tri_cl_req = cd.cl;
+    c->pp.persist_req = cd.ps;
   };
   auto run = [&](const Cand &cd, double *ms) -> int {
     poisson_free_plan(c);
@@ -113,10 +114,11 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     if (cd.zf > 0 && !(c->pp.z_fast && c->pp.ztile == cd.zf)) return PTC_E_UNSUPPORTED;
     if (cd.tile > 0 && c->pp.ztile != cd.tile) return PTC_E_UNSUPPORTED;  // clipped: covered elsewhere
     if (cd.slab > 0 && c->pp.slab != cd.slab) return PTC_E_UNSUPPORTED;
+    if (cd.ps > 0 && !c->pp.persist_x && !c->pp.persist_z) return PTC_E_UNSUPPORTED;
     return time_solve(c, s, d, 2, 3, ms);
   };
   double best = 1e300;
-  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0, 0, 0};
+  Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0, 0, 0, 0};
   auto consider = [&](const Cand &cd) {
     double ms = 0;
     if (run(cd, &ms) == PTC_OK && ms < best) {
@@ -124,19 +126,25 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
       pick = cd;
     }
   };
-  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0, 0, 0});
-  for (int zf : {4, 8, 16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0});
+  for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0, 0, 0, 0});
+  for (int zf : {4, 8, 16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0, 0});
   const int tile = pick.tile, zf = pick.zf;
   for (int mm : {4, 8, 12, 16}) {
-    for (int lanes : {64, 32, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, lanes, 0});
-    for (int ll : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll, zf, 0, 0});
-    for (int cl : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, 0, cl});
+    for (int lanes : {64, 32, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, lanes, 0, 0});
+    for (int ll : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, ll, zf, 0, 0, 0});
+    for (int cl : {16, 32}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, 0, cl, 0});
   }
-  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0, zf, 0, 0});
+  consider(Cand{PTC_TRI_THOMAS, 0, tile, 0, 0, zf, 0, 0, 0});
   if (tile == 0) {
     const Cand base = pick;
     for (int slab : {64, 32, 16, 8})
-      consider(Cand{base.tri, base.mm, 0, slab, base.ll, base.zf, base.lanes, base.cl});
+      consider(Cand{base.tri, base.mm, 0, slab, base.ll, base.zf, base.lanes, base.cl, 0});
+    // the persistent prefetching x/z passes with the plan picked so far, and
+    // with each compiled z tile
+    const Cand b2 = pick;
+    consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, b2.zf, b2.lanes, b2.cl, 1});
+    for (int zf : {4, 8, 16})
+      if (zf != b2.zf) consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, zf, b2.lanes, b2.cl, 1});
   }
   poisson_free_plan(c);
   apply(pick);
@@ -481,6 +489,8 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
               "complex lines per workgroup must be -1, 0, 16 or 32");
   } else if (knob == PTC_POISSON_STREAMING) {
     PTC_CHECK(value >= -1 && value <= 1, PTC_E_ARG, "streaming must be -1, 0 or 1");
+  } else if (knob == PTC_POISSON_PERSIST) {
+    PTC_CHECK(value >= -1 && value <= 1, PTC_E_ARG, "persistent passes must be -1, 0 or 1");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
@@ -492,6 +502,7 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   else if (knob == PTC_POISSON_TRI_LANES) ctx->pp.tri_lpb_req = value;
   else if (knob == PTC_POISSON_TRI_COMPLEX) ctx->pp.tri_cl_req = value;
   else if (knob == PTC_POISSON_STREAMING) ctx->pp.stream_req = value;
+  else if (knob == PTC_POISSON_PERSIST) ctx->pp.persist_req = value;
   else ctx->pp.zfast_req = value;
   return poisson_build_plan(ctx);
 }
@@ -509,6 +520,7 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value) {
     case PTC_POISSON_TRI_LANES: *value = part && !P.tri_ll && !P.tri_cl ? P.tri_lpb : 0; break;
     case PTC_POISSON_TRI_COMPLEX: *value = part ? P.tri_cl : 0; break;
     case PTC_POISSON_STREAMING: *value = P.stream ? 1 : 0; break;
+    case PTC_POISSON_PERSIST: *value = (P.persist_x ? 1 : 0) | (P.persist_z ? 2 : 0); break;
     default: PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }
   return PTC_OK;

@@ -357,12 +357,11 @@ __device__ __forceinline__ void run_rest(int g, const double2 (&W)[7][4], const 
   stage<N, P.ns - 1, G, INV, true>(g, W[P.ns - 1], ldsload, storeL);
 }
 
+// the whole transform with the first twiddles already in registers
 template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class LdsAt, class StoreL>
-__device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
-                                        const StoreL &storeL) {
+__device__ __forceinline__ void run_fft_w(int g, const double2 (&W)[7][4], const Load0 &load0, const LdsAt &lds,
+                                          const StoreL &storeL) {
   constexpr PlanC P = make_plan(N);
-  double2 W[7][4];
-  load_twiddles<N, G, INV>(g, tw, W);
   if constexpr (P.ns == 1) {
     stage<N, 0, G, INV, false>(g, W[0], load0, storeL);
   } else {
@@ -371,6 +370,14 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
     __syncthreads();
     run_rest<N, G, INV>(g, W, lds, storeL);
   }
+}
+
+template <int N, int G, bool INV, bool LOAD0_LDS = false, class Load0, class LdsAt, class StoreL>
+__device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, const Load0 &load0, const LdsAt &lds,
+                                        const StoreL &storeL) {
+  double2 W[7][4];
+  load_twiddles<N, G, INV>(g, tw, W);
+  run_fft_w<N, G, INV, LOAD0_LDS>(g, W, load0, lds, storeL);
 }
 
 }  // namespace f2
@@ -398,6 +405,40 @@ constexpr int f2_threads() {
 template <int N>
 constexpr int f2_xblock() {
   return f2_threads<N>() > 256 ? f2_threads<N>() : 256;
+}
+
+// The half spectrum of a real row of 2N points from the N-point transform Z
+// of its packed pairs (in ls): X_k = (S_k - i w^k D_k) / 2 with
+// S_k = Z_k + conj Z_{N-k}, D_k = Z_k - conj Z_{N-k}, w = exp(-2 pi i / 2N);
+// values past k = N (pitch padding) are zero.
+template <int N, int G, bool STREAM, typename S>
+__device__ __forceinline__ void x_untangle_store(int g, const double2 *ls, const double2 *__restrict__ wx,
+                                                 S *__restrict__ out, int pitch) {
+  for (int kx = g; kx < pitch; kx += G) {
+    double2 X = make_double2(0.0, 0.0);
+    if (kx <= N) {
+      const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
+      double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
+      zc.y = -zc.y;
+      const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
+      const double2 wd = f2::cmul(wx[kx], d);
+      X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
+    }
+    f2::st2x<STREAM>(out[kx], X);
+  }
+}
+
+// The inverse: element m of the N-point inverse transform's input from the
+// half spectrum X (in ls): X_m + conj X_{N-m} + i conj(w^m) (X_m - conj X_{N-m}).
+__device__ __forceinline__ double2 x_retangle(const double2 *ls, const double2 *__restrict__ wx, int m, int n) {
+  const double2 xk = ls[f2::pad8(m)];
+  double2 xc = ls[f2::pad8(n - m)];
+  xc.y = -xc.y;
+  const double2 s = f2::cadd(xk, xc), d = f2::csub(xk, xc);
+  double2 w = wx[m];
+  w.y = -w.y;
+  const double2 wd = f2::cmul(w, d);
+  return make_double2(s.x - wd.y, s.y + wd.x);
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
@@ -468,21 +509,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
   auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
   __syncthreads();
-  if (live) {
-    S *out = spec + sq * pitch;
-    for (int kx = g; kx < pitch; kx += G) {
-      double2 X = make_double2(0.0, 0.0);
-      if (kx <= N) {
-        const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
-        double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
-        zc.y = -zc.y;
-        const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
-        const double2 wd = f2::cmul(wx[kx], d);
-        X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
-      }
-      f2::st2x<STREAM>(out[kx], X);
-    }
-  }
+  if (live) x_untangle_store<N, G, STREAM>(g, ls, wx, spec + sq * pitch, pitch);
   if constexpr (DIV) {
     // fixed-order tree over the workgroup, in the FFT's LDS once its reads are done
     double *red = reinterpret_cast<double *>(sh);
@@ -533,16 +560,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restri
 #pragma unroll
   for (int t = 0; t < NT; ++t) ls[f2::pad8(min(g + t * G, N))] = pre[t];
   __syncthreads();
-  auto load0 = [&](int m) {
-    const double2 xk = ls[f2::pad8(m)];
-    double2 xc = ls[f2::pad8(N - m)];
-    xc.y = -xc.y;
-    const double2 s = f2::cadd(xk, xc), d = f2::csub(xk, xc);
-    double2 w = wx[m];
-    w.y = -w.y;
-    const double2 wd = f2::cmul(w, d);
-    return make_double2(s.x - wd.y, s.y + wd.x);
-  };
+  auto load0 = [&](int m) { return x_retangle(ls, wx, m, N); };
   auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
   double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   // the ghost row this line also fills (or none)
@@ -586,7 +604,272 @@ __global__ void __launch_bounds__(f2_threads<N>() * T) k2_fft_z(S *__restrict__ 
   f2::run_fft<N, G, INV>(g, tw, load0, lds, store_out);
 }
 
+// ---- persistent passes with a register prefetch of the next work item ------
+// The long lengths keep one workgroup per CU (their lines fill 83-90 KB of
+// LDS), so a one-shot workgroup idles through the HBM latency of its own
+// loads and the CU has no other workgroup to switch to.  These kernels run
+// one or a few workgroups per CU over the work items (line groups, or z
+// column tiles) in strides of the grid; each issues the global loads of its
+// next item -- the inputs of stage 0, straight into the registers stage 0
+// of the current item has just consumed -- before the remaining stages, so
+// those loads are in flight during the LDS stages.  The barriers wait on LDS traffic only (s_waitcnt lgkmcnt), not
+// on the outstanding global loads.  Same arithmetic as the one-shot kernels.
+
+template <int N, typename S, bool STREAM>
+__global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
+                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
+                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  constexpr int PN = f2::padded(N);
+  constexpr int R0 = f2::make_plan(N).r[0];
+  constexpr int NB0 = N / R0;
+  constexpr int BPT0 = f2::stage_bpt<N, 0, G>();
+  __shared__ double2 sh[L * PN];
+  const int l = threadIdx.x / G, g = threadIdx.x - l * G;
+  const long long nlines = (long long)nj * nz;
+  const long long ngroups = (nlines + L - 1) / L;
+  double2 *ls = sh + l * PN;
+  // line l of group grp: rows j fastest; sq = its spectral row
+  auto line = [&](long long grp, bool &live, long long &sq) {
+    const long long q = grp * L + l;
+    live = q < nlines;
+    const long long qq = live ? q : 0;
+    const long long jl = j0 + qq % nj, k = qq / nj;
+    sq = jl + (long long)ny * k;
+    return reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
+  };
+  auto fetch = [&](long long grp, f2::StageRegs<N, 0, G> &v) {
+    bool live;
+    long long sq;
+    const double2 *row = line(grp, live, sq);
+#pragma unroll
+    for (int i = 0; i < BPT0; ++i) {
+      const int b = g + i * G;
+      if (BPT0 * G == NB0 || b < NB0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[i][r] = live ? f2::ld2x<STREAM>(row[b + r * NB0]) : make_double2(0.0, 0.0);
+      }
+    }
+  };
+  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
+  f2::StageRegs<N, 0, G> v;
+  long long grp = blockIdx.x;
+  if (grp >= ngroups) return;
+  fetch(grp, v);
+  for (; grp < ngroups; grp += gridDim.x) {
+    // the lane index through an empty asm: the per-stage LDS addresses are
+    // recomputed in each iteration, as in the one-shot kernel, rather than
+    // hoisted out of the loop into registers
+    int gi = g;
+    asm volatile("" : "+v"(gi));
+    // the stage twiddles, reloaded per item (cache hits) rather than held
+    // across the loop
+    double2 W[7][4];
+    f2::load_twiddles<N, G, false>(gi, tw, W);
+    // stage 0 consumes the registers; the next group's loads then go into
+    // them (the last group re-reads its own: no branch around the loads, and
+    // the values are discarded).  The scheduling barrier keeps those loads
+    // after stage 0, so the two sets of registers are never live together.
+    f2::stage_exec<N, 0, G, false>(gi, W[0], v, store_lds);
+    __builtin_amdgcn_sched_barrier(0);
+    const long long gn = grp + gridDim.x < ngroups ? grp + gridDim.x : grp;
+    fetch(gn, v);
+    __syncthreads();
+    f2::run_rest<N, G, false>(gi, W, lds, store_lds);
+    __syncthreads();
+    bool live;
+    long long sq;
+    (void)line(grp, live, sq);
+    if (live) x_untangle_store<N, G, STREAM>(gi, ls, wx, spec + sq * pitch, pitch);
+    __syncthreads();
+  }
+}
+
+template <int N, typename S, bool ACC, bool STREAM>
+__global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
+                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
+                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
+                                                     double scale, const double *__restrict__ shift, int ghost) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  constexpr int PN = f2::padded(N);
+  constexpr int NT = N / G + 1;
+  __shared__ double2 sh[L * PN];
+  const int l = threadIdx.x / G, g = threadIdx.x - l * G;
+  const long long nlines = (long long)nj * nz;
+  const long long ngroups = (nlines + L - 1) / L;
+  double2 *ls = sh + l * PN;
+  const double sh0 = ACC ? shift[0] : 0.0;
+  auto coords = [&](long long grp, bool &live, long long &jl, long long &k) {
+    const long long q = grp * L + l;
+    live = q < nlines;
+    const long long qq = live ? q : 0;
+    jl = j0 + qq % nj;
+    k = qq / nj;
+  };
+  // the line's N+1 spectral values (indices past N repeat element N)
+  auto fetch = [&](long long grp, double2 (&v)[NT]) {
+    bool live;
+    long long jl, k;
+    coords(grp, live, jl, k);
+    const S *in = spec + (jl + (long long)ny * k) * pitch;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) v[t] = f2::ld2x<STREAM>(in[min(g + t * G, N)]);
+  };
+  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  double2 v[NT];
+  long long grp = blockIdx.x;
+  if (grp >= ngroups) return;
+  fetch(grp, v);
+  for (; grp < ngroups; grp += gridDim.x) {
+    int gi = g;  // recomputed addresses and twiddles per iteration, as in the forward pass
+    asm volatile("" : "+v"(gi));
+    double2 W[7][4];
+    f2::load_twiddles<N, G, true>(gi, tw, W);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) ls[f2::pad8(min(gi + t * G, N))] = v[t];
+    // the registers are free: the next group's loads go into them
+    __builtin_amdgcn_sched_barrier(0);
+    const long long gn = grp + gridDim.x < ngroups ? grp + gridDim.x : grp;
+    fetch(gn, v);
+    __syncthreads();
+    bool live;
+    long long jl, k;
+    coords(grp, live, jl, k);
+    double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
+    double2 *grow = nullptr;
+    if (ghost && jl == 0) grow = reinterpret_cast<double2 *>(dst + rowlen * ((long long)(ny + 2) * k));
+    if (ghost && jl == ny - 1) grow = reinterpret_cast<double2 *>(dst + rowlen * (ny + 1 + (long long)(ny + 2) * k));
+    auto load0 = [&](int m) { return x_retangle(ls, wx, m, N); };
+    auto store_out = [&](int m, double2 v) {
+      if (!live) return;
+      double2 x;
+      if (ACC) {
+        const double2 o = f2::ld2x<STREAM>(row[m]);
+        x = make_double2(o.x + v.x * scale - sh0, o.y + v.y * scale - sh0);
+      } else {
+        x = make_double2(v.x * scale, v.y * scale);
+      }
+      f2::st2x<STREAM>(row[m], x);
+      if (grow) f2::st2x<STREAM>(grow[m], x);
+    };
+    f2::run_fft_w<N, G, true, true>(gi, W, load0, lds, store_out);
+    __syncthreads();
+  }
+}
+
+// z pass over nitems = ntile * nj column tiles (XCD-grouped as the one-shot
+// kernel, so the tiles that share a 128-byte row segment are in flight on
+// one XCD together)
+template <int N, int T, bool INV, typename S>
+__global__ void __launch_bounds__(f2_threads<N>() * T) k2p_fft_z(S *__restrict__ spec, const double2 *__restrict__ tw,
+                                                  int ny, int pitch, int j0, int ntile, int nitems) {
+  constexpr int G = f2_threads<N>();
+  constexpr int GRP = (int)(128 / (T * sizeof(S))) > 1 ? (int)(128 / (T * sizeof(S))) : 1;
+  constexpr int R0 = f2::make_plan(N).r[0];
+  constexpr int NB0 = N / R0;
+  constexpr int BPT0 = f2::stage_bpt<N, 0, G>();
+  __shared__ double2 sh[N * T];
+  const int col = threadIdx.x % T, g = threadIdx.x / T;
+  const long long zs = (long long)pitch * ny;
+  auto base_of = [&](int item) {
+    const int w = xcd_group<GRP>(item, nitems);
+    const int tile = w % ntile, jj = w / ntile;
+    return spec + (long long)(j0 + jj) * pitch + (long long)tile * T + col;
+  };
+  auto fetch = [&](int item, f2::StageRegs<N, 0, G> &v) {
+    const S *base = base_of(item);
+#pragma unroll
+    for (int i = 0; i < BPT0; ++i) {
+      const int b = g + i * G;
+      if (BPT0 * G == NB0 || b < NB0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[i][r] = f2::ld2(base[(long long)(b + r * NB0) * zs]);
+      }
+    }
+  };
+  f2::StageRegs<N, 0, G> v;
+  int item = blockIdx.x;
+  if (item >= nitems) return;
+  fetch(item, v);
+  for (; item < nitems; item += gridDim.x) {
+    // lane indices through an empty asm: addresses recomputed per iteration
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int ci = tid % T, gi = tid / T;
+    double2 W[7][4];
+    f2::load_twiddles<N, G, INV>(gi, tw, W);
+    auto lds = [&](int m) -> double2 & { return sh[m * T + ci]; };
+    auto store_lds = [&](int m, double2 x) { sh[m * T + ci] = x; };
+    f2::stage_exec<N, 0, G, INV>(gi, W[0], v, store_lds);
+    __builtin_amdgcn_sched_barrier(0);
+    const int in = item + (int)gridDim.x < nitems ? item + (int)gridDim.x : item;
+    fetch(in, v);
+    S *base = base_of(item);
+    auto store_out = [&](int m, double2 x) { f2::st2(base[(long long)m * zs], x); };
+    __syncthreads();
+    f2::run_rest<N, G, INV>(gi, W, lds, store_out);
+    __syncthreads();
+  }
+}
+
 // ---- dispatch ----------------------------------------------------------------
+// lengths with persistent forms: the x lengths of the 1024-, 4608- and
+// 9216-point rows, the z lengths 512, 700 and 1400 with their tiles
+constexpr bool has_px(int n) { return n == 512 || n == 2304 || n == 4608; }
+constexpr bool has_pz(int n, int t) {
+  return (n == 512 && (t == 8 || t == 16)) || (n == 700 && (t == 4 || t == 8)) || (n == 1400 && (t == 2 || t == 4));
+}
+
+// workgroups of a persistent kernel resident per CU (at least one)
+template <class K>
+static int resident_blocks(K kernel, int threads) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  return nb;
+}
+
+static unsigned persistent_grid(const ptc_ctx *c, int per_cu, long long items) {
+  return (unsigned)std::max<long long>(1, std::min<long long>(items, (long long)per_cu * std::max(1, c->n_cu)));
+}
+
+template <int N, typename S, bool STREAM>
+static void px_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  static const int per_cu = resident_blocks(k2p_fft_x_fwd<N, S, STREAM>, L * G);
+  const long long groups = ((long long)nj * c->nz + L - 1) / L;
+  hipLaunchKernelGGL((k2p_fft_x_fwd<N, S, STREAM>), dim3(persistent_grid(c, per_cu, groups)), dim3(L * G), 0,
+                     c->stream, src, spec, c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch,
+                     (long long)c->nx);
+}
+
+template <int N, typename S, bool ACC, bool STREAM>
+static void px_inv_launch(ptc_ctx *c, const S *spec, double *dst, double scale, int j0, int nj, bool ghost) {
+  constexpr int G = f2_threads<N>();
+  constexpr int L = (256 / G) > 0 ? 256 / G : 1;
+  static const int per_cu = resident_blocks(k2p_fft_x_inv<N, S, ACC, STREAM>, L * G);
+  const long long groups = ((long long)nj * c->nz + L - 1) / L;
+  hipLaunchKernelGGL((k2p_fft_x_inv<N, S, ACC, STREAM>), dim3(persistent_grid(c, per_cu, groups)), dim3(L * G), 0,
+                     c->stream, spec, dst, c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch,
+                     (long long)c->nx, scale, c->d_scal + kSlotPinShift, ghost ? 1 : 0);
+}
+
+template <int N, int T, bool INV, typename S>
+static void pz_launch(ptc_ctx *c, S *spec, int j0, int nj) {
+  constexpr int G = f2_threads<N>();
+  static const int per_cu = resident_blocks(k2p_fft_z<N, T, INV, S>, G * T);
+  const int ntile = c->pp.pitch / T;
+  const int nitems = ntile * nj;
+  hipLaunchKernelGGL((k2p_fft_z<N, T, INV, S>), dim3(persistent_grid(c, per_cu, nitems)), dim3(G * T), 0, c->stream,
+                     spec, c->pp.fz.tw, c->ny, c->pp.pitch, j0, ntile, nitems);
+}
+
 template <int N>
 static long long x_fwd_blocks(long long nlines) {
   constexpr int G = f2_threads<N>();
@@ -611,6 +894,16 @@ static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bo
     PTC_CHECK(!f32, PTC_E_STATE, "the divergence-forming x pass needs FP64 spectra");
     if (c->pp.stream) x_fwd_launch<N, double2, true, true>(c, src, (double2 *)spec, j0, nj, *div);
     else x_fwd_launch<N, double2, false, true>(c, src, (double2 *)spec, j0, nj, *div);
+  } else if (c->pp.persist_x) {
+    if constexpr (has_px(N)) {
+      if (f32 && c->pp.stream) px_fwd_launch<N, float2, true>(c, src, (float2 *)spec, j0, nj);
+      else if (f32) px_fwd_launch<N, float2, false>(c, src, (float2 *)spec, j0, nj);
+      else if (c->pp.stream) px_fwd_launch<N, double2, true>(c, src, (double2 *)spec, j0, nj);
+      else px_fwd_launch<N, double2, false>(c, src, (double2 *)spec, j0, nj);
+    } else {
+      set_error("no persistent x pass for this length");
+      return PTC_E_UNSUPPORTED;
+    }
   } else if (f32 && c->pp.stream) {
     x_fwd_launch<N, float2, true, false>(c, src, (float2 *)spec, j0, nj, none);
   } else if (f32) {
@@ -640,6 +933,25 @@ static int x_inv_t(ptc_ctx *c, const void *spec, double *dst, double scale, int 
   const bool f32 = c->pp.prec == PTC_PREC_FP32_REFINE, st = c->pp.stream;
   const float2 *s4 = (const float2 *)spec;
   const double2 *s8 = (const double2 *)spec;
+  if (c->pp.persist_x) {
+    if constexpr (has_px(N)) {
+      if (f32) {
+        if (acc && st) px_inv_launch<N, float2, true, true>(c, s4, dst, scale, j0, nj, ghost);
+        else if (acc) px_inv_launch<N, float2, true, false>(c, s4, dst, scale, j0, nj, ghost);
+        else if (st) px_inv_launch<N, float2, false, true>(c, s4, dst, scale, j0, nj, ghost);
+        else px_inv_launch<N, float2, false, false>(c, s4, dst, scale, j0, nj, ghost);
+      } else {
+        PTC_CHECK(!acc, PTC_E_STATE, "accumulating x-inverse with FP64 spectra");
+        if (st) px_inv_launch<N, double2, false, true>(c, s8, dst, scale, j0, nj, ghost);
+        else px_inv_launch<N, double2, false, false>(c, s8, dst, scale, j0, nj, ghost);
+      }
+      PTC_HIP(hipGetLastError());
+      return PTC_OK;
+    } else {
+      set_error("no persistent x pass for this length");
+      return PTC_E_UNSUPPORTED;
+    }
+  }
   if (f32) {
     if (acc && st) x_inv_launch<N, float2, true, true>(c, s4, dst, scale, j0, nj, ghost);
     else if (acc) x_inv_launch<N, float2, true, false>(c, s4, dst, scale, j0, nj, ghost);
@@ -662,6 +974,22 @@ static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64) {
   dim3 grid((unsigned)(ntile * nj));
   const auto *tw = c->pp.fz.tw;
   const int ny = c->ny, pitch = c->pp.pitch;
+  if (c->pp.persist_z) {
+    if constexpr (has_pz(N, T)) {
+      if (!f64 && c->pp.prec == PTC_PREC_FP32_REFINE) {
+        if (inverse) pz_launch<N, T, true, float2>(c, (float2 *)spec, j0, nj);
+        else pz_launch<N, T, false, float2>(c, (float2 *)spec, j0, nj);
+      } else {
+        if (inverse) pz_launch<N, T, true, double2>(c, (double2 *)spec, j0, nj);
+        else pz_launch<N, T, false, double2>(c, (double2 *)spec, j0, nj);
+      }
+      PTC_HIP(hipGetLastError());
+      return PTC_OK;
+    } else {
+      set_error("no persistent z pass for this length and tile");
+      return PTC_E_UNSUPPORTED;
+    }
+  }
   if (!f64 && c->pp.prec == PTC_PREC_FP32_REFINE) {
     if (inverse)
       hipLaunchKernelGGL((k2_fft_z<N, T, true, float2>), grid, dim3(G * T), 0, c->stream, (float2 *)spec, tw, ny, pitch,
@@ -685,6 +1013,9 @@ static int z_t(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64) {
 // (2304 and 4608: the 4608- and 9216-point rows of the largest cases; their
 // lines take 41 KB and 83 KB of LDS, which gfx950's 160 KB allows)
 #define PTC_X_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256) X(512) X(768) X(2304) X(4608)
+
+bool fft2_has_px(int h) { return has_px(h); }
+bool fft2_has_pz(int nz, int tile) { return has_pz(nz, tile); }
 
 bool fft2_has_x(int h) {
 #define PTC_CASE(N) if (h == N) return true;

@@ -107,6 +107,8 @@ struct PoissonPlan {
   int tri_cl = 0;     // in use
   int stream_req = 0; // streaming (nontemporal) x and y passes: -1 off, 0 by size, 1 on
   bool stream = false;  // in use
+  int persist_req = 0;  // persistent prefetching x/z passes: 1 where built, else (0, -1) off
+  bool persist_x = false, persist_z = false;  // in use
   FftLen fx{}, fz{};          // complex lengths h and nz
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
@@ -258,6 +260,9 @@ int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f
                const DivSrc *div = nullptr);
 // workgroups of one specialised x forward launch over nj planes (0: no such kernel)
 long long fft2_x_fwd_blocks(const ptc_ctx *c, int nj);
+// persistent prefetching forms built for this x length (nx/2) / z length and tile
+bool fft2_has_px(int h);
+bool fft2_has_pz(int nz, int tile);
 int fft2_x_inv(ptc_ctx *c, const void *spec, double *dst, double scale, int j0, int nj, bool acc, bool ghost);
 int fft2_z(ptc_ctx *c, void *spec, bool inverse, int j0, int nj, bool f64 = false);
 

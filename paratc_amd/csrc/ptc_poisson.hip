@@ -63,6 +63,7 @@ int poisson_free_plan(ptc_ctx *c) {
   keep.tri_lpb_req = P.tri_lpb_req;
   keep.tri_cl_req = P.tri_cl_req;
   keep.stream_req = P.stream_req;
+  keep.persist_req = P.persist_req;
   keep.sweeps = P.sweeps;
   keep.ztile_req = P.ztile_req;
   keep.zfast_req = P.zfast_req;
@@ -118,6 +119,9 @@ int poisson_build_plan(ptc_ctx *c) {
   // then reads a slab the x pass has just written while it is still held in
   // the device's last-level cache, and likewise on the way back
   P.slab = (P.x_fast && P.z_fast && P.slab_req > 0 && P.slab_req < c->ny) ? P.slab_req : 0;
+  // persistent prefetching x and z passes where built for these lengths
+  P.persist_x = P.persist_req > 0 && P.x_fast && fft2_has_px(P.h);
+  P.persist_z = P.persist_req > 0 && P.z_fast && fft2_has_pz(c->nz, P.ztile);
 
   if (P.prec == PTC_PREC_FP32_REFINE) {
     PTC_CHECK(P.x_fast && P.z_fast, PTC_E_UNSUPPORTED,

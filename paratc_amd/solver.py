@@ -154,6 +154,9 @@ class Solver:
     def poisson_streaming(self, mode: int = 0):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_STREAMING, mode), "ptc_poisson_set")
 
+    def poisson_persist(self, mode: int = 1):
+        check(self.L.ptc_poisson_set(self.h, capi.POISSON_PERSIST, mode), "ptc_poisson_set")
+
     def poisson_get_knob(self, knob: int) -> int:
         v = C.c_int()
         check(self.L.ptc_poisson_get_knob(self.h, knob, C.byref(v)), "ptc_poisson_get_knob")
@@ -167,7 +170,8 @@ class Solver:
                 "z_tile": o[2], "sweeps": o[3], "x_fast": bool(o[4] & 1), "z_fast": bool(o[4] & 2),
                 "chunk_rows": o[5], "chunks": o[6], "lanes_per_block": o[7], "slab_planes": o[8],
                 "tri_lds_lines": o[9], "tri_complex_lines": self.poisson_get_knob(capi.POISSON_TRI_COMPLEX),
-                "streaming": self.poisson_get_knob(capi.POISSON_STREAMING)}
+                "streaming": self.poisson_get_knob(capi.POISSON_STREAMING),
+                "persist": self.poisson_get_knob(capi.POISSON_PERSIST)}
 
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()

@@ -345,6 +345,57 @@ def test_streaming_passes_match_cached_ones(grid, prec):
     assert rel_err(outs[1], inner(ref)) <= (TOL if prec == capi.PREC_FP64 else 1e-10)
 
 
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_FP32_REFINE], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("grid,ztile,slab", [
+    ((1024, 6, 32), 0, 0),      # x 512: several lines per workgroup
+    ((9216, 6, 16), 0, 0),      # x 4608: one 576-thread line per workgroup
+    ((4608, 6, 16), 0, 2),      # x 2304, slabs of two planes
+    ((16, 6, 512), 8, 0), ((16, 6, 512), 16, 0),
+    ((16, 6, 700), 4, 0), ((32, 6, 700), 8, 0),
+    ((16, 6, 1400), 2, 0), ((16, 6, 1400), 4, 0),
+    ((1024, 6, 512), 8, 0)], ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+def test_persistent_passes_match_one_shot(grid, ztile, slab, prec):
+    """The persistent x/z passes that prefetch their next line or column tile
+    run the same operations as the one-workgroup-per-line kernels (the
+    compiler may fuse different multiply-adds in the two, so they agree to
+    rounding rather than bit for bit), and both match the oracle."""
+    nx, ny, nz = grid
+    case = CS.case_poisson(nx, ny, nz, 2.0)
+    f = ic.poisson_rhs(case, seed=12)
+    outs = []
+    with Solver(case) as s:
+        s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
+        for mode in (-1, 1):
+            # FP32 storage: two correction sweeps (one leaves 2e-10 at the
+            # 9216-point rows, either way)
+            s.poisson_config(prec, capi.TRI_AUTO, 0, 2)
+            if ztile:
+                s.poisson_fast_ztile(ztile)
+            if slab:
+                s.poisson_slab_planes(slab)
+            s.poisson_persist(mode)
+            got = s.poisson_get_knob(capi.POISSON_PERSIST)
+            if mode > 0:
+                # x: the 512-, 2304- and 4608-point lengths; z: 512, 700, 1400
+                want = (1 if nx // 2 in (512, 2304, 4608) else 0) | (2 if nz in (512, 700, 1400) else 0)
+                assert got == want, (got, want)
+            else:
+                assert got == 0
+            s.clcPPE(capi.F_ARR1, capi.F_HALO)
+            outs.append(s.download(capi.F_HALO, capi.L_INTERIOR))
+    # FP32 storage: the two round their FP32 spectra differently before the
+    # FP64 correction sweep
+    assert rel_err(outs[1], outs[0]) <= (1e-13 if prec == capi.PREC_FP64 else 1e-10)
+    orc = Oracle(case)
+    ref = orc.halo()
+    orc.clcPPE(f.copy(order="F"), ref, 1)
+    # FP32 storage: the defect correction stops at the accuracy of the FP64
+    # residual, about cond(L) eps; with the 9216-point rows (dx = 4 pi / 9216)
+    # that is 2.3e-10 relative, however many sweeps
+    for o in outs:
+        assert rel_err(o, inner(ref)) <= (TOL if prec == capi.PREC_FP64 else 5e-10)
+
+
 # ---- FP32 storage + FP64 defect correction ---------------------------------
 @pytest.mark.parametrize("grid,slab,lines", [((64, 48, 32, 2.0), 0, 0), ((128, 96, 128, 1.8), 0, 0),
                                              ((128, 96, 128, 1.8), 16, 0), ((96, 140, 64, 2.2), 0, 0),
