@@ -242,6 +242,18 @@ int ptc_clc_u3hat(ptc_ctx *ctx, int uz, int rhsz);                /* :442-471 */
 int ptc_clc_scalar(ptc_ctx *ctx, int scalar, int rhsc);           /* :477-506 */
 /* clcPrSrc (:48-77); divmax may be NULL (then no host synchronisation) */
 int ptc_clc_prsrc(ptc_ctx *ctx, int ux, int uy, int uz, int prsrc, double *divmax);
+/* ptc_clc_prsrc without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as ptc_clc_prsrc, with the per-workgroup partial
+ * maxima in a 16 MiB device array of the library instead of a buffer
+ * allocated on first use.  Limits: ceil(nx/64)*ceil(ny/4)*nz <= 2^21 (else
+ * PTC_E_ARG); the four fields must already have device memory (uploaded,
+ * filled or written before; else PTC_E_STATE).  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (contexts on different
+ * streams) race.  divmax = NULL under graph capture (a non-NULL divmax
+ * synchronises the stream). */
+int ptc_clc_prsrc_noalloc(ptc_ctx *ctx, int ux, int uy, int uz, int prsrc, double *divmax);
 int ptc_fluid_vel_update(ptc_ctx *ctx, int phi, int ux, int uy, int uz); /* :82-107 */
 int ptc_pressure_update(ptc_ctx *ctx, int pressure, int phi);           /* :512-532 */
 

@@ -43,7 +43,7 @@ EXPORTS = [
     "ptc_pm_coe_update", "ptc_get_coe", "ptc_set_coe", "ptc_init_poisson_solver", "ptc_clc_ppe",
     "ptc_destroy_poisson_plan", "ptc_poisson_config", "ptc_poisson_set", "ptc_poisson_get_knob", "ptc_poisson_get_config", "ptc_poisson_residual",
     "ptc_poisson_stage_times", "ptc_init_time_scheme", "ptc_clc_rhs_xyz", "ptc_clc_u1hat",
-    "ptc_clc_u2hat", "ptc_clc_u3hat", "ptc_clc_scalar", "ptc_clc_prsrc", "ptc_fluid_vel_update",
+    "ptc_clc_u2hat", "ptc_clc_u3hat", "ptc_clc_scalar", "ptc_clc_prsrc", "ptc_clc_prsrc_noalloc", "ptc_fluid_vel_update",
     "ptc_pressure_update", "ptc_set_bc_and_update_halo", "ptc_set_bc_and_update_halo_for_prsrc",
     "ptc_set_bc_and_update_halo_pr", "ptc_set_bc_and_update_halo_scalar", "ptc_update_uy_ym",
     "ptc_calc_max_cfl", "ptc_check_divergence", "ptc_calc_vmax", "ptc_calc_ux_aver",
@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
     L.ptc_clc_u3hat.argtypes = [vp, ip, ip]
     L.ptc_clc_scalar.argtypes = [vp, ip, ip]
     L.ptc_clc_prsrc.argtypes = [vp, ip, ip, ip, ip, dp]
+    L.ptc_clc_prsrc_noalloc.argtypes = [vp, ip, ip, ip, ip, dp]
     L.ptc_fluid_vel_update.argtypes = [vp, ip, ip, ip, ip]
     L.ptc_pressure_update.argtypes = [vp, ip, ip]
     L.ptc_set_bc_and_update_halo.argtypes = [vp, ip, ip, ip, ip]

@@ -65,6 +65,14 @@ int check_field(ptc_ctx *c, int id) {
   return PTC_OK;
 }
 
+int field_existing(ptc_ctx *c, int id, double **out) {
+  PTC_TRY(check_field(c, id));
+  PTC_CHECK(c->d_field[id] != nullptr, PTC_E_STATE,
+            "field " + std::to_string(id) + " has no device memory yet (upload or fill it first)");
+  *out = c->d_field[id];
+  return PTC_OK;
+}
+
 int field(ptc_ctx *c, int id, double **out) {
   PTC_TRY(check_field(c, id));
   if (!c->d_field[id]) {
@@ -306,6 +314,7 @@ int create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ct
     set_error("no HIP device");
     return fail(PTC_E_HIP);
   }
+  if (noalloc_init_device(c->device) != PTC_OK) return fail(PTC_E_HIP);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
     c->lds_limit = prop.sharedMemPerBlock;

@@ -191,7 +191,13 @@ namespace ptc {
 
 // field access (allocating on first use, zero-filled)
 int field(ptc_ctx *c, int id, double **out);
+// field access without allocating: PTC_E_STATE if the field has no device
+// memory yet
+int field_existing(ptc_ctx *c, int id, double **out);
 int check_field(ptc_ctx *c, int id);
+// resolve, once per device, the device address of the file-scope scratch of
+// the allocation-free entry points (ptc_noalloc.hip); the device is current
+int noalloc_init_device(int device);
 
 // The divergence of clcPrSrc formed in the x forward pass (src/m_TScheme.f90:48-77):
 // the pass transforms sudtal * div(ux, uy, uz) and writes each workgroup's
@@ -299,6 +305,11 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
                double *hx, double *hy, double *hz, double *hc, const double *p);
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host, bool reduce = true);
+// the same with the workgroup partials in part (room for prsrc_partials(c)
+// values) instead of the context's reduction buffer; allocates nothing
+int launch_prsrc_part(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
+                      double *divmax_host, bool reduce, double *part);
+int prsrc_partials(const ptc_ctx *c);
 // the divergence maximum over n partials into its device slot
 int launch_divmax(ptc_ctx *c, int n);
 int launch_vel_update(ptc_ctx *c, const double *phi, double *ux, double *uy, double *uz);

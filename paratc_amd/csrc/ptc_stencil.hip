@@ -933,13 +933,20 @@ int launch_divmax(ptc_ctx *c, int n) {
 // read back on the host)
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host, bool reduce) {
+  PTC_TRY(ensure_red(c, pt_nblocks(c)));
+  return launch_prsrc_part(c, ux, uy, uz, prsrc, divmax_host, reduce, c->d_red);
+}
+
+int prsrc_partials(const ptc_ctx *c) { return pt_nblocks(c); }
+
+int launch_prsrc_part(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
+                      double *divmax_host, bool reduce, double *part) {
   const int nb = pt_nblocks(c);
-  PTC_TRY(ensure_red(c, nb));
   hipLaunchKernelGGL(k_prsrc, pt_grid(c), dim3(64, 4), 0, c->stream, c->geo, ux, uy, uz, prsrc, 1.0 / c->pmAlpha,
-                     c->d_red);
+                     part);
   PTC_HIP(hipGetLastError());
   if (!reduce && !divmax_host) return PTC_OK;
-  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal + S_DIVMAX);
+  hipLaunchKernelGGL(k_final_max, dim3(1), dim3(kReduceThreads), 0, c->stream, part, nb, c->d_scal + S_DIVMAX);
   PTC_HIP(hipGetLastError());
   if (divmax_host) {
     PTC_HIP(hipMemcpyAsync(c->h_pinned, c->d_scal + S_DIVMAX, sizeof(double), hipMemcpyDeviceToHost, c->stream));
