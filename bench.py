@@ -48,11 +48,16 @@ def stage_kernels(plan: dict, nx: int, nz: int) -> list:
     d = "HIP_vector_type<double"
     h = nx // 2
     st = "true" if plan.get("streaming") else "false"
-    if plan["x_fast"]:
+    persist = plan.get("persist", 0)
+    if plan["x_fast"] and persist & 1:
+        xf, xi = f"k2p_fft_x_fwd<{h}, {d}, 2u>, {st}>", f"k2p_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
+    elif plan["x_fast"]:
         xf, xi = f"k2_fft_x_fwd<{h}, {d}, 2u>, {st}>", f"k2_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
     else:
         xf, xi = "k_fft_x_fwd<", "k_fft_x_inv<"
-    if plan["z_fast"]:
+    if plan["z_fast"] and persist & 2:
+        zf, zi = (f"k2p_fft_z<{nz}, {plan['z_tile']}, {inv}, {d}" for inv in ("false", "true"))
+    elif plan["z_fast"]:
         zf, zi = (f"k2_fft_z<{nz}, {plan['z_tile']}, {inv}, {d}" for inv in ("false", "true"))
     else:
         zf, zi = "k_fft_z<" + str(plan["z_tile"]) + ", false>", "k_fft_z<" + str(plan["z_tile"]) + ", true>"
@@ -286,6 +291,17 @@ def bench_c4a(ev, solves=10):
     ms64, st64, r64 = timed(stages=True)
     ms_plain, _, _ = timed()
     progress(f"9216x140x1400 FP64: {ms64:.3f} ms per solve (with pass events), {ms_plain:.3f} ms plain")
+    # per-kernel pass times for the roofline: a slabbed plan interleaves the
+    # x and z kernels slab by slab, so its pass events time them together;
+    # the same kernels are timed over whole fields (slabs off)
+    plan_k = plan64
+    st_k = st64
+    if plan64["slab_planes"]:
+        s.poisson_slab_planes(0)
+        plan_k = s.poisson_get_config()
+        _, st_k, _ = timed(stages=True)
+        s.poisson_slab_planes(plan64["slab_planes"])
+        progress(f"9216x140x1400 pass times without slabs: {[round(v, 3) for v in st_k]}")
     p64 = s.download(capi.F_HALO, capi.L_INTERIOR)
     scale = float(np.abs(p64).max())
     s.poisson_config(capi.PREC_FP32_REFINE, capi.TRI_AUTO, 0, 1)
@@ -295,19 +311,21 @@ def bench_c4a(ev, solves=10):
     progress(f"9216x140x1400 FP32 + 1 sweep: {ms32:.3f} ms per solve, field error {err32:.2e} vs FP64")
     s.close()
     ms = ms_plain
-    dom = int(np.argmax(st64)) if not plan64["slab_planes"] else 2
-    names = stage_kernels(plan64, case.nx, case.nz)
+    dom = int(np.argmax(st_k))
+    names = stage_kernels(plan_k, case.nx, case.nz)
     return {
         "workload": "Poisson 9216x140x1400, gamma 2.2, N(0,1) seed 4 minus dyp-weighted mean (seeded in HBM)",
         "fp64_ms_per_solve": round(ms, 3), "fp64_mpts_per_s": round(n / (ms * 1e-3) / 1e6, 1),
         "fp64_solves_per_s": round(1e3 / ms, 2),
         "fp64_hbm_gbs_alg": round(80.0 * n / (ms * 1e-3) / 1e9, 1),
         "fp64_frac_of_peak": round(80.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "stage_ms": {k: round(v, 4) for k, v in zip(STAGES, st64)},
-        "roofline": {"bound": "hbm", "kernel": names[dom], "launch_ms": round(st64[dom], 4),
+        "stage_ms": {k: round(v, 4) for k, v in zip(
+            ["xz_fwd", "-", "y_solve", "-", "zx_inv"] if plan64["slab_planes"] else STAGES, st64) if k != "-"},
+        "kernel_pass_ms": {k: round(v, 4) for k, v in zip(STAGES, st_k)},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "launch_ms": round(st_k[dom], 4),
                      "bytes_per_launch": 16 * n,
-                     "achieved": round(16.0 * n / (st64[dom] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(16.0 * n / (st64[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "achieved": round(16.0 * n / (st_k[dom] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(16.0 * n / (st_k[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fp32_refine_1_sweep_ms_per_solve": round(ms32, 3), "fp32_refine_speed_vs_fp64": round(ms / ms32, 3),
         "fp32_refine_field_error_vs_fp64": err32,
         "residual_fp64": r64, "residual_fp32_refine": r32, "plan_fp64": plan64,
@@ -377,9 +395,9 @@ def bench_gravity_substep(ev, grid, gamma=1.6, steps=5, fp32=True):
 
 
 def copy_ceiling(r, ev, reps=20):
-    """Measured copy bandwidth on this part: the library's plain streaming
-    copy (k_copy, 16-byte lanes) of one 1024 x 258 x 512 field, read once and
-    written once."""
+    """Measured copy bandwidth on this part: the library's streaming copy
+    (k_copy_nt: nontemporal 16-byte loads and stores, four in flight per
+    thread) of one 1024 x 258 x 512 field, read once and written once."""
     from paratc_amd import capi
 
     s = r["solver"]
@@ -393,7 +411,7 @@ def copy_ceiling(r, ev, reps=20):
     ev.record(e1, stream)
     ms = ev.elapsed_ms(e0, e1) / reps
     nbytes = 2 * 8 * 1024 * 258 * 512
-    return {"kernel": "k_copy", "bytes_per_launch": nbytes, "ms": round(ms, 4),
+    return {"kernel": "k_copy_nt", "bytes_per_launch": nbytes, "ms": round(ms, 4),
             "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
@@ -532,6 +550,9 @@ def main():
     ap.add_argument("--no-spectra", action="store_true", help="skip the statistics-spectra leg")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 9216x140x1400 Poisson and the 4608x140x700 and 1536x192x256 RK legs")
+    ap.add_argument("--large-legs", default="c4a,c4b,c2",
+                    help="large legs to run: c4a (9216x140x1400 Poisson), c4b (4608x140x700 substep), "
+                         "c2 (1536x192x256 substep)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -552,9 +573,16 @@ def main():
     large = None
     if not args.no_large:
         r["solver"].close()  # free the 1024x256x512 fields before the large case
-        large = {"poisson_9216x140x1400": bench_c4a(ev),
-                 "rk_substep_4608x140x700": bench_gravity_substep(ev, (4608, 140, 700), steps=3),
-                 "rk_substep_1536x192x256": bench_gravity_substep(ev, (1536, 192, 256), steps=5)}
+        legs = set(args.large_legs.split(","))
+        large = {}
+        if "c4a" in legs:
+            large["poisson_9216x140x1400"] = bench_c4a(ev)
+        if "c4b" in legs:
+            large["rk_substep_4608x140x700"] = bench_gravity_substep(ev, (4608, 140, 700), steps=3,
+                                                                     fp32=not args.no_fp32)
+        if "c2" in legs:
+            large["rk_substep_1536x192x256"] = bench_gravity_substep(ev, (1536, 192, 256), steps=5,
+                                                                     fp32=not args.no_fp32)
     n = case.npts
     kernel = stage_kernels(r["plan"], case.nx, case.nz)[r["dom"]]
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /

@@ -33,19 +33,44 @@ int fill_device(ptc_ctx *c, double *a, long long n, double v) {
   return PTC_OK;
 }
 
-// plain streaming copy (also the reference kernel for HBM counter calibration)
+// plain copy (fields that fit the last-level cache)
 __global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ d, const double2 *__restrict__ s, long long n2) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += (long long)gridDim.x * blockDim.x)
     d[q] = s[q];
 }
+
+// streaming copy of fields larger than the last-level cache: nontemporal
+// 16-byte loads and stores, four per thread in flight per iteration
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_copy_nt(double2 *__restrict__ d, const double2 *__restrict__ s, long long n2) {
+  const nt_d2 *sv = reinterpret_cast<const nt_d2 *>(s);
+  nt_d2 *dv = reinterpret_cast<nt_d2 *>(d);
+  const long long st = (long long)gridDim.x * blockDim.x;
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; q + 3 * st < n2; q += 4 * st) {
+    const nt_d2 a = __builtin_nontemporal_load(sv + q), b = __builtin_nontemporal_load(sv + q + st);
+    const nt_d2 e = __builtin_nontemporal_load(sv + q + 2 * st), f = __builtin_nontemporal_load(sv + q + 3 * st);
+    __builtin_nontemporal_store(a, dv + q);
+    __builtin_nontemporal_store(b, dv + q + st);
+    __builtin_nontemporal_store(e, dv + q + 2 * st);
+    __builtin_nontemporal_store(f, dv + q + 3 * st);
+  }
+  for (; q < n2; q += st) __builtin_nontemporal_store(__builtin_nontemporal_load(sv + q), dv + q);
+}
 __global__ void k_copy_tail(double *__restrict__ d, const double *__restrict__ s, long long n) { d[n - 1] = s[n - 1]; }
+
+constexpr long long kStreamCopyDoubles = 32ll << 20;  // 256 MB
 
 int copy_device(ptc_ctx *c, double *dst, const double *src, long long n) {
   const long long n2 = n / 2;
   const int nblk = (int)std::min<long long>((n2 + 255) / 256, (long long)c->n_cu * 32);
   if (n2 > 0) {
-    hipLaunchKernelGGL(k_copy, dim3(std::max(1, nblk)), dim3(256), 0, c->stream, reinterpret_cast<double2 *>(dst),
-                       reinterpret_cast<const double2 *>(src), n2);
+    if (n >= kStreamCopyDoubles)
+      hipLaunchKernelGGL(k_copy_nt, dim3(std::max(1, nblk)), dim3(256), 0, c->stream, reinterpret_cast<double2 *>(dst),
+                         reinterpret_cast<const double2 *>(src), n2);
+    else
+      hipLaunchKernelGGL(k_copy, dim3(std::max(1, nblk)), dim3(256), 0, c->stream, reinterpret_cast<double2 *>(dst),
+                         reinterpret_cast<const double2 *>(src), n2);
     PTC_HIP(hipGetLastError());
   }
   if (n % 2) {

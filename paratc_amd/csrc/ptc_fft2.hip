@@ -382,19 +382,6 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
 
 }  // namespace f2
 
-// Workgroups go to the 8 XCDs round-robin by linear id (w -> XCD w % 8), and
-// each XCD has its own L2.  xcd_group maps w to a work index so that GRP
-// consecutive work indices run on one XCD as consecutive workgroups there;
-// the last (nwg mod 8 GRP) ids keep their own index.
-template <int GRP>
-__device__ inline int xcd_group(int w, int nwg) {
-  if (GRP == 1) return w;
-  constexpr int span = 8 * GRP;
-  if (w >= (nwg / span) * span) return w;
-  const int x = w % 8, s = w / 8;
-  return GRP * (x + 8 * (s / GRP)) + s % GRP;
-}
-
 // threads per line: the largest-radix stage has one butterfly per thread
 template <int N>
 constexpr int f2_threads() {

@@ -69,7 +69,7 @@ def _counter_means(path: str) -> Dict[str, Tuple[int, float]]:
 
 def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
     fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
-    copy = [k for k in fetch if k.startswith("k_copy @")]
+    copy = [k for k in fetch if k.startswith("k_copy @") or k.startswith("k_copy_nt @")]
     if not copy or copy[0] not in write:
         raise SystemExit("k_copy is not in the counter files: profile bench.py --calib-copy")
     rf = nbytes / (fetch[copy[0]][1] * 1024.0)
