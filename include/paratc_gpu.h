@@ -382,12 +382,18 @@ int ptc_seed_fields(ptc_ctx *ctx, int kind, int seed, double amp, double x0, dou
  *  With a fixed time step (iCFL != 1) the case's step is captured once in a
  *  HIP graph and each later step is one graph launch on the case's stream;
  *  CFL-adaptive cases are stepped with ordinary launches.
- *  ptc_ensemble_step advances every admitted case by nsteps steps (rounds of
- *  one step per case, cases overlapping on their streams) and returns when
- *  all are done.  Each case-step's latency is measured from the start of its
- *  round; ptc_ensemble_latency gives the 50th and 99th percentiles (ms).
- *  ptc_ensemble_release removes a case and hands its context back (ctx !=
- *  NULL) or destroys it.  One host thread per ensemble. */
+ *  ptc_ensemble_step enqueues nsteps steps of every admitted case (one step
+ *  per case in turn, each case on its own stream) and returns without a host
+ *  barrier: a case runs ahead by at most four steps, beyond which the call
+ *  waits on that case's own oldest step only.  Work on a case's context
+ *  (downloads, monitors) is ordered after its enqueued steps.
+ *  ptc_ensemble_sync waits for every enqueued step.  A case-step's latency
+ *  is the time from the completion of the case's previous step (or its
+ *  admission) to its own; ptc_ensemble_latency waits for the enqueued steps
+ *  and gives the 50th and 99th percentiles (ms).  ptc_ensemble_case's steps
+ *  count the enqueued steps.  ptc_ensemble_release waits for the case's
+ *  steps, removes it and hands its context back (ctx != NULL) or destroys
+ *  it.  One host thread per ensemble. */
 typedef struct ptc_ensemble ptc_ensemble;
 int ptc_ensemble_create(size_t hbm_budget_bytes, ptc_ensemble **out);
 int ptc_ensemble_destroy(ptc_ensemble *e);
@@ -395,6 +401,7 @@ int ptc_ensemble_destroy(ptc_ensemble *e);
 int ptc_case_footprint(ptc_ctx *ctx, size_t *bytes);
 int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id);
 int ptc_ensemble_step(ptc_ensemble *e, int nsteps);
+int ptc_ensemble_sync(ptc_ensemble *e);
 int ptc_ensemble_case(ptc_ensemble *e, int case_id, ptc_ctx **ctx, long long *steps);
 int ptc_ensemble_release(ptc_ensemble *e, int case_id, ptc_ctx **ctx);
 int ptc_ensemble_usage(ptc_ensemble *e, size_t *used_bytes, size_t *budget_bytes, int *live_cases);

@@ -50,7 +50,7 @@ EXPORTS = [
     "ptc_clc_mean_value", "ptc_channel_iterate", "ptc_get_scalar", "ptc_clc_stat", "ptc_get_stat", "ptc_init_spectra", "ptc_clc_spectra",
     "ptc_get_spectra", "ptc_visu_field", "ptc_step_graphs",
     "ptc_ensemble_create", "ptc_ensemble_destroy", "ptc_case_footprint", "ptc_ensemble_admit",
-    "ptc_ensemble_step", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
+    "ptc_ensemble_step", "ptc_ensemble_sync", "ptc_ensemble_case", "ptc_ensemble_release", "ptc_ensemble_usage",
     "ptc_ensemble_latency", "ptc_seed_normal", "ptc_seed_fields",
 ]
 
@@ -151,6 +151,7 @@ def load(path: str = LIB_PATH):
     L.ptc_case_footprint.argtypes = [vp, C.POINTER(sz)]
     L.ptc_ensemble_admit.argtypes = [vp, vp, C.POINTER(ip)]
     L.ptc_ensemble_step.argtypes = [vp, ip]
+    L.ptc_ensemble_sync.argtypes = [vp]
     L.ptc_ensemble_case.argtypes = [vp, ip, C.POINTER(vp), C.POINTER(ll)]
     L.ptc_ensemble_release.argtypes = [vp, ip, C.POINTER(vp)]
     L.ptc_ensemble_usage.argtypes = [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(ip)]

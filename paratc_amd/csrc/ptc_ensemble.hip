@@ -12,10 +12,16 @@
 //    cases with CFL-adaptive dt (host-side dt from a device reduction) are
 //    stepped with ordinary launches;
 //  * cases run on their own streams, so small grids that cannot fill the
-//    chip on their own overlap each other;
-//  * every case-step records a completion event; its latency is measured
-//    from the start of the round that issued it (p50 / p99 on request).
+//    chip on their own overlap each other, and no host barrier separates one
+//    round of steps from the next: ptc_ensemble_step enqueues and returns,
+//    each case running ahead on its stream by at most kMaxAhead steps (the
+//    host waits only on that case's own older step);
+//  * every case-step records a completion event on the case's stream; its
+//    latency is the time from the case's previous step completion (or its
+//    admission) to its own, read once both events have completed (p50 / p99
+//    on request).
 #include <algorithm>
+#include <deque>
 #include <vector>
 
 #include "ptc_internal.hpp"
@@ -27,17 +33,20 @@ struct ptc_ensemble {
     hipGraphExec_t exec = nullptr;
     unsigned long long gen = 0;  // the context's buffer generation at capture
     size_t bytes = 0;
-    long long steps = 0;
+    long long steps = 0;          // steps enqueued
     bool live = false;
+    hipEvent_t prev = nullptr;    // completion of the case's previous step (or its admission)
+    std::deque<hipEvent_t> inflight;  // step completions not yet harvested, oldest first
   };
   size_t budget = 0, used = 0;
   std::vector<Case> cases;
-  hipStream_t side = nullptr;   // round-start events
   std::vector<hipEvent_t> pool;  // reusable events
   std::vector<double> lat_ms;    // per case-step latencies since the last reset
 };
 
 namespace ptc {
+
+constexpr size_t kMaxAhead = 4;  // steps in flight per case
 
 // device bytes a context holds (fields, reduction partials, Poisson plan)
 static size_t ctx_bytes(const ptc_ctx *c) {
@@ -58,6 +67,43 @@ static int take_event(ptc_ensemble *e, hipEvent_t *ev) {
     return PTC_OK;
   }
   PTC_HIP(hipEventCreate(ev));
+  return PTC_OK;
+}
+
+// Harvest the case's completed step events in order: each one's latency from
+// the previous completion.  block: wait for the oldest (then harvest what else
+// has completed); all: wait for every one.
+static int harvest(ptc_ensemble *e, ptc_ensemble::Case &k, bool block, bool all) {
+  bool first = true;
+  while (!k.inflight.empty()) {
+    hipEvent_t ev = k.inflight.front();
+    if ((block && first) || all) {
+      PTC_HIP(hipEventSynchronize(ev));
+    } else {
+      const hipError_t q = hipEventQuery(ev);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();
+        break;
+      }
+      PTC_HIP(q);
+    }
+    first = false;
+    float ms = 0.f;
+    if (k.prev) {
+      PTC_HIP(hipEventElapsedTime(&ms, k.prev, ev));
+      e->lat_ms.push_back(ms);
+      e->pool.push_back(k.prev);
+    }
+    k.prev = ev;
+    k.inflight.pop_front();
+  }
+  return PTC_OK;
+}
+
+// every admitted case's steps done, every latency read
+static int sync_all(ptc_ensemble *e) {
+  for (auto &k : e->cases)
+    if (k.live) PTC_TRY(harvest(e, k, true, true));
   return PTC_OK;
 }
 
@@ -97,11 +143,6 @@ int ptc_ensemble_create(size_t hbm_budget, ptc_ensemble **out) {
   *out = nullptr;
   auto *e = new ptc_ensemble();
   e->budget = hbm_budget;
-  if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess) {
-    delete e;
-    set_error("hipStreamCreate failed");
-    return PTC_E_HIP;
-  }
   *out = e;
   return PTC_OK;
 }
@@ -111,10 +152,11 @@ int ptc_ensemble_destroy(ptc_ensemble *e) {
   for (auto &k : e->cases) {
     if (k.ctx) (void)hipStreamSynchronize(k.ctx->stream);
     drop_case_graph(k);
+    for (hipEvent_t ev : k.inflight) (void)hipEventDestroy(ev);
+    if (k.prev) (void)hipEventDestroy(k.prev);
     if (k.ctx) ptc_destroy(k.ctx);
   }
   for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
-  if (e->side) (void)hipStreamDestroy(e->side);
   delete e;
   return PTC_OK;
 }
@@ -145,6 +187,9 @@ int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id) {
     PTC_TRY(capture_step(ctx, &k.graph, &k.exec));
     k.gen = ctx->buf_gen;
   }
+  // the first step's latency runs from here
+  PTC_TRY(take_event(e, &k.prev));
+  PTC_HIP(hipEventRecord(k.prev, ctx->stream));
   e->used += b;
   int id = -1;
   for (size_t q = 0; q < e->cases.size(); ++q)
@@ -165,39 +210,30 @@ int ptc_ensemble_admit(ptc_ensemble *e, ptc_ctx *ctx, int *case_id) {
 int ptc_ensemble_step(ptc_ensemble *e, int nsteps) {
   PTC_CHECK(e, PTC_E_ARG, "null ensemble");
   PTC_CHECK(nsteps >= 1, PTC_E_ARG, "nsteps must be >= 1");
+  // steps go out case by case, round-robin, so the streams advance together
   for (int s = 0; s < nsteps; ++s) {
-    hipEvent_t start;
-    PTC_TRY(take_event(e, &start));
-    PTC_HIP(hipEventRecord(start, e->side));
-    std::vector<hipEvent_t> done;
-    done.reserve(e->cases.size());
-    int rc = PTC_OK;
     for (auto &k : e->cases) {
       if (!k.live) continue;
-      rc = step_case(k);
-      if (rc != PTC_OK) break;
+      if (k.inflight.size() >= kMaxAhead) PTC_TRY(harvest(e, k, true, false));
+      PTC_TRY(step_case(k));
       hipEvent_t ev;
-      rc = take_event(e, &ev);
-      if (rc != PTC_OK) break;
+      PTC_TRY(take_event(e, &ev));
       if (hipEventRecord(ev, k.ctx->stream) != hipSuccess) {
         e->pool.push_back(ev);
         set_error("hipEventRecord failed");
-        rc = PTC_E_HIP;
-        break;
+        return PTC_E_HIP;
       }
-      done.push_back(ev);
+      k.inflight.push_back(ev);
     }
-    // wait for everything this round launched, also after a failure
-    for (hipEvent_t ev : done) {
-      float ms = 0.f;
-      if (hipEventSynchronize(ev) == hipSuccess && hipEventElapsedTime(&ms, start, ev) == hipSuccess)
-        e->lat_ms.push_back(ms);
-      e->pool.push_back(ev);
-    }
-    e->pool.push_back(start);
-    if (rc != PTC_OK) return rc;
   }
+  for (auto &k : e->cases)
+    if (k.live) PTC_TRY(harvest(e, k, false, false));
   return PTC_OK;
+}
+
+int ptc_ensemble_sync(ptc_ensemble *e) {
+  PTC_CHECK(e, PTC_E_ARG, "null ensemble");
+  return sync_all(e);
 }
 
 int ptc_ensemble_case(ptc_ensemble *e, int case_id, ptc_ctx **ctx, long long *steps) {
@@ -212,7 +248,10 @@ int ptc_ensemble_release(ptc_ensemble *e, int case_id, ptc_ctx **ctx) {
   PTC_CHECK(e && case_id >= 0 && case_id < (int)e->cases.size() && e->cases[case_id].live, PTC_E_ARG,
             "unknown case");
   auto &k = e->cases[case_id];
+  PTC_TRY(harvest(e, k, true, true));
   PTC_HIP(hipStreamSynchronize(k.ctx->stream));
+  if (k.prev) e->pool.push_back(k.prev);
+  k.prev = nullptr;
   drop_case_graph(k);
   e->used -= k.bytes;
   if (ctx) {
@@ -236,6 +275,7 @@ int ptc_ensemble_usage(ptc_ensemble *e, size_t *used, size_t *budget, int *live)
 
 int ptc_ensemble_latency(ptc_ensemble *e, int reset, double *p50, double *p99, long long *nsamples) {
   PTC_CHECK(e, PTC_E_ARG, "null ensemble");
+  PTC_TRY(sync_all(e));
   std::vector<double> v = e->lat_ms;
   if (nsamples) *nsamples = (long long)v.size();
   auto pct = [&](double q) -> double {

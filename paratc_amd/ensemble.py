@@ -2,7 +2,8 @@
 
 ``Ensemble`` wraps ptc_ensemble_* of the C-ABI: admission by HBM footprint,
 one graph launch per case per step for fixed-dt cases, cases overlapping on
-their own streams, per case-step latency percentiles.
+their own streams with no host barrier between rounds, per case-step latency
+(from the case's previous step) percentiles.
 
 ``c3_workload`` draws the 64-case closed-channel ensemble: with the project
 RNG (seed 3) nx from {64, 96, 128, 192, 256}, ny from {48, 64, 96, 128}, nz
@@ -110,7 +111,12 @@ class Ensemble:
         return cid.value
 
     def step(self, nsteps: int = 1):
+        """Enqueue nsteps steps of every admitted case (no host barrier)."""
         check(self.L.ptc_ensemble_step(self.h, nsteps), "ptc_ensemble_step")
+
+    def sync(self):
+        """Wait for every enqueued step."""
+        check(self.L.ptc_ensemble_sync(self.h), "ptc_ensemble_sync")
 
     def steps_done(self, cid: int) -> int:
         n = C.c_longlong()
@@ -157,14 +163,29 @@ class _View(Solver):
         self.h = None
 
 
+# algorithmic HBM bytes per grid point of one RK substep (the FP64 path:
+# right-hand side, Crank-Nicolson solves, Poisson with the source formed in
+# its x pass, update): 344 without the scalar, 432 with it
+SUBSTEP_BYTES = {False: 344.0, True: 432.0}
+
+
+def step_bytes(case: CS.Case) -> float:
+    """Algorithmic bytes of one step (three RK substeps) of a case."""
+    return 3.0 * SUBSTEP_BYTES[bool(case.scalar)] * case.npts
+
+
 def serve(wl: Workload, steps: int, budget_bytes: int, solvers: Sequence[Solver], log=None) -> dict:
     """Replay the arrival process: a case becomes admissible at its arrival
-    time, waits in a queue while the HBM budget is full, runs `steps` steps in
-    rounds with every live case, and leaves.  Returns throughput and latency."""
+    time, waits in a queue while the HBM budget is full, runs `steps` steps
+    alongside every live case, and leaves.  The loop enqueues one step of
+    every live case per pass (each case runs ahead on its own stream, bounded
+    by the ensemble), so arrivals are admitted while earlier steps run.
+    Returns throughput and latency."""
     order = list(np.argsort(wl.arrivals, kind="stable"))
     queue: List[int] = []
     live: Dict[int, int] = {}  # ensemble id -> workload index
     done_pts = 0.0
+    done_bytes = 0.0
     case_steps = 0
     admit_wait = []
     with Ensemble(budget_bytes) as ens:
@@ -191,15 +212,18 @@ def serve(wl: Workload, steps: int, budget_bytes: int, solvers: Sequence[Solver]
                 q = live[cid]
                 case_steps += 1
                 done_pts += wl.cases[q].npts
+                done_bytes += step_bytes(wl.cases[q])
                 if ens.steps_done(cid) >= steps:
                     ens.release(cid)
                     del live[cid]
                     if log:
                         log(f"case {q} ({wl.cases[q].nx}x{wl.cases[q].ny}x{wl.cases[q].nz}) done")
+        ens.sync()
         wall = time.perf_counter() - t0
         p50, p99, n = ens.latency()
     return {"wall_s": wall, "case_steps": case_steps, "case_steps_per_s": case_steps / wall,
-            "mpts_steps_per_s": done_pts / wall / 1e6, "p50_ms": p50, "p99_ms": p99, "samples": n,
+            "mpts_steps_per_s": done_pts / wall / 1e6, "alg_gbs": done_bytes / wall / 1e9,
+            "p50_ms": p50, "p99_ms": p99, "samples": n,
             "admit_wait_p99_s": float(np.percentile(admit_wait, 99)) if admit_wait else 0.0}
 
 
