@@ -2,6 +2,9 @@
 
     python -m paratc_amd.profsum trace <run_results.db>
         per-kernel dispatch count and durations from a --kernel-trace run
+    python -m paratc_amd.profsum lds <lds.csv>
+        per-kernel mean SQ_LDS_BANK_CONFLICT and SQ_ACTIVE_INST_LDS from one
+        --pmc run of both counters, and their ratio
     python -m paratc_amd.profsum traffic <fetch.csv> <write.csv> <copy_bytes>
         per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
         --pmc runs, scaled to HBM bytes by what the same counters read for
@@ -58,13 +61,30 @@ def trace_table(db: str) -> str:
     return "\n".join(out)
 
 
-def _counter_means(path: str) -> Dict[str, Tuple[int, float]]:
-    """Mean counter value per kernel and grid size ("name @ grid")."""
+def _counter_means(path: str, counter: Optional[str] = None) -> Dict[str, Tuple[int, float]]:
+    """Mean counter value per kernel and grid size ("name @ grid"); with
+    `counter`, only that counter's rows."""
     acc: Dict[str, List[float]] = collections.defaultdict(list)
     with open(path) as fh:
         for r in csv.DictReader(fh):
+            if counter is not None and r["Counter_Name"] != counter:
+                continue
             acc[f"{short_name(r['Kernel_Name'])} @ {r['Grid_Size']}"].append(float(r["Counter_Value"]))
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def lds_table(path: str) -> str:
+    """LDS bank-conflict cycles against LDS-active cycles per kernel (kernels
+    with LDS traffic only)."""
+    conf = _counter_means(path, "SQ_LDS_BANK_CONFLICT")
+    act = _counter_means(path, "SQ_ACTIVE_INST_LDS")
+    out = ["| kernel @ grid (threads) | dispatches | SQ_LDS_BANK_CONFLICT | SQ_ACTIVE_INST_LDS | ratio |",
+           "|---|---:|---:|---:|---:|"]
+    for k in sorted(act, key=lambda k: -act[k][1]):
+        if act[k][1] <= 0 or k not in conf:
+            continue
+        out.append(f"| {k} | {act[k][0]} | {conf[k][1]:.0f} | {act[k][1]:.0f} | {conf[k][1] / act[k][1]:.3f} |")
+    return "\n".join(out)
 
 
 def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
@@ -105,6 +125,8 @@ def read_traffic(md_path: str, kernel: str) -> Optional[float]:
 def main(argv: List[str]) -> None:
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace_table(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "lds":
+        print(lds_table(argv[1]))
     elif len(argv) >= 4 and argv[0] == "traffic":
         print(traffic_table(argv[1], argv[2], float(argv[3])))
     else:
