@@ -217,6 +217,26 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value);
  * workgroup, streaming on (1) or off (0), persistent passes (1 x, 2 z,
  * 3 both); 0 where the knob's variant is not in use) */
 int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
+/* Workgroup shape of the row-marching clcRhsXYZ kernel (0 = automatic):
+ *   PTC_RHS_SEGMENT_THREADS  threads per row segment where rows are longer
+ *                            than 512 points: 64, 128 or 256 (automatic 128)
+ *   PTC_RHS_ROWS             rows j per workgroup, at most 512 threads in
+ *                            all: the inner rows' j-1 / j+1 values come
+ *                            through LDS (automatic: 2 where a row or
+ *                            segment is at most 128 threads, else 1)
+ *   PTC_RHS_XCD_RUN          consecutive tiles run on one XCD (1 = plain
+ *                            order; automatic 32)
+ * A change drops the captured step graph.  Results do not depend on the
+ * shape except through the order of the IsUxConst flow-rate sum. */
+enum {
+  PTC_RHS_SEGMENT_THREADS = 1,
+  PTC_RHS_ROWS = 2,
+  PTC_RHS_XCD_RUN = 3
+};
+int ptc_rhs_set(ptc_ctx *ctx, int knob, int value);
+/* the shape in use for this grid: threads per row segment (the padded row
+ * where a row is one segment), rows per workgroup, tiles per XCD run */
+int ptc_rhs_get(ptc_ctx *ctx, int knob, int *value);
 /* the plan in use: out10 = [prec, tri, z tile, sweeps, specialised-kernel
  * mask (1 = x pass, 2 = z pass), chunk rows, chunks per line, lanes per
  * workgroup, slab planes, LDS-staged lines (0 = register kernel)] (after

@@ -32,6 +32,7 @@ TRI_AUTO, TRI_THOMAS, TRI_PARTITIONED = 0, 1, 2
  VISU_LAMBDA2) = range(11)
 (POISSON_CHUNK_ROWS, POISSON_SLAB_PLANES, POISSON_TRI_LINES, POISSON_FAST_ZTILE, POISSON_TRI_LANES,
  POISSON_TRI_COMPLEX, POISSON_STREAMING, POISSON_PERSIST) = 1, 2, 3, 4, 5, 6, 7, 8
+RHS_SEGMENT_THREADS, RHS_ROWS, RHS_XCD_RUN = 1, 2, 3
 S_DIVMAX_PRSRC, S_PRGRADAVER, S_SIMTIME, S_DT = 0, 1, 2, 3
 
 # every symbol declared in include/paratc_gpu.h
@@ -41,7 +42,7 @@ EXPORTS = [
     "ptc_create", "ptc_destroy", "ptc_last_error", "ptc_abi_version", "ptc_upload", "ptc_download",
     "ptc_fill", "ptc_copy", "ptc_field_device_ptr", "ptc_synchronize", "ptc_get_stream", "ptc_get_metric",
     "ptc_pm_coe_update", "ptc_get_coe", "ptc_set_coe", "ptc_init_poisson_solver", "ptc_clc_ppe",
-    "ptc_destroy_poisson_plan", "ptc_poisson_config", "ptc_poisson_set", "ptc_poisson_get_knob", "ptc_poisson_get_config", "ptc_poisson_residual",
+    "ptc_destroy_poisson_plan", "ptc_poisson_config", "ptc_poisson_set", "ptc_poisson_get_knob", "ptc_rhs_set", "ptc_rhs_get", "ptc_poisson_get_config", "ptc_poisson_residual",
     "ptc_poisson_stage_times", "ptc_init_time_scheme", "ptc_clc_rhs_xyz", "ptc_clc_u1hat",
     "ptc_clc_u2hat", "ptc_clc_u3hat", "ptc_clc_scalar", "ptc_clc_prsrc", "ptc_clc_prsrc_noalloc", "ptc_fluid_vel_update",
     "ptc_pressure_update", "ptc_set_bc_and_update_halo", "ptc_set_bc_and_update_halo_for_prsrc",
@@ -113,6 +114,8 @@ def load(path: str = LIB_PATH):
     L.ptc_poisson_get_config.argtypes = [vp, C.POINTER(ip)]
     L.ptc_poisson_set.argtypes = [vp, ip, ip]
     L.ptc_poisson_get_knob.argtypes = [vp, ip, C.POINTER(ip)]
+    L.ptc_rhs_set.argtypes = [vp, ip, ip]
+    L.ptc_rhs_get.argtypes = [vp, ip, C.POINTER(ip)]
     L.ptc_poisson_stage_times.argtypes = [vp, ip, C.POINTER(C.c_float), C.POINTER(C.c_int)]
     L.ptc_init_time_scheme.argtypes = [vp]
     L.ptc_clc_rhs_xyz.argtypes = [vp] + [ip] * 13

@@ -526,6 +526,40 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value) {
   return PTC_OK;
 }
 
+int ptc_rhs_set(ptc_ctx *ctx, int knob, int value) {
+  PTC_CTX(ctx);
+  if (knob == PTC_RHS_SEGMENT_THREADS) {
+    PTC_CHECK(value == 0 || value == 64 || value == 128 || value == 256, PTC_E_ARG,
+              "RHS segment threads must be 0, 64, 128 or 256");
+  } else if (knob == PTC_RHS_ROWS) {
+    PTC_CHECK(value >= 0 && value <= 8, PTC_E_ARG, "RHS rows per workgroup must be in 0..8");
+  } else if (knob == PTC_RHS_XCD_RUN) {
+    PTC_CHECK(value >= 0 && value <= 256, PTC_E_ARG, "RHS tiles per XCD run must be in 0..256");
+  } else {
+    PTC_CHECK(false, PTC_E_ARG, "unknown RHS knob");
+  }
+  PTC_HIP(hipStreamSynchronize(ctx->stream));
+  if (knob == PTC_RHS_SEGMENT_THREADS) ctx->rhs_seg_req = value;
+  else if (knob == PTC_RHS_ROWS) ctx->rhs_rows_req = value;
+  else ctx->rhs_run_req = value;
+  // captured steps hold the old launch shape and reduction-partial count
+  drop_step_graph(ctx);
+  ++ctx->buf_gen;
+  return PTC_OK;
+}
+
+int ptc_rhs_get(ptc_ctx *ctx, int knob, int *value) {
+  PTC_CTX(ctx);
+  PTC_CHECK(value != nullptr, PTC_E_ARG, "null output");
+  int tx = 0, ty = 0, run = 0;
+  rhs_shape_in_use(ctx, &tx, &ty, &run);
+  if (knob == PTC_RHS_SEGMENT_THREADS) *value = tx;
+  else if (knob == PTC_RHS_ROWS) *value = ty;
+  else if (knob == PTC_RHS_XCD_RUN) *value = run;
+  else PTC_CHECK(false, PTC_E_ARG, "unknown RHS knob");
+  return PTC_OK;
+}
+
 int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
   PTC_CTX(ctx);
   PTC_CHECK(o != nullptr, PTC_E_ARG, "null output");

@@ -185,6 +185,8 @@ struct ptc_ctx {
   // before a replay when the two differ.
   unsigned long long buf_gen = 1;
   unsigned long long step_gen = 0;       // generation of step_graph
+  // clcRhsXYZ workgroup shape requests (0 = automatic; ptc_rhs_set)
+  int rhs_seg_req = 0, rhs_rows_req = 0, rhs_run_req = 0;
 };
 
 namespace ptc {
@@ -200,6 +202,15 @@ __device__ inline int xcd_group(int w, int nwg) {
   if (w >= (nwg / span) * span) return w;
   const int x = w % 8, s = w / 8;
   return GRP * (x + 8 * (s / GRP)) + s % GRP;
+}
+
+// xcd_group with a run length chosen at launch (1: identity)
+__device__ inline int xcd_run(int w, int nwg, int grp) {
+  if (grp <= 1) return w;
+  const int span = 8 * grp;
+  if (w >= (nwg / span) * span) return w;
+  const int x = w % 8, s = w / 8;
+  return grp * (x + 8 * (s / grp)) + s % grp;
 }
 
 // field access (allocating on first use, zero-filled)
@@ -260,6 +271,9 @@ void drop_step_graph(ptc_ctx *c);
 // the reduction partials d_red with room for n values; growing them bumps the
 // buffer generation and drops the context's captured step
 int ensure_red(ptc_ctx *c, int n);
+// the clcRhsXYZ workgroup shape in use: threads per row segment (the padded
+// row where a row is one segment), rows per workgroup, tiles per XCD run
+void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 

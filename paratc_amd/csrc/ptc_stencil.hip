@@ -109,6 +109,8 @@ enum {
   X_UYP,     // uy(., j + 1, k)
   X_R,       // uzhx * (uxhz + ucrf) at (., j, k)
   X_SC,      // scalar(., j, k)
+  X_UYF,     // uy(., j, k + 2): the next window entry of row j - 1's uy(j + 1) window
+  X_UZF,     // uz(., j, k + 3): the next window entry of row j + 1's uz(j - 1) window
   X_NROWS
 };
 
@@ -135,26 +137,57 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
                                                   double *__restrict__ rhsz, double *__restrict__ rhsc,
                                                   double *__restrict__ hx, double *__restrict__ hy,
                                                   double *__restrict__ hz, double *__restrict__ hc,
-                                                  double *__restrict__ s3part, int seg, int lw) {
-  extern __shared__ double xr[];  // X_NROWS rows of lw values
+                                                  double *__restrict__ s3part, int seg, int lw, int grp) {
+  extern __shared__ double xr[];  // X_NROWS x (rows of the workgroup) rows of lw values
   const int nx = g.nx, nz = g.nz;
   const int i = threadIdx.x;
-  // whole row: thread = column; segment: thread t = column blockIdx.x * seg - 3 + t (wrapped)
-  int gcol = seg ? (int)blockIdx.x * seg - 3 + i : i;
+  // tile (segment bx, row group by, z chunk bz): runs of grp consecutive tiles,
+  // segments fastest, go to one XCD, so the halo columns and edge rows that
+  // neighbouring tiles share are read through the same L2
+  const int nwg = (int)(gridDim.x * gridDim.y * gridDim.z);
+  const int wt = xcd_run((int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)), nwg, grp);
+  const int bx = wt % (int)gridDim.x, by = (wt / (int)gridDim.x) % (int)gridDim.y,
+            bz = wt / (int)(gridDim.x * gridDim.y);
+  // whole row: thread = column; segment: thread t = column bx * seg - 3 + t (wrapped)
+  int gcol = seg ? bx * seg - 3 + i : i;
   gcol = gcol < 0 ? gcol + nx : (gcol >= nx ? gcol - nx : gcol);
-  const bool act = seg ? (i >= 3 && i < 3 + seg && (int)blockIdx.x * seg + i - 3 < nx) : i < nx;
-  const bool lds_in = seg ? true : act;  // writes its values to the LDS rows
-  const int ic = seg ? gcol : (act ? i : 0);
-  const int j = blockIdx.y + 1, jp = j + 1;
+  // rows j of the workgroup, this thread's (uniform in a wave: blockDim.x is a multiple of 64)
+  const int nr = (int)blockDim.y, r = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int jrow = by * nr + r + 1;
+  const bool vrow = jrow <= g.ny;  // rows past ny (last workgroup) mirror row ny and store nothing
+  const bool act = (seg ? (i >= 3 && i < 3 + seg && bx * seg + i - 3 < nx) : i < nx) && vrow;
+  const bool lds_in = seg ? true : i < nx;  // writes its values to the LDS rows
+  const int ic = seg ? gcol : (i < nx ? i : 0);
+  const int j = vrow ? jrow : g.ny, jp = j + 1;
   const int nyp = g.ny + 1;
-  const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, nz);
+  // the j-1 and j+1 rows of a row inside the workgroup are its neighbours'
+  // rows: their values come through LDS, and only the workgroup's edge rows
+  // load them from HBM
+  const bool lo_lds = r > 0;
+  const bool hi_lds = r + 1 < nr && jrow + 1 <= g.ny;
+  const int k0 = bz * kc, k1 = min(k0 + kc, nz);
   const long long pl = g.plane;
   const double *M = g.met;
   const int W = g.nyh;
   const double rdyp = M[M_RDYP * W + j], rdyc = M[M_RDYC * W + j];
   const double IY1 = M[M_YINT * W + j], IY2 = 1.0 - IY1;
   const double IY3 = M[M_YINT * W + jp], IY4 = 1.0 - IY3;
-  const double c1 = g.ic1, c2 = g.ic2, c3 = g.ic3, c4 = g.ic4;
+  // the row's y-Laplacian weights, read once: a load inside the loop could
+  // not move across the per-plane barrier and would wait on every load in
+  // flight before it
+  const double ap2c = M[M_AP2C * W + j], ac2c = M[M_AC2C * W + j], am2c = M[M_AM2C * W + j];
+  const double ap2p = M[M_AP2P * W + j], ac2p = M[M_AC2P * W + j], am2p = M[M_AM2P * W + j];
+  const double dyp = M[M_DYP * W + j];
+  const double ap2s = SC ? M[M_AP2CSC * W + j] : 0.0, ac2s = SC ? M[M_AC2CSC * W + j] : 0.0,
+               am2s = SC ? M[M_AM2CSC * W + j] : 0.0;
+  // The 4th-order weights (src/m_MeshAndMetries.f90:86-101) are symmetric or
+  // antisymmetric in their stored values too (dxc4 = -dxc1, dxc3 = -dxc2,
+  // dxxc5 = dxxc1, dxxc4 = dxxc2, the same in z), and the interpolation
+  // weights are the exact constants -1/16 and 9/16: only the distinct values
+  // are held, as the loop otherwise runs out of scalar registers.
+  constexpr double c1 = -0.0625, c2 = 0.5625, c3 = 0.5625, c4 = -0.0625;
+  const double ex1 = g.dxc1, ex2 = g.dxc2, ez1 = g.dzc1, ez2 = g.dzc2;
+  const double exx1 = g.dxxc1, exx2 = g.dxxc2, exx3 = g.dxxc3, ezz1 = g.dzzc1, ezz2 = g.dzzc2, ezz3 = g.dzzc3;
   const double ucrf = cf.ucrf;
   const bool top = jp == nyp;  // uy interpolations on row ny+1 are zero (src/m_TScheme.f90:157-160)
   // LDS slots of the x neighbours i-3..i+3: wrapped in a whole row; in a
@@ -168,7 +201,14 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
     else q = q < 0 ? q + nx : (q >= nx ? q - nx : q);
     nb[s] = q;
   }
-  auto X = [&](int row, int d) { return xr[row * lw + nb[d + 3]]; };
+  // LDS row of quantity `row` for workgroup row rr
+  auto LR = [&](int row, int rr) { return (row * nr + rr) * lw; };
+  auto X = [&](int row, int d) { return xr[LR(row, r) + nb[d + 3]]; };
+  // ux(j-1) and uy(j+1) with their x neighbours: the neighbouring rows' ux and
+  // uy rows inside the workgroup, else this row's own copies
+  const int uxm_row = lo_lds ? LR(X_UX, r - 1) : LR(X_UXM, r), uyp_row = hi_lds ? LR(X_UY, r + 1) : LR(X_UYP, r);
+  auto XUXM = [&](int d) { return xr[uxm_row + nb[d + 3]]; };
+  auto XUYP = [&](int d) { return xr[uyp_row + nb[d + 3]]; };
   auto wz = [&](int kk) { return kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk); };
   auto at = [&](int kk) { return (long long)wz(kk) * pl; };
   // byte offsets of this column at rows j, j-1, j+1 inside a plane: loads and
@@ -210,15 +250,15 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
     return c1 * wux[s] + c2 * wux[s + 1] + c3 * wux[s + 2] + c4 * wux[s + 3];
   };
   if (lds_in) {
-    xr[0 * lw + i] = wuz[2];
-    xr[1 * lw + i] = wuz[3];
-    xr[2 * lw + i] = wuz[4];
+    xr[LR(0, r) + i] = wuz[2];
+    xr[LR(1, r) + i] = wuz[3];
+    xr[LR(2, r) + i] = wuz[4];
   }
   __syncthreads();
   {
     double zx[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) zx[r] = c1 * X(r, -2) + c2 * X(r, -1) + c3 * X(r, 0) + c4 * X(r, 1);
+    for (int q = 0; q < 3; ++q) zx[q] = c1 * X(q, -2) + c2 * X(q, -1) + c3 * X(q, 0) + c4 * X(q, 1);
     const double hz0 = uxhz_at(0), hz1 = uxhz_at(1), hz2 = uxhz_at(2);  // planes k0-1, k0, k0+1
     Q[0] = hz0 * zx[0];
     Q[1] = hz1 * zx[1];
@@ -227,24 +267,45 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
     R[1] = zx[2] * (hz2 + ucrf);
   }
   double s3 = 0.0;
+  // the old right-hand sides of a plane are loaded with the previous plane's
+  // window entries, a plane ahead of their use
+  auto ldh = [&](const double *a, int kk) { return bld(plane_rsrc(a + (long long)kk * pl, pbytes), bj); };
+  double hxo = ldh(hx, k0), hyo = ldh(hy, k0), hzo = ldh(hz, k0), hco = SC ? ldh(hc, k0) : 0.0;
   for (int k = k0; k < k1; ++k) {
     const long long kp = (long long)k * pl;
-    const double hxo = bld(plane_rsrc(hx + kp, pbytes), bj), hyo = bld(plane_rsrc(hy + kp, pbytes), bj),
-                 hzo = bld(plane_rsrc(hz + kp, pbytes), bj);
-    const double hco = SC ? bld(plane_rsrc(hc + kp, pbytes), bj) : 0.0;
     __syncthreads();  // the previous plane's reads are done
     if (lds_in) {
-      xr[X_UX * lw + i] = wux[3];
-      xr[X_UY * lw + i] = wuy[3];
-      xr[X_UZ * lw + i] = wuz[3];
-      xr[X_UZ2 * lw + i] = wuz[5];
-      xr[X_P * lw + i] = wp[2];
-      xr[X_UXM * lw + i] = ux_m;
-      xr[X_UYP * lw + i] = wuyp[2];
-      xr[X_R * lw + i] = R[0];
-      if (SC) xr[X_SC * lw + i] = wsc[3];
+      xr[LR(X_UX, r) + i] = wux[3];
+      xr[LR(X_UY, r) + i] = wuy[3];
+      xr[LR(X_UZ, r) + i] = wuz[3];
+      xr[LR(X_UZ2, r) + i] = wuz[5];
+      xr[LR(X_P, r) + i] = wp[2];
+      if (!lo_lds) xr[LR(X_UXM, r) + i] = ux_m;
+      if (!hi_lds) xr[LR(X_UYP, r) + i] = wuyp[2];
+      xr[LR(X_R, r) + i] = R[0];
+      if (SC) xr[LR(X_SC, r) + i] = wsc[3];
+      if (nr > 1) {
+        xr[LR(X_UYF, r) + i] = wuy[5];
+        xr[LR(X_UZF, r) + i] = wuz[6];
+      }
     }
     __syncthreads();
+    // this plane's j-1 / j+1 values of the rows inside the workgroup, and the
+    // next entries of the uy(j+1) and uz(j-1) windows
+    double nwuyp = 0.0, nwuzm = 0.0;
+    if (lo_lds) {
+      ux_m = xr[LR(X_UX, r - 1) + nb[3]];
+      uy_m = xr[LR(X_UY, r - 1) + nb[3]];
+      p_m = xr[LR(X_P, r - 1) + nb[3]];
+      if (SC) sc_m = xr[LR(X_SC, r - 1) + nb[3]];
+      nwuzm = xr[LR(X_UZF, r - 1) + nb[3]];
+    }
+    if (hi_lds) {
+      ux_p = xr[LR(X_UX, r + 1) + nb[3]];
+      uz_p = xr[LR(X_UZ, r + 1) + nb[3]];
+      if (SC) sc_p = xr[LR(X_SC, r + 1) + nb[3]];
+      nwuyp = xr[LR(X_UYF, r + 1) + nb[3]];
+    }
     // the products two planes ahead
     {
       const double zx2 = c1 * X(X_UZ2, -2) + c2 * X(X_UZ2, -1) + c3 * X(X_UZ2, 0) + c4 * X(X_UZ2, 1);
@@ -269,36 +330,36 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       const double a1 = c1 * xu[1] + c2 * xu[2] + c3 * xu[3] + c4 * xu[4];
       const double a2 = c1 * xu[2] + c2 * xu[3] + c3 * xu[4] + c4 * xu[5];
       const double a3 = c1 * xu[3] + c2 * xu[4] + c3 * xu[5] + c4 * xu[6];
-      const double h11 = g.dxc1 * a0 * (a0 + ucrf) + g.dxc2 * a1 * (a1 + ucrf) + g.dxc3 * a2 * (a2 + ucrf) +
-                         g.dxc4 * a3 * (a3 + ucrf);
+      const double h11 = ex1 * a0 * (a0 + ucrf) + ex2 * a1 * (a1 + ucrf) + (-ex2) * a2 * (a2 + ucrf) +
+                         (-ex1) * a3 * (a3 + ucrf);
       const double uyhx_p =
-          top ? 0.0 : c1 * X(X_UYP, -2) + c2 * X(X_UYP, -1) + c3 * wuyp[2] + c4 * X(X_UYP, 1);
+          top ? 0.0 : c1 * XUYP(-2) + c2 * XUYP(-1) + c3 * wuyp[2] + c4 * XUYP(1);
       const double uyhx_c = c1 * xy[1] + c2 * xy[2] + c3 * xy[3] + c4 * xy[4];
       const double h12 = (uyhx_p * uxhyp - uyhx_c * uxhyc) * rdyp;
-      const double h13 = g.dzc1 * Q[0] + g.dzc2 * Q[1] + g.dzc3 * Q[2] + g.dzc4 * Q[3];
+      const double h13 = ez1 * Q[0] + ez2 * Q[1] + (-ez2) * Q[2] + (-ez1) * Q[3];
       const double d11 =
-          g.dxxc1 * xu[1] + g.dxxc2 * xu[2] + g.dxxc3 * xu[3] + g.dxxc4 * xu[4] + g.dxxc5 * xu[5];
-      d22q1 = M[M_AP2C * W + j] * ux_p + M[M_AC2C * W + j] * ux0 + M[M_AM2C * W + j] * ux_m;
+          exx1 * xu[1] + exx2 * xu[2] + exx3 * xu[3] + exx2 * xu[4] + exx1 * xu[5];
+      d22q1 = ap2c * ux_p + ac2c * ux0 + am2c * ux_m;
       const double d33 =
-          g.dzzc1 * wux[1] + g.dzzc2 * wux[2] + g.dzzc3 * wux[3] + g.dzzc4 * wux[4] + g.dzzc5 * wux[5];
+          ezz1 * wux[1] + ezz2 * wux[2] + ezz3 * wux[3] + ezz2 * wux[4] + ezz1 * wux[5];
       dcq13x = d11 + d33;
       cx = -h11 - h12 - h13 + cf.xnu * dcq13x + cf.gx;
-      const double gp = g.dxc1 * X(X_P, -2) + g.dxc2 * X(X_P, -1) + g.dxc3 * wp[2] + g.dxc4 * X(X_P, 1);
+      const double gp = ex1 * X(X_P, -2) + ex2 * X(X_P, -1) + (-ex2) * wp[2] + (-ex1) * X(X_P, 1);
       rx = cf.gamma * cx + cf.theta * hxo - cf.alpha * gp + 2.0 * cf.beta * d22q1;
     }
     if (act) {
-      if (cf.s3_mode == 1) s3 += (dcq13x + d22q1) * M[M_DYP * W + j];
-      else if (cf.s3_mode == 2) s3 += ux0 * M[M_DYP * W + j];
+      if (cf.s3_mode == 1) s3 += (dcq13x + d22q1) * dyp;
+      else if (cf.s3_mode == 2) s3 += ux0 * dyp;
     }
 
     // ---- RhsY (:206-236)
     double ry, cy;
     {
       const double qs = 0.25 * rdyc;
-      const double uxhym = IY1 * X(X_UXM, -1) + IY2 * xu[2] + ucrf;
+      const double uxhym = IY1 * XUXM(-1) + IY2 * xu[2] + ucrf;
       const double uxhyc = IY1 * ux_m + IY2 * xu[3] + ucrf;
-      const double uxhyp = IY1 * X(X_UXM, 1) + IY2 * xu[4] + ucrf;
-      const double uxhyu = IY1 * X(X_UXM, 2) + IY2 * xu[5] + ucrf;
+      const double uxhyp = IY1 * XUXM(1) + IY2 * xu[4] + ucrf;
+      const double uxhyu = IY1 * XUXM(2) + IY2 * xu[5] + ucrf;
       const double uzhym = IY1 * wuzm[0] + IY2 * wuz[2];
       const double uzhyc = IY1 * wuzm[1] + IY2 * wuz[3];
       const double uzhyp = IY1 * wuzm[2] + IY2 * wuz[4];
@@ -307,7 +368,7 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       const double y1 = c1 * xy[1] + c2 * xy[2] + c3 * xy[3] + c4 * xy[4];
       const double y2 = c1 * xy[2] + c2 * xy[3] + c3 * xy[4] + c4 * xy[5];
       const double y3 = c1 * xy[3] + c2 * xy[4] + c3 * xy[5] + c4 * xy[6];
-      const double h21 = g.dxc1 * y0 * uxhym + g.dxc2 * y1 * uxhyc + g.dxc3 * y2 * uxhyp + g.dxc4 * y3 * uxhyu;
+      const double h21 = ex1 * y0 * uxhym + ex2 * y1 * uxhyc + (-ex2) * y2 * uxhyp + (-ex1) * y3 * uxhyu;
       const double s1 = wuyp[2] + uy0;
       const double s2 = uy0 + uy_m;
       const double h22 = (s1 * s1 - s2 * s2) * qs;
@@ -315,12 +376,12 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       const double z1 = c1 * wuy[1] + c2 * wuy[2] + c3 * wuy[3] + c4 * wuy[4];
       const double z2 = c1 * wuy[2] + c2 * wuy[3] + c3 * wuy[4] + c4 * wuy[5];
       const double z3 = c1 * wuy[3] + c2 * wuy[4] + c3 * wuy[5] + c4 * wuy[6];
-      const double h23 = g.dzc1 * z0 * uzhym + g.dzc2 * z1 * uzhyc + g.dzc3 * z2 * uzhyp + g.dzc4 * z3 * uzhyu;
+      const double h23 = ez1 * z0 * uzhym + ez2 * z1 * uzhyc + (-ez2) * z2 * uzhyp + (-ez1) * z3 * uzhyu;
       const double d11 =
-          g.dxxc1 * xy[1] + g.dxxc2 * xy[2] + g.dxxc3 * xy[3] + g.dxxc4 * xy[4] + g.dxxc5 * xy[5];
-      const double d22 = M[M_AP2P * W + j] * wuyp[2] + M[M_AC2P * W + j] * uy0 + M[M_AM2P * W + j] * uy_m;
+          exx1 * xy[1] + exx2 * xy[2] + exx3 * xy[3] + exx2 * xy[4] + exx1 * xy[5];
+      const double d22 = ap2p * wuyp[2] + ac2p * uy0 + am2p * uy_m;
       const double d33 =
-          g.dzzc1 * wuy[1] + g.dzzc2 * wuy[2] + g.dzzc3 * wuy[3] + g.dzzc4 * wuy[4] + g.dzzc5 * wuy[5];
+          ezz1 * wuy[1] + ezz2 * wuy[2] + ezz3 * wuy[3] + ezz2 * wuy[4] + ezz1 * wuy[5];
       cy = -h21 - h22 - h23 + cf.xnu * (d11 + d33) + cf.gy;
       const double gp = (wp[2] - p_m) * rdyc;
       ry = cf.gamma * cy + cf.theta * hyo - cf.alpha * gp + 2.0 * cf.beta * d22;
@@ -331,7 +392,7 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
     {
       const double uzhyc = IY1 * wuzm[1] + IY2 * uz0;
       const double uzhyp = IY3 * uz0 + IY4 * uz_p;
-      const double h31 = g.dxc1 * X(X_R, -1) + g.dxc2 * R[0] + g.dxc3 * X(X_R, 1) + g.dxc4 * X(X_R, 2);
+      const double h31 = ex1 * X(X_R, -1) + ex2 * R[0] + (-ex2) * X(X_R, 1) + (-ex1) * X(X_R, 2);
       const double uyhz_p = top ? 0.0 : c1 * wuyp[0] + c2 * wuyp[1] + c3 * wuyp[2] + c4 * wuyp[3];
       const double uyhz_c = c1 * wuy[1] + c2 * wuy[2] + c3 * wuy[3] + c4 * wuy[4];
       const double h32 = (uyhz_p * uzhyp - uyhz_c * uzhyc) * rdyp;
@@ -339,17 +400,18 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       const double b1 = c1 * wuz[1] + c2 * wuz[2] + c3 * wuz[3] + c4 * wuz[4];
       const double b2 = c1 * wuz[2] + c2 * wuz[3] + c3 * wuz[4] + c4 * wuz[5];
       const double b3 = c1 * wuz[3] + c2 * wuz[4] + c3 * wuz[5] + c4 * wuz[6];
-      const double h33 = g.dzc1 * b0 * b0 + g.dzc2 * b1 * b1 + g.dzc3 * b2 * b2 + g.dzc4 * b3 * b3;
-      const double d11 = g.dxxc1 * X(X_UZ, -2) + g.dxxc2 * X(X_UZ, -1) + g.dxxc3 * uz0 + g.dxxc4 * X(X_UZ, 1) +
-                         g.dxxc5 * X(X_UZ, 2);
-      const double d22 = M[M_AP2C * W + j] * uz_p + M[M_AC2C * W + j] * uz0 + M[M_AM2C * W + j] * wuzm[1];
+      const double h33 = ez1 * b0 * b0 + ez2 * b1 * b1 + (-ez2) * b2 * b2 + (-ez1) * b3 * b3;
+      const double d11 = exx1 * X(X_UZ, -2) + exx2 * X(X_UZ, -1) + exx3 * uz0 + exx2 * X(X_UZ, 1) +
+                         exx1 * X(X_UZ, 2);
+      const double d22 = ap2c * uz_p + ac2c * uz0 + am2c * wuzm[1];
       const double d33 =
-          g.dzzc1 * wuz[1] + g.dzzc2 * wuz[2] + g.dzzc3 * wuz[3] + g.dzzc4 * wuz[4] + g.dzzc5 * wuz[5];
+          ezz1 * wuz[1] + ezz2 * wuz[2] + ezz3 * wuz[3] + ezz2 * wuz[4] + ezz1 * wuz[5];
       cz = -h31 - h32 - h33 + cf.xnu * (d11 + d33) + cf.gz;
-      const double gp = g.dzc1 * wp[0] + g.dzc2 * wp[1] + g.dzc3 * wp[2] + g.dzc4 * wp[3];
+      const double gp = ez1 * wp[0] + ez2 * wp[1] + (-ez2) * wp[2] + (-ez1) * wp[3];
       rz = cf.gamma * cz + cf.theta * hzo - cf.alpha * gp + 2.0 * cf.beta * d22;
     }
 
+    double rc = 0.0, cc = 0.0;
     if (SC) {
       // ---- scalar transport (:302-357)
       double xs[7];
@@ -373,22 +435,20 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       cy = cy + e2;
       cz = cz + e3;
       const double f0 = cf.fvv[0], f1 = cf.fvv[1], f2 = cf.fvv[2];
-      const double hc1 = g.dxc1 * (xu[2] + f0) * sh_m + g.dxc2 * (xu[3] + f0) * x0 + g.dxc3 * (xu[4] + f0) * sh_p +
-                         g.dxc4 * (xu[5] + f0) * sh_u;
+      const double hc1 = ex1 * (xu[2] + f0) * sh_m + ex2 * (xu[3] + f0) * x0 + (-ex2) * (xu[4] + f0) * sh_p +
+                         (-ex1) * (xu[5] + f0) * sh_u;
       const double hc2 = ((wuyp[2] + f1) * shyp - (uy0 + f1) * shyc) * rdyp;
-      const double hc3 = g.dzc1 * (wuz[2] + f2) * sz_m + g.dzc2 * (wuz[3] + f2) * z0 + g.dzc3 * (wuz[4] + f2) * sz_p +
-                         g.dzc4 * (wuz[5] + f2) * sz_u;
+      const double hc3 = ez1 * (wuz[2] + f2) * sz_m + ez2 * (wuz[3] + f2) * z0 + (-ez2) * (wuz[4] + f2) * sz_p +
+                         (-ez1) * (wuz[5] + f2) * sz_u;
       const double d11 =
-          g.dxxc1 * xs[1] + g.dxxc2 * xs[2] + g.dxxc3 * xs[3] + g.dxxc4 * xs[4] + g.dxxc5 * xs[5];
-      const double d22 = M[M_AP2CSC * W + j] * sc_p + M[M_AC2CSC * W + j] * wsc[3] + M[M_AM2CSC * W + j] * sc_m;
+          exx1 * xs[1] + exx2 * xs[2] + exx3 * xs[3] + exx2 * xs[4] + exx1 * xs[5];
+      const double d22 = ap2s * sc_p + ac2s * wsc[3] + am2s * sc_m;
       const double d33 =
-          g.dzzc1 * wsc[1] + g.dzzc2 * wsc[2] + g.dzzc3 * wsc[3] + g.dzzc4 * wsc[4] + g.dzzc5 * wsc[5];
-      const double ccv = -hc1 - hc2 - hc3 + cf.sdiff * (d11 + d33);
-      if (act) {
-        bst(plane_rsrc(rhsc + kp, pbytes), bj, cf.gamma * ccv + cf.theta * hco + 2.0 * cf.beta_sc * d22);
-        bst(plane_rsrc(hc + kp, pbytes), bj, ccv);
-      }
+          ezz1 * wsc[1] + ezz2 * wsc[2] + ezz3 * wsc[3] + ezz2 * wsc[4] + ezz1 * wsc[5];
+      cc = -hc1 - hc2 - hc3 + cf.sdiff * (d11 + d33);
+      rc = cf.gamma * cc + cf.theta * hco + 2.0 * cf.beta_sc * d22;
     }
+
     if (act) {
       bst(plane_rsrc(rhsx + kp, pbytes), bj, rx);
       bst(plane_rsrc(rhsy + kp, pbytes), bj, ry);
@@ -396,6 +456,10 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       bst(plane_rsrc(hx + kp, pbytes), bj, cx);
       bst(plane_rsrc(hy + kp, pbytes), bj, cy);
       bst(plane_rsrc(hz + kp, pbytes), bj, cz);
+      if (SC) {
+        bst(plane_rsrc(rhsc + kp, pbytes), bj, rc);
+        bst(plane_rsrc(hc + kp, pbytes), bj, cc);
+      }
     }
 
     // advance one plane: shift the windows and queues, load the new entries
@@ -421,31 +485,45 @@ __global__ void __launch_bounds__(MAXT) k_rhs_rows(Geo g, RhsCoef cf, int kc, co
       wuz[6] = ldp(uz, k + 4, bj);
       if (SC) wsc[6] = ldp(sc, k + 4, bj);
       wp[3] = ldp(p, k + 2, bj);
-      wuyp[3] = ldp(uy, k + 2, bjp);
-      wuzm[3] = ldp(uz, k + 3, bjm);
-      ux_m = ldp(ux, k + 1, bjm);
-      ux_p = ldp(ux, k + 1, bjp);
-      uy_m = ldp(uy, k + 1, bjm);
-      uz_p = ldp(uz, k + 1, bjp);
-      p_m = ldp(p, k + 1, bjm);
-      if (SC) {
-        sc_m = ldp(sc, k + 1, bjm);
-        sc_p = ldp(sc, k + 1, bjp);
+      // the edge rows load their outside neighbour row; the others take it
+      // from LDS in the next plane
+      if (lo_lds) {
+        wuzm[3] = nwuzm;
+      } else {
+        wuzm[3] = ldp(uz, k + 3, bjm);
+        ux_m = ldp(ux, k + 1, bjm);
+        uy_m = ldp(uy, k + 1, bjm);
+        p_m = ldp(p, k + 1, bjm);
+        if (SC) sc_m = ldp(sc, k + 1, bjm);
       }
+      if (hi_lds) {
+        wuyp[3] = nwuyp;
+      } else {
+        wuyp[3] = ldp(uy, k + 2, bjp);
+        ux_p = ldp(ux, k + 1, bjp);
+        uz_p = ldp(uz, k + 1, bjp);
+        if (SC) sc_p = ldp(sc, k + 1, bjp);
+      }
+      const int kn = k + 1 < k1 ? k + 1 : k;  // the last plane reloads its own
+      hxo = ldh(hx, kn);
+      hyo = ldh(hy, kn);
+      hzo = ldh(hz, kn);
+      if (SC) hco = ldh(hc, kn);
     }
   }
   if (cf.s3_mode) {
     // fixed-order pairwise tree over the workgroup's threads
+    const int t = r * (int)blockDim.x + i;
     __syncthreads();
-    xr[i] = s3;
+    xr[t] = s3;
     __syncthreads();
-    for (int n = blockDim.x; n > 1;) {
+    for (int n = (int)(blockDim.x * blockDim.y); n > 1;) {
       const int h = (n + 1) / 2;
-      if (i < n - h) xr[i] += xr[i + h];
+      if (t < n - h) xr[t] += xr[t + h];
       __syncthreads();
       n = h;
     }
-    if (i == 0) s3part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = xr[0];
+    if (t == 0) s3part[wt] = xr[0];
   }
 }
 
@@ -814,29 +892,52 @@ static int pt_nblocks(const ptc_ctx *c) {
   return (int)(gr.x * gr.y * gr.z);
 }
 
-// row-marching RHS: one workgroup per (row segment, j, k chunk).  Rows of at
-// most 512 points are one segment (a 1024-thread workgroup would spill);
-// longer rows go in segments of kRhsSeg columns with a 3-column halo either
-// side (kRhsSegThreads threads: at ~200 VGPRs two such workgroups share a CU
-// and overlap each other's per-plane barriers).  The k chunks are at least 16
-// planes long and many enough for about 48 wavefronts per CU over the launch.
+// row-marching RHS: one workgroup per (row segment, group of rows j, k
+// chunk).  Rows of at most 512 points are one segment (a 1024-thread
+// workgroup would spill); longer rows go in segments of tx - 6 columns with a
+// 3-column halo either side.  A workgroup of ty rows takes the j-1 and j+1
+// rows of its inner rows from LDS, so only its edge rows read a neighbour row
+// from HBM: at 4608x140x700 (SC) 128-thread segments of two rows ran 16.4-16.7
+// ms against 17.2 ms for one row of 256 threads, and four rows of 64 threads
+// 16.3 ms (the halo columns cost what the edge rows save); 512-thread
+// workgroups (two rows of 256) fit one per CU and ran 17.3 ms.  The k chunks
+// are at least 16 planes long and many enough for about 48 wavefronts per CU
+// over the launch.
 constexpr int kRhsRowsMaxNx = 512;
-constexpr int kRhsSegThreads = 256;
-constexpr int kRhsSeg = kRhsSegThreads - 6;
-static int rhs_seg(const ptc_ctx *c) { return c->nx <= kRhsRowsMaxNx ? 0 : kRhsSeg; }
-// z chunks: enough that the grid holds about 48 waves per CU (at ~200 VGPRs
-// two workgroups are resident per CU, so a grid of only a few rounds of them
-// leaves a long partial last round: at 4608x140x700 one 700-plane chunk per
-// row ran 54.1 ms per substep, two 350-plane chunks 52.8 ms), at least 16
-// planes per chunk (each chunk re-reads a 6-plane window prologue)
+constexpr int kRhsSegThreads = 128;
+constexpr int kRhsXcdRun = 32;
+struct RhsShape {
+  int tx, ty, seg, run;
+};
+static RhsShape rhs_shape(const ptc_ctx *c) {
+  RhsShape s{};
+  if (c->nx <= kRhsRowsMaxNx) {
+    s.tx = (c->nx + 63) / 64 * 64;
+    s.seg = 0;
+  } else {
+    s.tx = c->rhs_seg_req ? c->rhs_seg_req : kRhsSegThreads;
+    s.seg = s.tx - 6;
+  }
+  const int ty = c->rhs_rows_req ? c->rhs_rows_req : (s.tx <= 128 ? 2 : 1);
+  s.ty = std::max(1, std::min({ty, 512 / s.tx, c->ny}));
+  s.run = c->rhs_run_req ? c->rhs_run_req : kRhsXcdRun;
+  return s;
+}
+void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run) {
+  const RhsShape s = rhs_shape(c);
+  *seg_threads = s.tx;
+  *rows = s.ty;
+  *run = s.run;
+}
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
-  const int seg = rhs_seg(c);
-  const int nseg = seg ? (c->nx + seg - 1) / seg : 1;
-  const int wpb = seg ? kRhsSegThreads / 64 : (c->nx + 63) / 64;
-  const int want = (48 * c->n_cu + c->ny * nseg * wpb - 1) / (c->ny * nseg * wpb);
+  const RhsShape sh = rhs_shape(c);
+  const int nseg = sh.seg ? (c->nx + sh.seg - 1) / sh.seg : 1;
+  const int nyb = (c->ny + sh.ty - 1) / sh.ty;
+  const int wpb = sh.tx * sh.ty / 64;
+  const int want = (48 * c->n_cu + nyb * nseg * wpb - 1) / (nyb * nseg * wpb);
   const int nch = std::max(1, std::min(want, c->nz / 16));
   *kc = (c->nz + nch - 1) / nch;
-  return dim3((unsigned)nseg, (unsigned)c->ny, (unsigned)((c->nz + *kc - 1) / *kc));
+  return dim3((unsigned)nseg, (unsigned)nyb, (unsigned)((c->nz + *kc - 1) / *kc));
 }
 static int rhs_nblocks(const ptc_ctx *c) {
   int kc = 0;
@@ -883,13 +984,13 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   PTC_TRY(ensure_red(c, nb));
   int kc = 0;
   const dim3 grid = rhs_rows_grid(c, &kc);
-  const int seg = rhs_seg(c);
-  const int nt = seg ? kRhsSegThreads : (c->nx + 63) / 64 * 64;
-  const int lw = seg ? kRhsSegThreads : c->nx;  // LDS row width
-  const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * lw, nt);
+  const RhsShape sh = rhs_shape(c);
+  const int nt = sh.tx * sh.ty;
+  const int lw = sh.seg ? sh.tx : c->nx;  // LDS row width
+  const size_t lds = sizeof(double) * (size_t)std::max(X_NROWS * sh.ty * lw, nt);
 #define PTC_ROWS(SCAL, MAXT)                                                                                  \
-  hipLaunchKernelGGL((k_rhs_rows<SCAL, MAXT>), grid, dim3(nt), lds, c->stream, c->geo, cf, kc, ux, uy, uz, sc, p, \
-                     rhsx, rhsy, rhsz, rhsc, hx, hy, hz, hc, c->d_red, seg, lw)
+  hipLaunchKernelGGL((k_rhs_rows<SCAL, MAXT>), grid, dim3(sh.tx, sh.ty), lds, c->stream, c->geo, cf, kc, ux, uy, \
+                     uz, sc, p, rhsx, rhsy, rhsz, rhsc, hx, hy, hz, hc, c->d_red, sh.seg, lw, sh.run)
   if (nt <= 256) {
     if (cf.scalar) PTC_ROWS(true, 256);
     else PTC_ROWS(false, 256);

@@ -157,6 +157,21 @@ class Solver:
     def poisson_persist(self, mode: int = 1):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_PERSIST, mode), "ptc_poisson_set")
 
+    def rhs_shape(self, segment_threads: int = 0, rows: int = 0, xcd_run: int = 0):
+        """Workgroup shape of the clcRhsXYZ kernel (0 = automatic)."""
+        for knob, v in ((capi.RHS_SEGMENT_THREADS, segment_threads), (capi.RHS_ROWS, rows),
+                        (capi.RHS_XCD_RUN, xcd_run)):
+            check(self.L.ptc_rhs_set(self.h, knob, v), "ptc_rhs_set")
+
+    def rhs_get_shape(self) -> dict:
+        out = {}
+        for name, knob in (("segment_threads", capi.RHS_SEGMENT_THREADS), ("rows", capi.RHS_ROWS),
+                           ("xcd_run", capi.RHS_XCD_RUN)):
+            v = C.c_int()
+            check(self.L.ptc_rhs_get(self.h, knob, C.byref(v)), "ptc_rhs_get")
+            out[name] = v.value
+        return out
+
     def poisson_get_knob(self, knob: int) -> int:
         v = C.c_int()
         check(self.L.ptc_poisson_get_knob(self.h, knob, C.byref(v)), "ptc_poisson_get_knob")

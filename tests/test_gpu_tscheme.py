@@ -78,7 +78,33 @@ def test_clc_rhs_xyz_row_lengths(kind, grid):
     rhs_against_oracle(kind, 2, grid)
 
 
-def rhs_against_oracle(kind, ns, grid=None):
+# workgroup shapes of the row kernel: (segment threads, rows, tiles per XCD
+# run); rows that do not divide ny leave part-filled last workgroups, and the
+# 1024-point rows go in segments of 58, 122 or 250 columns
+@pytest.mark.parametrize("kind", ["channel", "gravity"])
+@pytest.mark.parametrize("grid", [(1024, 10, 20), (96, 10, 12), (200, 14, 16)], ids=lambda g: "x".join(map(str, g)))
+@pytest.mark.parametrize("shape", [(64, 4, 8), (128, 3, 1), (256, 2, 64), (128, 1, 32), (64, 8, 16)],
+                         ids=lambda t: "s%d-r%d-g%d" % t)
+def test_clc_rhs_xyz_workgroup_shapes(kind, grid, shape):
+    rhs_against_oracle(kind, 2, grid, shape)
+
+
+def test_rhs_shape_knobs():
+    case = small_channel(1024, 10, 20)
+    with Solver(case) as s:
+        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32}
+        s.rhs_shape(256, 3, 1)
+        # at most 512 threads per workgroup
+        assert s.rhs_get_shape() == {"segment_threads": 256, "rows": 2, "xcd_run": 1}
+        for knob, bad in ((F.RHS_SEGMENT_THREADS, 96), (F.RHS_ROWS, 9), (F.RHS_XCD_RUN, -1), (7, 0)):
+            assert s.L.ptc_rhs_set(s.h, knob, bad) == F.PTC_E_ARG
+    with Solver(small_channel(96, 10, 12)) as s:
+        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32}
+    with Solver(small_channel(400, 8, 8)) as s:
+        assert s.rhs_get_shape()["rows"] == 1
+
+
+def rhs_against_oracle(kind, ns, grid=None, shape=None):
     if grid is None:
         case = small_channel() if kind == "channel" else small_gravity()
     else:
@@ -94,6 +120,8 @@ def rhs_against_oracle(kind, ns, grid=None):
     hc = hist[3].copy(order="F") if case.scalar else None
     orc.clcRhsXYZ(ux, uy, uz, sc, rx, ry, rz, rc, hx, hy, hz, hc, p)
     with Solver(case) as s:
+        if shape is not None:
+            s.rhs_shape(*shape)
         upload_state(s, ux, uy, uz, p, sc)
         for fid, h in zip((F.F_HISTX, F.F_HISTY, F.F_HISTZ, F.F_HISTC), hist):
             s.upload(fid, h, F.L_INTERIOR)
