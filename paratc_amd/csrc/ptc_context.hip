@@ -354,6 +354,7 @@ int create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ct
       hipMalloc(&c->d_wnx, sizeof(double) * c->nx) != hipSuccess ||
       hipMalloc(&c->d_wnz, sizeof(double) * c->nz) != hipSuccess ||
       hipMalloc(&c->d_scal, sizeof(double) * 64) != hipSuccess ||
+      hipMalloc(&c->d_cnfac, sizeof(double) * kCnMaxJobs * kCnFacStride) != hipSuccess ||
       hipHostMalloc(&c->h_pinned, sizeof(double) * 64) != hipSuccess) {
     set_error("device allocation failed in ptc_create");
     return fail(PTC_E_ALLOC);
@@ -384,6 +385,7 @@ void destroy(ptc_ctx *c) {
   if (c->d_stat) (void)hipFree(c->d_stat);
   if (c->d_visu) (void)hipFree(c->d_visu);
   if (c->d_scal) (void)hipFree(c->d_scal);
+  if (c->d_cnfac) (void)hipFree(c->d_cnfac);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

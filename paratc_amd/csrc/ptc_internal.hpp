@@ -159,6 +159,7 @@ struct ptc_ctx {
   double *d_red = nullptr;     // reduction partials
   int red_cap = 0;
   double *d_scal = nullptr;    // device scalars
+  double *d_cnfac = nullptr;   // Crank-Nicolson factor tables, kCnMaxJobs x kCnFacStride
   double *h_pinned = nullptr;  // pinned host scalars
   hipStream_t stream = nullptr;
   int device = 0;
@@ -317,6 +318,9 @@ struct CnJob {
   double *uym_upd, *duym_upd;
 };
 constexpr int kCnMaxJobs = 4;
+// doubles of one job's Crank-Nicolson factor table (chunk rows P * MM <= 256,
+// 2P <= 32 reduced rows, at most 5 cyclic-reduction steps)
+constexpr int kCnFacStride = 2048;
 struct CnJobs {
   CnJob j[kCnMaxJobs];
   int n;

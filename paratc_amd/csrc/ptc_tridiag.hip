@@ -499,14 +499,120 @@ __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict_
 
 // ---- Crank-Nicolson y-solves (u += du), partitioned -----------------------
 // kind: 0 = u/w (ForCN tables), 1 = v (row 1 Dirichlet), 2 = scalar (ScCN tables)
+//
+// The matrix I - beta L_y of a job is the same for every line (i, k).  Its
+// chunk elimination factors, the reduced rows and the coefficients of every
+// cyclic-reduction step are formed once per launch by k_cn_factor (one
+// workgroup per job, the same arithmetic chunk_reduce and pcr_row apply to
+// the coefficients), and the solve reads them as wave-uniform scalars: it
+// keeps only the right-hand side and the old u in registers and does no
+// division.  Table of one job (doubles):
+//   [0, PM)        rr_q  reciprocal pivot of chunk row q = s + r (r >= 1)
+//   [PM, 2PM)      a_q   sub-diagonal of row q (r >= 2)
+//   [2PM, 3PM)     C_q   upper factor (r >= 1)
+//   [3PM, 4PM)     A_q   first-row factor (r >= 1)
+//   [4PM, 4PM+P)   r0_p  reciprocal pivot of chunk p's reduced first row
+//   [4PM+P, +2P)   c_p   super-diagonal of chunk p's first row
+//   then (a, c, inv, 0) of reduced row q at cyclic-reduction step l at
+//   4PM + 2P + 4 (l 2P + q)
+// with PM = P * MM.
+struct CnFacIdx {
+  int pm, P;
+  __device__ __forceinline__ int rr(int q) const { return q; }
+  __device__ __forceinline__ int a(int q) const { return pm + q; }
+  __device__ __forceinline__ int C(int q) const { return 2 * pm + q; }
+  __device__ __forceinline__ int A(int q) const { return 3 * pm + q; }
+  __device__ __forceinline__ int r0(int p) const { return 4 * pm + p; }
+  __device__ __forceinline__ int c0(int p) const { return 4 * pm + P + p; }
+  __device__ __forceinline__ int pcr(int l, int q) const { return 4 * pm + 2 * P + 4 * (l * 2 * P + q); }
+};
+
+template <int MM>
+__global__ void __launch_bounds__(64) k_cn_factor(CnJobs jobs, const double *__restrict__ met, int nyh, int ny,
+                                                  int P, double *__restrict__ fac) {
+  __shared__ double sa[2][32], sc[2][32];
+  const CnJob &J = jobs.j[blockIdx.x];
+  double *T = fac + (long long)blockIdx.x * kCnFacStride;
+  const CnFacIdx X{P * MM, P};
+  const int t = threadIdx.x, n2 = 2 * P;
+  if (t < P) {
+    // chunk_reduce on the coefficients alone
+    CnRows getc{met + J.rA * nyh, met + J.rB * nyh, met + J.rC * nyh, J.beta, ny, J.kind};
+    const int s = t * MM;
+    double A[MM], C[MM];
+    double aj, bj, cj;
+    getc(s + 2, aj, bj, cj);
+    double rr = 1.0 / bj;
+    C[1] = cj * rr;
+    A[1] = -aj * rr;
+    T[X.rr(s + 1)] = rr;
+    T[X.a(s + 1)] = aj;
+#pragma unroll
+    for (int r = 2; r < MM; ++r) {
+      getc(s + r + 1, aj, bj, cj);
+      rr = 1.0 / (bj - aj * C[r - 1]);
+      C[r] = cj * rr;
+      A[r] = -rr * aj * A[r - 1];
+      T[X.rr(s + r)] = rr;
+      T[X.a(s + r)] = aj;
+    }
+    const double aE = -A[MM - 1], cE = C[MM - 1];
+    double al = A[MM - 2], ga = -C[MM - 2];
+#pragma unroll
+    for (int r = MM - 3; r >= 1; --r) {
+      al = A[r] - C[r] * al;
+      ga = -C[r] * ga;
+    }
+    getc(s + 1, aj, bj, cj);
+    const double r0 = 1.0 / (bj + cj * al);
+    T[X.r0(t)] = r0;
+    T[X.c0(t)] = cj;
+#pragma unroll
+    for (int r = 1; r < MM; ++r) {
+      T[X.C(s + r)] = C[r];
+      T[X.A(s + r)] = A[r];
+    }
+    sa[0][2 * t] = r0 * aj;
+    sc[0][2 * t] = r0 * cj * ga;
+    sa[0][2 * t + 1] = aE;
+    sc[0][2 * t + 1] = cE;
+  }
+  // pcr_row on the coefficients: each step's (a, c, inv) of every reduced row
+  int in = 0, l = 0;
+  for (int d = 1; d < n2; d *= 2, ++l) {
+    __syncthreads();
+    if (t < n2) {
+      const double a = sa[in][t], c = sc[in][t];
+      double am = 0.0, cm = 0.0, ap = 0.0, cp = 0.0;
+      if (t - d >= 0) {
+        am = sa[in][t - d];
+        cm = sc[in][t - d];
+      }
+      if (t + d < n2) {
+        ap = sa[in][t + d];
+        cp = sc[in][t + d];
+      }
+      const double inv = 1.0 / (1.0 - a * cm - c * ap);
+      double *e = T + X.pcr(l, t);
+      e[0] = a;
+      e[1] = c;
+      e[2] = inv;
+      e[3] = 0.0;
+      sa[1 - in][t] = -inv * a * am;
+      sc[1 - in][t] = -inv * c * cp;
+    }
+    in = 1 - in;
+  }
+}
+
 template <int MM, int LPB>
-__global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs jobs, const double *__restrict__ met,
-                                                                      int nyh, int nx, int ny, int P) {
-  extern __shared__ double smem[];
-  double *RA = smem;
-  double *RC = RA + 4 * P * LPB;
-  double *RF = RC + 4 * P * LPB;
+__global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_fac(CnJobs jobs, const double *__restrict__ fac,
+                                                                     int nx, int ny, int P) {
+  extern __shared__ double smem[];  // f of the 2P reduced rows x LPB lines, double-buffered
+  double *RF = smem;
   const CnJob &J = jobs.j[blockIdx.z];
+  const double *__restrict__ T = fac + (long long)blockIdx.z * kCnFacStride;
+  const CnFacIdx X{P * MM, P};
   double *__restrict__ u = J.u;
   const double *__restrict__ rhs = J.rhs;
   const int kind = J.kind;
@@ -517,41 +623,78 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
   const int s = p * MM;
   const long long col = (long long)(valid ? i : 0) + (long long)nx * ((long long)(ny + 2) * k);
   const double sh = J.shift ? J.shift[0] : 0.0;
-  // unconditional loads (padding rows read row ny), masked by a multiply
-  double f[MM];
+  // the right-hand side and the old u, all rows in flight at once
+  // (unconditional loads: padding rows read row ny, masked by a multiply)
+  double f[MM], uo[MM];
 #pragma unroll
   for (int r = 0; r < MM; ++r) {
-    const double v = rhs[col + (long long)nx * (min(s + r, ny - 1) + 1)];
+    const long long o = col + (long long)nx * (min(s + r, ny - 1) + 1);
+    const double v = rhs[o];
+    uo[r] = u[o];
     f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   }
   // du at the wall row: duy_ym, or with Update_uy_ym folded in (src/m_FlowCase.f90:112-122)
-  // the new uy_ym (zero) minus the old one, both planes written by this thread
-  double wall_new = 0.0;
-  if (kind == 1 && s == 0 && valid) {
+  // the new uy_ym (zero) minus the old one, both planes written after the solve
+  const double wall_new = 0.0;
+  double wall_old = 0.0;
+  const bool wall_thread = kind == 1 && s == 0 && valid;
+  if (wall_thread) {
     const long long q = i + (long long)nx * k;
     if (J.uym_upd) {
-      const double old = J.uym_upd[q];
-      J.uym_upd[q] = 0.0;
-      J.duym_upd[q] = wall_new - old;
-      f[0] = wall_new - old;
+      wall_old = J.uym_upd[q];
+      f[0] = wall_new - wall_old;
     } else {
       f[0] = J.duy_ym[q];
     }
   }
   if (kind == 1 && s == 0 && !valid) f[0] = 0.0;
-  CnRows getc{met + J.rA * nyh, met + J.rB * nyh, met + J.rC * nyh, J.beta, ny, kind};
-  double A[MM], C[MM];
-  double aS, cS, aE, cE, fS, fE, xs, xe;
-  chunk_reduce<MM>(f, A, C, s, getc, aS, cS, fS, aE, cE, fE);
-  solve_reduced<LPB>(aS, cS, fS, aE, cE, fE, P, p, lane, RA, RC, RF, xs, xe);
-  chunk_backsub<MM>(f, A, C, xs, xe);
+  // chunk elimination (chunk_reduce with the factors read from the table)
+  f[1] = T[X.rr(s + 1)] * f[1];
+#pragma unroll
+  for (int r = 2; r < MM; ++r) f[r] = T[X.rr(s + r)] * (f[r] - T[X.a(s + r)] * f[r - 1]);
+  const double fE = f[MM - 1];
+  double g = f[MM - 2];
+#pragma unroll
+  for (int r = MM - 3; r >= 1; --r) g = f[r] - T[X.C(s + r)] * g;
+  const double fS = T[X.r0(p)] * (f[0] - T[X.c0(p)] * g);
+  // reduced rows by cyclic reduction (pcr_row with the step coefficients read)
+  const int n2 = 2 * P, q0 = 2 * p, q1 = 2 * p + 1, half = n2 * LPB;
+  RF[q0 * LPB + lane] = fS;
+  RF[q1 * LPB + lane] = fE;
+  int in = 0, l = 0;
+  for (int d = 1; d < n2; d *= 2, ++l) {
+    __syncthreads();
+    const int out = half - in;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = h ? q1 : q0;
+      const int o = q * LPB + lane;
+      const double *e = T + X.pcr(l, q);
+      const double fc = RF[in + o];
+      const double fm = q - d >= 0 ? RF[in + o - d * LPB] : 0.0;
+      const double fp = q + d < n2 ? RF[in + o + d * LPB] : 0.0;
+      RF[out + o] = e[2] * ((fc - e[0] * fm) - e[1] * fp);
+    }
+    in = out;
+  }
+  const double xs = RF[in + q0 * LPB + lane], xe = RF[in + q1 * LPB + lane];
+  // back substitution (chunk_backsub)
+  f[MM - 1] = xe;
+#pragma unroll
+  for (int r = MM - 2; r >= 1; --r) f[r] = (f[r] - (-T[X.A(s + r)]) * xs) - T[X.C(s + r)] * f[r + 1];
+  f[0] = xs;
+  if (wall_thread && J.uym_upd) {
+    const long long q = i + (long long)nx * k;
+    J.uym_upd[q] = wall_new;
+    J.duym_upd[q] = wall_new - wall_old;
+  }
   if (valid) {
     const bool wall = kind == 1 && (J.wall != nullptr || J.uym_upd != nullptr);
 #pragma unroll
     for (int r = 0; r < MM; ++r)
       if (s + r < ny) {
         const long long o = col + (long long)nx * (s + r + 1);
-        u[o] = (wall && s + r == 0) ? (J.uym_upd ? wall_new : J.wall[i + (long long)nx * k]) : u[o] + f[r];
+        u[o] = (wall && s + r == 0) ? (J.uym_upd ? wall_new : J.wall[i + (long long)nx * k]) : uo[r] + f[r];
       }
     if (wall && s <= ny - 1 && ny - 1 < s + MM) u[col + (long long)nx * (ny + 1)] = J.top;
   }
@@ -562,11 +705,10 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_part(CnJobs j
 // Launch shape of the partitioned solver: chunk length MM (compile-time), lines
 // per block LPB and chunk count P = ceil(ny / MM).  The first candidate with
 // at most 1/8 padded rows whose block (LPB * P threads) and LDS fit wins.
-// Under the 1024-thread bound the Crank-Nicolson kernels of 8- and 12-row
-// chunks spill registers (24 and 91 VGPRs); spill-free 32-lane shapes with
-// tighter bounds measured slower (4608x140x700 substep 60.0 against 53.4 ms,
-// 128x96x128 0.179 against 0.166 ms): the 512-byte rows of 64 lanes matter
-// more, and the spills stay in cache.
+// The factored Crank-Nicolson solve of 12-row chunks and 64 lanes holds 79
+// VGPRs, so two of its 768-thread workgroups share a CU: one loads while the
+// other eliminates (4608x140x700: 8.65 ms for the four solves against 12.8 ms
+// for the form that factored every line itself, one workgroup per CU).
 struct TriShape {
   int mm, lpb, cap;  // cap: the instantiation's launch bound
 };
@@ -768,17 +910,21 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
   return J;
 }
 
-// one launch for all the jobs (grid z = job)
+// one launch for all the jobs (grid z = job), after the factor tables
 int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
   PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
   int nP = 1, mmax = 16, lpb = 64;
   PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
+  PTC_CHECK(nP * mmax <= 256 && 2 * nP <= 32, PTC_E_STATE, "Crank-Nicolson chunk plan outside the factor table");
   dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz, (unsigned)jobs.n);
-  const size_t lds = (size_t)4 * nP * lpb * 24;
+  const size_t lds = sizeof(double) * 2 * (size_t)(2 * nP) * lpb;
 #define PTC_CN(MM, LPB)                                                                                      \
   if (mmax == MM && lpb == LPB) {                                                                            \
-    hipLaunchKernelGGL((k_tri_cn_part<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_met,       \
-                       c->ny + 2, c->nx, c->ny, nP);                                                         \
+    hipLaunchKernelGGL((k_cn_factor<MM>), dim3(jobs.n), dim3(64), 0, c->stream, jobs, c->d_met, c->ny + 2,  \
+                       c->ny, nP, c->d_cnfac);                                                               \
+    PTC_HIP(hipGetLastError());                                                                              \
+    hipLaunchKernelGGL((k_tri_cn_fac<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_cnfac,      \
+                       c->nx, c->ny, nP);                                                                    \
     PTC_HIP(hipGetLastError());                                                                              \
     return PTC_OK;                                                                                           \
   }
