@@ -401,14 +401,25 @@ constexpr int f2_xblock() {
 template <int N, int G, bool STREAM, typename S>
 __device__ __forceinline__ void x_untangle_store(int g, const double2 *ls, const double2 *__restrict__ wx,
                                                  S *__restrict__ out, int pitch) {
-  for (int kx = g; kx < pitch; kx += G) {
+  // pitch <= N + 32: a thread stores at most NK values.  Their twiddles are
+  // all loaded before the first is used, so the thread waits once on them
+  // (a load per iteration waited on every load in flight, the next line's
+  // prefetch included, once per value).
+  constexpr int NK = (N + 32 + G - 1) / G;
+  double2 w[NK];
+#pragma unroll
+  for (int t = 0; t < NK; ++t) w[t] = wx[min(g + t * G, N)];
+#pragma unroll
+  for (int t = 0; t < NK; ++t) {
+    const int kx = g + t * G;
+    if (kx >= pitch) break;
     double2 X = make_double2(0.0, 0.0);
     if (kx <= N) {
       const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
       double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
       zc.y = -zc.y;
       const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
-      const double2 wd = f2::cmul(wx[kx], d);
+      const double2 wd = f2::cmul(w[t], d);
       X = make_double2(0.5 * (s.x + wd.y), 0.5 * (s.y - wd.x));
     }
     f2::st2x<STREAM>(out[kx], X);
@@ -518,8 +529,17 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
 // device scalar *shift (the mean the correction removes).  ghost: the rows
 // j = 1 and j = ny also go to the Neumann ghost rows j = 0 and j = ny+1
 // (SetBC_and_UpdateHalo_pr, src/m_BC_and_Halo.f90:115-120).
+// 2304-point lines: 288 threads and 41 KB of LDS, three workgroups (15 waves)
+// per CU if a wave holds at most 128 VGPRs (four per SIMD); left alone the
+// compiler takes 129 and two workgroups fit
+template <int N>
+constexpr int f2_xinv_min_waves() {
+  return N == 2304 ? 4 : 1;
+}
+
 template <int N, typename S, bool ACC, bool STREAM>
-__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
+__global__ void __launch_bounds__(f2_xblock<N>(), f2_xinv_min_waves<N>())
+    k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
                                                     double scale, const double *__restrict__ shift, int ghost) {
