@@ -198,9 +198,10 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *                           one (x: nx = 1024, 4608, 9216; z: nz = 512, 700,
  *                           1400) run as persistent workgroups that load
  *                           their next line or column tile while they
- *                           transform the current one; 0 or -1: one
+ *                           transform the current one; 2: the z passes
+ *                           only; 3: the x passes only; 0 or -1: one
  *                           workgroup per line or tile.  The search of
- *                           InitPoissonSolver tries it. */
+ *                           InitPoissonSolver tries them. */
 enum {
   PTC_POISSON_CHUNK_ROWS = 1,
   PTC_POISSON_SLAB_PLANES = 2,

@@ -127,7 +127,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     }
   };
   for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0, 0, 0, 0});
-  for (int zf : {4, 8, 16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0, 0});
+  for (int zf : {2, 4, 8, 16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0, 0});
   const int tile = pick.tile, zf = pick.zf;
   for (int mm : {4, 8, 12, 16}) {
     for (int lanes : {64, 32, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, lanes, 0, 0});
@@ -142,9 +142,11 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     // the persistent prefetching x/z passes with the plan picked so far, and
     // with each compiled z tile
     const Cand b2 = pick;
-    consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, b2.zf, b2.lanes, b2.cl, 1});
-    for (int zf : {4, 8, 16})
-      if (zf != b2.zf) consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, zf, b2.lanes, b2.cl, 1});
+    for (int ps : {1, 2, 3}) {
+      consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, b2.zf, b2.lanes, b2.cl, ps});
+      for (int zf : {2, 4, 8, 16})
+        if (zf != b2.zf && ps != 3) consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, zf, b2.lanes, b2.cl, ps});
+    }
   }
   poisson_free_plan(c);
   apply(pick);
@@ -490,7 +492,7 @@ int ptc_poisson_set(ptc_ctx *ctx, int knob, int value) {
   } else if (knob == PTC_POISSON_STREAMING) {
     PTC_CHECK(value >= -1 && value <= 1, PTC_E_ARG, "streaming must be -1, 0 or 1");
   } else if (knob == PTC_POISSON_PERSIST) {
-    PTC_CHECK(value >= -1 && value <= 1, PTC_E_ARG, "persistent passes must be -1, 0 or 1");
+    PTC_CHECK(value >= -1 && value <= 3, PTC_E_ARG, "persistent passes must be -1, 0, 1, 2 or 3");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown Poisson plan knob");
   }

@@ -394,6 +394,18 @@ constexpr int f2_xblock() {
   return f2_threads<N>() > 256 ? f2_threads<N>() : 256;
 }
 
+// LDS layout of an x-pass line: one padding value after every 8 (bank
+// spreading), after every 16 for the 4608-point lines, so that such a line
+// (78 KB) leaves room for a second workgroup on the CU's 160 KB
+template <int N>
+__device__ __forceinline__ int xpad(int m) {
+  return N == 4608 ? m + (m >> 4) : f2::pad8(m);
+}
+template <int N>
+constexpr int xpadded() {
+  return N == 4608 ? N + N / 16 + 1 : f2::padded(N);
+}
+
 // The half spectrum of a real row of 2N points from the N-point transform Z
 // of its packed pairs (in ls): X_k = (S_k - i w^k D_k) / 2 with
 // S_k = Z_k + conj Z_{N-k}, D_k = Z_k - conj Z_{N-k}, w = exp(-2 pi i / 2N);
@@ -415,8 +427,8 @@ __device__ __forceinline__ void x_untangle_store(int g, const double2 *ls, const
     if (kx >= pitch) break;
     double2 X = make_double2(0.0, 0.0);
     if (kx <= N) {
-      const double2 zk = ls[f2::pad8(kx == N ? 0 : kx)];
-      double2 zc = ls[f2::pad8(kx == 0 ? 0 : N - kx)];
+      const double2 zk = ls[xpad<N>(kx == N ? 0 : kx)];
+      double2 zc = ls[xpad<N>(kx == 0 ? 0 : N - kx)];
       zc.y = -zc.y;
       const double2 s = f2::cadd(zk, zc), d = f2::csub(zk, zc);
       const double2 wd = f2::cmul(w[t], d);
@@ -428,15 +440,26 @@ __device__ __forceinline__ void x_untangle_store(int g, const double2 *ls, const
 
 // The inverse: element m of the N-point inverse transform's input from the
 // half spectrum X (in ls): X_m + conj X_{N-m} + i conj(w^m) (X_m - conj X_{N-m}).
-__device__ __forceinline__ double2 x_retangle(const double2 *ls, const double2 *__restrict__ wx, int m, int n) {
-  const double2 xk = ls[f2::pad8(m)];
-  double2 xc = ls[f2::pad8(n - m)];
+template <int N>
+__device__ __forceinline__ double2 x_retangle(const double2 *ls, const double2 *__restrict__ wx, int m) {
+  const double2 xk = ls[xpad<N>(m)];
+  double2 xc = ls[xpad<N>(N - m)];
   xc.y = -xc.y;
   const double2 s = f2::cadd(xk, xc), d = f2::csub(xk, xc);
   double2 w = wx[m];
   w.y = -w.y;
   const double2 wd = f2::cmul(w, d);
   return make_double2(s.x - wd.y, s.y + wd.x);
+}
+
+// Waves per SIMD the one-shot x kernels are held to.  2304-point lines: 288
+// threads and 41 KB of LDS, three workgroups (15 waves) per CU at four waves
+// per SIMD (128 VGPRs; left alone the inverse takes 129 and two fit).
+// 4608-point lines: 576 threads and 78 KB, two workgroups (18 waves) per CU
+// at five (102 VGPRs).
+template <int N>
+constexpr int f2_x_min_waves() {
+  return N == 2304 ? 4 : (N == 4608 ? 5 : 1);
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
@@ -446,13 +469,13 @@ __device__ __forceinline__ double2 x_retangle(const double2 *ls, const double2 *
 // order y difference, in k_prsrc's operand order.  Each workgroup then writes
 // the maximum |divergence| over its rows to dv.part[blockIdx.x].
 template <int N, typename S, bool STREAM, bool DIV>
-__global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
+__global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
                                                     DivSrc dv, Geo geo) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  constexpr int PN = f2::padded(N);
+  constexpr int PN = xpadded<N>();
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
   // lines (j, k) of the planes j0 .. j0+nj-1 (0-based), all k.  Plain rows go
@@ -503,8 +526,8 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
     if constexpr (DIV) return load_div(m);
     else return f2::ld2x<STREAM>(row[m]);
   };
-  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
-  auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
+  auto lds = [&](int m) -> double2 & { return ls[xpad<N>(m)]; };
+  auto store_lds = [&](int m, double2 v) { ls[xpad<N>(m)] = v; };
   f2::run_fft<N, G, false>(g, tw, load0, lds, store_lds);
   __syncthreads();
   if (live) x_untangle_store<N, G, STREAM>(g, ls, wx, spec + sq * pitch, pitch);
@@ -529,23 +552,16 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2_fft_x_fwd(const double *__r
 // device scalar *shift (the mean the correction removes).  ghost: the rows
 // j = 1 and j = ny also go to the Neumann ghost rows j = 0 and j = ny+1
 // (SetBC_and_UpdateHalo_pr, src/m_BC_and_Halo.f90:115-120).
-// 2304-point lines: 288 threads and 41 KB of LDS, three workgroups (15 waves)
-// per CU if a wave holds at most 128 VGPRs (four per SIMD); left alone the
-// compiler takes 129 and two workgroups fit
-template <int N>
-constexpr int f2_xinv_min_waves() {
-  return N == 2304 ? 4 : 1;
-}
 
 template <int N, typename S, bool ACC, bool STREAM>
-__global__ void __launch_bounds__(f2_xblock<N>(), f2_xinv_min_waves<N>())
+__global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>())
     k2_fft_x_inv(const S *__restrict__ spec, double *__restrict__ dst,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
                                                     double scale, const double *__restrict__ shift, int ghost) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  constexpr int PN = f2::padded(N);
+  constexpr int PN = xpadded<N>();
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
   const long long nlines = (long long)nj * nz;
@@ -565,10 +581,10 @@ __global__ void __launch_bounds__(f2_xblock<N>(), f2_xinv_min_waves<N>())
 #pragma unroll
   for (int t = 0; t < NT; ++t) pre[t] = f2::ld2x<STREAM>(in[min(g + t * G, N)]);
 #pragma unroll
-  for (int t = 0; t < NT; ++t) ls[f2::pad8(min(g + t * G, N))] = pre[t];
+  for (int t = 0; t < NT; ++t) ls[xpad<N>(min(g + t * G, N))] = pre[t];
   __syncthreads();
-  auto load0 = [&](int m) { return x_retangle(ls, wx, m, N); };
-  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  auto load0 = [&](int m) { return x_retangle<N>(ls, wx, m); };
+  auto lds = [&](int m) -> double2 & { return ls[xpad<N>(m)]; };
   double2 *row = reinterpret_cast<double2 *>(dst + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   // the ghost row this line also fills (or none)
   double2 *grow = nullptr;
@@ -628,7 +644,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_fwd(const double *__
                                                      int ny, int nz, int j0, int nj, int pitch, long long rowlen) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  constexpr int PN = f2::padded(N);
+  constexpr int PN = xpadded<N>();
   constexpr int R0 = f2::make_plan(N).r[0];
   constexpr int NB0 = N / R0;
   constexpr int BPT0 = f2::stage_bpt<N, 0, G>();
@@ -659,8 +675,8 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_fwd(const double *__
       }
     }
   };
-  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
-  auto store_lds = [&](int m, double2 v) { ls[f2::pad8(m)] = v; };
+  auto lds = [&](int m) -> double2 & { return ls[xpad<N>(m)]; };
+  auto store_lds = [&](int m, double2 v) { ls[xpad<N>(m)] = v; };
   f2::StageRegs<N, 0, G> v;
   long long grp = blockIdx.x;
   if (grp >= ngroups) return;
@@ -701,7 +717,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_inv(const S *__restr
                                                      double scale, const double *__restrict__ shift, int ghost) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
-  constexpr int PN = f2::padded(N);
+  constexpr int PN = xpadded<N>();
   constexpr int NT = N / G + 1;
   __shared__ double2 sh[L * PN];
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
@@ -725,7 +741,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_inv(const S *__restr
 #pragma unroll
     for (int t = 0; t < NT; ++t) v[t] = f2::ld2x<STREAM>(in[min(g + t * G, N)]);
   };
-  auto lds = [&](int m) -> double2 & { return ls[f2::pad8(m)]; };
+  auto lds = [&](int m) -> double2 & { return ls[xpad<N>(m)]; };
   double2 v[NT];
   long long grp = blockIdx.x;
   if (grp >= ngroups) return;
@@ -736,7 +752,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_inv(const S *__restr
     double2 W[7][4];
     f2::load_twiddles<N, G, true>(gi, tw, W);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) ls[f2::pad8(min(gi + t * G, N))] = v[t];
+    for (int t = 0; t < NT; ++t) ls[xpad<N>(min(gi + t * G, N))] = v[t];
     // the registers are free: the next group's loads go into them
     __builtin_amdgcn_sched_barrier(0);
     const long long gn = grp + gridDim.x < ngroups ? grp + gridDim.x : grp;
@@ -749,7 +765,7 @@ __global__ void __launch_bounds__(f2_xblock<N>()) k2p_fft_x_inv(const S *__restr
     double2 *grow = nullptr;
     if (ghost && jl == 0) grow = reinterpret_cast<double2 *>(dst + rowlen * ((long long)(ny + 2) * k));
     if (ghost && jl == ny - 1) grow = reinterpret_cast<double2 *>(dst + rowlen * (ny + 1 + (long long)(ny + 2) * k));
-    auto load0 = [&](int m) { return x_retangle(ls, wx, m, N); };
+    auto load0 = [&](int m) { return x_retangle<N>(ls, wx, m); };
     auto store_out = [&](int m, double2 v) {
       if (!live) return;
       double2 x;

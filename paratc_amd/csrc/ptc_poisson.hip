@@ -120,8 +120,9 @@ int poisson_build_plan(ptc_ctx *c) {
   // the device's last-level cache, and likewise on the way back
   P.slab = (P.x_fast && P.z_fast && P.slab_req > 0 && P.slab_req < c->ny) ? P.slab_req : 0;
   // persistent prefetching x and z passes where built for these lengths
-  P.persist_x = P.persist_req > 0 && P.x_fast && fft2_has_px(P.h);
-  P.persist_z = P.persist_req > 0 && P.z_fast && fft2_has_pz(c->nz, P.ztile);
+  // (1 both, 2 z only, 3 x only)
+  P.persist_x = (P.persist_req == 1 || P.persist_req == 3) && P.x_fast && fft2_has_px(P.h);
+  P.persist_z = (P.persist_req == 1 || P.persist_req == 2) && P.z_fast && fft2_has_pz(c->nz, P.ztile);
 
   if (P.prec == PTC_PREC_FP32_REFINE) {
     PTC_CHECK(P.x_fast && P.z_fast, PTC_E_UNSUPPORTED,
