@@ -369,10 +369,26 @@ def bench_gravity_substep(ev, grid, gamma=1.6, steps=5, fp32=True):
         mon = s.ChannelIterate(1)
         fields = [s.download(f, capi.L_INTERIOR) for f in (capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_SCALAR)]
         plan = s.poisson_get_config()
+        rhs_ms = None
+        if prec == "fp64":
+            # the dominant kernel of the substep, the row-marching RHS, on its
+            # own (this case has no flow-rate forcing, so clcRhsXYZ is that
+            # one launch): 5 launches after one untimed
+            s.PMcoeUpdate(1, 1e-3)
+            rhs = lambda: s.clcRhsXYZ(capi.F_UX, capi.F_UY, capi.F_UZ, capi.F_SCALAR, capi.F_ARR1, capi.F_ARR2,
+                                      capi.F_HALO, capi.F_ARRC, capi.F_HISTX, capi.F_HISTY, capi.F_HISTZ,
+                                      capi.F_HISTC, capi.F_PR)
+            rhs()
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, stream)
+            for _ in range(5):
+                rhs()
+            ev.record(e1, stream)
+            rhs_ms = ev.elapsed_ms(e0, e1) / 5
         s.close()
-        return ms, mon, fields, plan
+        return ms, mon, fields, plan, rhs_ms
 
-    ms, mon, f64, plan = run("fp64")
+    ms, mon, f64, plan, rhs_ms = run("fp64")
     progress(f"{nx}x{ny}x{nz} gravity current: {ms:.3f} ms per RK substep")
     out = {"workload": f"gravity current {nx}x{ny}x{nz}, RK3, dt 1e-3, FP64, scalar (seeded in HBM)",
            "ms_per_substep": round(ms, 3), "mpts_per_s": round(case.npts / (ms * 1e-3) / 1e6, 1),
@@ -381,8 +397,16 @@ def bench_gravity_substep(ev, grid, gamma=1.6, steps=5, fp32=True):
            # scalar ghosts 72 + 32
            "hbm_gbs_alg": round(432.0 * case.npts / (ms * 1e-3) / 1e9, 1),
            "divmax_after": float(mon[1]), "plan": plan}
+    # k_rhs_rows moves 136 B per point: ux uy uz p c and the four old
+    # right-hand sides read, the four right-hand sides and four new ones written
+    rhs_bytes = 136 * case.npts
+    out["rhs_roofline"] = {"kernel": "k_rhs_rows<true, ...>", "launch_ms": round(rhs_ms, 4),
+                           "bytes_per_launch": rhs_bytes,
+                           "achieved": round(rhs_bytes / (rhs_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(rhs_bytes / (rhs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    progress(f"{nx}x{ny}x{nz} RHS rows alone: {rhs_ms:.3f} ms, frac {out['rhs_roofline']['frac']}")
     if fp32:
-        ms32, mon32, f32, _ = run("fp32")
+        ms32, mon32, f32, _, _ = run("fp32")
         errs = {name: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
                 for name, a, b in zip(("ux", "uy", "uz", "scalar"), f32, f64)}
         progress(f"{nx}x{ny}x{nz} gravity current, FP32 Poisson + 1 sweep: {ms32:.3f} ms per substep, "
