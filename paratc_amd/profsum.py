@@ -5,6 +5,10 @@
     python -m paratc_amd.profsum lds <lds.csv>
         per-kernel mean SQ_LDS_BANK_CONFLICT and SQ_ACTIVE_INST_LDS from one
         --pmc run of both counters, and their ratio
+    python -m paratc_amd.profsum sq <sq.csv>
+        per-kernel shares of wave time from one --pmc run of SQ_WAVE_CYCLES,
+        SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_WAIT_INST_LDS, SQ_ACTIVE_INST_VALU and
+        SQ_ACTIVE_INST_LDS (with SQ_WAVES and SQ_BUSY_CYCLES)
     python -m paratc_amd.profsum traffic <fetch.csv> <write.csv> <copy_bytes>
         per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
         --pmc runs, scaled to HBM bytes by what the same counters read for
@@ -87,6 +91,27 @@ def lds_table(path: str) -> str:
     return "\n".join(out)
 
 
+SQ_SHARES = ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS")
+
+
+def sq_table(path: str, min_wave_cycles: float = 1e9) -> str:
+    """Each counter as a share of SQ_WAVE_CYCLES (the summed lifetime of the
+    kernel's waves), per kernel with at least min_wave_cycles of wave time."""
+    means = {c: _counter_means(path, c) for c in ("SQ_WAVE_CYCLES", "SQ_WAVES") + SQ_SHARES}
+    wc = means["SQ_WAVE_CYCLES"]
+    out = ["| kernel @ grid (threads) | dispatches | waves | wave cycles | " +
+           " | ".join(c[3:].lower() for c in SQ_SHARES) + " |",
+           "|---|---:|---:|---:|" + "---:|" * len(SQ_SHARES)]
+    for k in sorted(wc, key=lambda k: -wc[k][1]):
+        n, cyc = wc[k]
+        if cyc < min_wave_cycles:
+            continue
+        cells = [f"{means[c][k][1] / cyc:.2f}" if k in means[c] else "-" for c in SQ_SHARES]
+        waves = means["SQ_WAVES"][k][1] if k in means["SQ_WAVES"] else 0
+        out.append(f"| {k} | {n} | {waves:.0f} | {cyc:.3g} | " + " | ".join(cells) + " |")
+    return "\n".join(out)
+
+
 def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
     fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
     copy = [k for k in fetch if k.startswith("k_copy @") or k.startswith("k_copy_nt @")]
@@ -127,6 +152,8 @@ def main(argv: List[str]) -> None:
         print(trace_table(argv[1]))
     elif len(argv) >= 2 and argv[0] == "lds":
         print(lds_table(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "sq":
+        print(sq_table(argv[1]))
     elif len(argv) >= 4 and argv[0] == "traffic":
         print(traffic_table(argv[1], argv[2], float(argv[3])))
     else:
