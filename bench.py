@@ -205,14 +205,21 @@ def bench_fp32_refine(r, ev, steps=20, max_sweeps=4):
     ev.record(e1, stream)
     ms = ev.elapsed_ms(e0, e1) / steps
     res = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
+    plan32 = s.poisson_get_config()
+    # the specialised x pass of an unslabbed plan forms the residual as it
+    # loads its lines
+    fused = plan32["x_fast"] and not plan32["slab_planes"]
     # back to the FP64 plan
     s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED if plan64["tri"] == "partitioned" else capi.TRI_THOMAS,
                      plan64["z_tile"] if not (plan64["x_fast"] and plan64["z_fast"]) else 0, 0)
     return {"sweeps": chosen[0], "field_error_vs_fp64": chosen[1], "ms_per_solve": round(ms, 4),
             "speed_vs_fp64": round(r["ms_per"] / ms, 3), "residual_max": res,
             # first solve 48 B/pt (FP64 in, FP32 spectra, FP64 out); a sweep:
-            # residual 24 + correction 56 (its x-inverse adds into phi)
-            "bytes_per_point": 48 + 80 * chosen[0]}
+            # the correction with its residual formed in the x pass 64 (f and
+            # phi read there, the x-inverse adds into phi), or a separate
+            # residual 24 + correction 56
+            "residual_in_x_pass": bool(fused),
+            "bytes_per_point": 48 + (64 if fused else 80) * chosen[0]}
 
 
 def bench_spectra(r, ev, samples=5):

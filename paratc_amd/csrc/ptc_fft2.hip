@@ -463,16 +463,21 @@ constexpr int f2_x_min_waves() {
 }
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
-// DIV: the rows are not read from src but formed from the velocities, as
-// clcPrSrc does (src/m_TScheme.f90:48-77): sudtal times the divergence of
+// SRC = kXDiv: the rows are not read from src but formed from the velocities,
+// as clcPrSrc does (src/m_TScheme.f90:48-77): sudtal times the divergence of
 // (ux, uy, uz) with the 4th-order staggered x and z differences and the 2nd-
 // order y difference, in k_prsrc's operand order.  Each workgroup then writes
 // the maximum |divergence| over its rows to dv.part[blockIdx.x].
-template <int N, typename S, bool STREAM, bool DIV>
+// SRC = kXRes: the rows are the refinement's FP64 residual f - L phi, in
+// k_residual's operand order; each workgroup writes the sum of phi over its
+// lines of row j = 1 to rs.psum[blockIdx.x].
+constexpr int kXPlain = 0, kXDiv = 1, kXRes = 2;
+template <int N, typename S, bool STREAM, int SRC>
 __global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
-                                                    DivSrc dv, Geo geo) {
+                                                    DivSrc dv, ResSrc rs, Geo geo, int tj, int tk) {
+  constexpr bool DIV = SRC == kXDiv, RES = SRC == kXRes, NBR = DIV || RES;
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   constexpr int PN = xpadded<N>();
@@ -480,17 +485,21 @@ __global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_
   const int l = threadIdx.x / G, g = threadIdx.x - l * G;
   // lines (j, k) of the planes j0 .. j0+nj-1 (0-based), all k.  Plain rows go
   // j fastest.  DIV reads the uz rows of planes k-1 .. k+2 and the uy rows j,
-  // j+1 of each line: its lines go k fastest, runs of consecutive k on one XCD
-  // (xcd_group), so the neighbouring lines that re-read a uz row do so through
-  // the same L2.
+  // j+1 of each line, RES phi's rows of planes k-3 .. k+3 and j-1, j+1: their
+  // lines go in tiles of tj x tk lines (tj | nj, tk | nz; k fastest inside a
+  // tile, the tiles k fastest) and runs of two tiles' workgroups go to one XCD
+  // (xcd_run), so the neighbouring lines that re-read a row, in j or in k, do
+  // so through the same L2.
   const long long nlines = (long long)nj * nz;
-  const long long wb = DIV ? (long long)xcd_group<(L == 1 ? 32 : 4)>((int)blockIdx.x, (int)gridDim.x)
+  const int tt = tj * tk;
+  const long long wb = NBR ? (long long)xcd_run((int)blockIdx.x, (int)gridDim.x, tt >= L ? 2 * tt / L : 1)
                            : (long long)blockIdx.x;
   const long long q = wb * L + l;
   const bool live = l < L && q < nlines;
   const long long qq = live ? q : 0;
-  const long long jl = DIV ? j0 + qq / nz : j0 + qq % nj;
-  const long long k = DIV ? qq % nz : qq / nj;
+  const long long tq = qq / tt, wq = qq - tq * tt, ntk = nz / tk;
+  const long long jl = NBR ? j0 + (tq / ntk) * tj + wq / tk : j0 + qq % nj;
+  const long long k = NBR ? (tq % ntk) * tk + wq % tk : qq / nj;
   const long long sq = jl + (long long)ny * k;  // spectral row S[k][jl]
   const double2 *row = reinterpret_cast<const double2 *>(src + rowlen * (jl + 1 + (long long)(ny + 2) * k));
   double2 *ls = sh + (l < L ? l : 0) * PN;
@@ -521,9 +530,49 @@ __global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_
     dmax = fmax(dmax, fmax(fabs(r0), fabs(r1)));
     return make_double2(dv.sudtal * r0, dv.sudtal * r1);
   };
+  double psum = 0.0;
+  auto load_res = [&](int m) {
+    // the pair of points (2m, 2m+1) of row j = jl + 1, plane k
+    constexpr double st0 = -1460.0 / 576.0, st1 = 783.0 / 576.0, st2 = -54.0 / 576.0, st3 = 1.0 / 576.0;
+    const long long rj = rowlen * (jl + 1 + (long long)(ny + 2) * k);
+    const double2 *pr = reinterpret_cast<const double2 *>(rs.phi + rj);
+    double2 p[5];  // pairs m-2 .. m+2: points 2m-4 .. 2m+5
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      int mm = m + s - 2;
+      mm = mm < 0 ? mm + N : (mm >= N ? mm - N : mm);
+      p[s] = pr[mm];
+    }
+    const long long pl = rowlen * (ny + 2);
+    double2 z[7];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+      long long kk = k - 3 + s;
+      kk = kk < 0 ? kk + nz : (kk >= nz ? kk - nz : kk);
+      z[s] = reinterpret_cast<const double2 *>(rs.phi + rj + (kk - k) * pl)[m];
+    }
+    const int j = (int)jl + 1;
+    const double2 c0 = p[2];
+    const double2 ym = j > 1 ? pr[m - N] : c0, yp = j < ny ? pr[m + N] : c0;
+    const double2 fv = reinterpret_cast<const double2 *>(rs.f + rj)[m];
+    const double am = geo.met[M_AM2PH * geo.nyh + j], ac = geo.met[M_AC2PH * geo.nyh + j],
+                 ap = geo.met[M_AP2PH * geo.nyh + j];
+    const double lx0 = st0 * c0.x + st1 * (p[1].y + c0.y) + st2 * (p[1].x + p[3].x) + st3 * (p[0].y + p[3].y);
+    const double lx1 = st0 * c0.y + st1 * (c0.x + p[3].x) + st2 * (p[1].y + p[3].y) + st3 * (p[1].x + p[4].x);
+    const double lz0 = st0 * c0.x + st1 * (z[2].x + z[4].x) + st2 * (z[1].x + z[5].x) + st3 * (z[0].x + z[6].x);
+    const double lz1 = st0 * c0.y + st1 * (z[2].y + z[4].y) + st2 * (z[1].y + z[5].y) + st3 * (z[0].y + z[6].y);
+    const double lap0 = rs.rdx2 * lx0 + rs.rdz2 * lz0 + (am * ym.x + ac * c0.x + ap * yp.x);
+    const double lap1 = rs.rdx2 * lx1 + rs.rdz2 * lz1 + (am * ym.y + ac * c0.y + ap * yp.y);
+    if (j == 1) {
+      psum += c0.x;
+      psum += c0.y;
+    }
+    return make_double2(fv.x - lap0, fv.y - lap1);
+  };
   auto load0 = [&](int m) {
     if (!live) return make_double2(0.0, 0.0);
     if constexpr (DIV) return load_div(m);
+    else if constexpr (RES) return load_res(m);
     else return f2::ld2x<STREAM>(row[m]);
   };
   auto lds = [&](int m) -> double2 & { return ls[xpad<N>(m)]; };
@@ -544,6 +593,20 @@ __global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_
       n = h;
     }
     if (threadIdx.x == 0) dv.part[blockIdx.x] = red[0];
+  }
+  if constexpr (RES) {
+    // fixed-order tree over the workgroup, in the FFT's LDS once its reads are done
+    double *red = reinterpret_cast<double *>(sh);
+    __syncthreads();
+    red[threadIdx.x] = psum;
+    __syncthreads();
+    for (int n = blockDim.x; n > 1;) {
+      const int h = (n + 1) / 2;
+      if ((int)threadIdx.x < n - h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+      n = h;
+    }
+    if (threadIdx.x == 0) rs.psum[blockIdx.x] = red[0];
   }
 }
 
@@ -900,23 +963,37 @@ static long long x_fwd_blocks(long long nlines) {
   return (nlines + L - 1) / L;
 }
 
-template <int N, typename S, bool STREAM, bool DIV>
-static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj, const DivSrc &dv) {
+template <int N, typename S, bool STREAM, int SRC>
+static void x_fwd_launch(ptc_ctx *c, const double *src, S *spec, int j0, int nj, const DivSrc &dv,
+                         const ResSrc &rs = ResSrc{}) {
   constexpr int G = f2_threads<N>();
   constexpr int L = (256 / G) > 0 ? 256 / G : 1;
   const unsigned nblk = (unsigned)x_fwd_blocks<N>((long long)nj * c->nz);
-  hipLaunchKernelGGL((k2_fft_x_fwd<N, S, STREAM, DIV>), dim3(nblk), dim3(L * G), 0, c->stream, src, spec,
-                     c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, dv, c->geo);
+  // line tiles of the neighbour-reading forms: the largest of 8, 4, 2, 1 rows
+  // dividing nj, of 16, 8, 4, 2, 1 planes dividing nz
+  int tj = 8, tk = 16;
+  while (nj % tj) tj /= 2;
+  while (c->nz % tk) tk /= 2;
+  hipLaunchKernelGGL((k2_fft_x_fwd<N, S, STREAM, SRC>), dim3(nblk), dim3(L * G), 0, c->stream, src, spec,
+                     c->pp.fx.tw, c->pp.d_wx, c->ny, c->nz, j0, nj, c->pp.pitch, (long long)c->nx, dv, rs, c->geo,
+                     tj, tk);
 }
 
 template <int N>
-static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div) {
+static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div,
+                   const ResSrc *res) {
   const bool f32 = !f64 && c->pp.prec == PTC_PREC_FP32_REFINE;
   const DivSrc none{};
   if (div) {
     PTC_CHECK(!f32, PTC_E_STATE, "the divergence-forming x pass needs FP64 spectra");
-    if (c->pp.stream) x_fwd_launch<N, double2, true, true>(c, src, (double2 *)spec, j0, nj, *div);
-    else x_fwd_launch<N, double2, false, true>(c, src, (double2 *)spec, j0, nj, *div);
+    if (c->pp.stream) x_fwd_launch<N, double2, true, kXDiv>(c, src, (double2 *)spec, j0, nj, *div);
+    else x_fwd_launch<N, double2, false, kXDiv>(c, src, (double2 *)spec, j0, nj, *div);
+  } else if (res) {
+    // the one-shot kernel forms the residual, whatever the plan's x form
+    if (f32 && c->pp.stream) x_fwd_launch<N, float2, true, kXRes>(c, src, (float2 *)spec, j0, nj, none, *res);
+    else if (f32) x_fwd_launch<N, float2, false, kXRes>(c, src, (float2 *)spec, j0, nj, none, *res);
+    else if (c->pp.stream) x_fwd_launch<N, double2, true, kXRes>(c, src, (double2 *)spec, j0, nj, none, *res);
+    else x_fwd_launch<N, double2, false, kXRes>(c, src, (double2 *)spec, j0, nj, none, *res);
   } else if (c->pp.persist_x) {
     if constexpr (has_px(N)) {
       if (f32 && c->pp.stream) px_fwd_launch<N, float2, true>(c, src, (float2 *)spec, j0, nj);
@@ -928,13 +1005,13 @@ static int x_fwd_t(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bo
       return PTC_E_UNSUPPORTED;
     }
   } else if (f32 && c->pp.stream) {
-    x_fwd_launch<N, float2, true, false>(c, src, (float2 *)spec, j0, nj, none);
+    x_fwd_launch<N, float2, true, kXPlain>(c, src, (float2 *)spec, j0, nj, none);
   } else if (f32) {
-    x_fwd_launch<N, float2, false, false>(c, src, (float2 *)spec, j0, nj, none);
+    x_fwd_launch<N, float2, false, kXPlain>(c, src, (float2 *)spec, j0, nj, none);
   } else if (c->pp.stream) {
-    x_fwd_launch<N, double2, true, false>(c, src, (double2 *)spec, j0, nj, none);
+    x_fwd_launch<N, double2, true, kXPlain>(c, src, (double2 *)spec, j0, nj, none);
   } else {
-    x_fwd_launch<N, double2, false, false>(c, src, (double2 *)spec, j0, nj, none);
+    x_fwd_launch<N, double2, false, kXPlain>(c, src, (double2 *)spec, j0, nj, none);
   }
   PTC_HIP(hipGetLastError());
   return PTC_OK;
@@ -1047,9 +1124,10 @@ bool fft2_has_x(int h) {
   return false;
 }
 
-int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div) {
+int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64, const DivSrc *div,
+               const ResSrc *res) {
   switch (c->pp.h) {
-#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj, f64, div);
+#define PTC_CASE(N) case N: return x_fwd_t<N>(c, src, spec, j0, nj, f64, div, res);
     PTC_X_SIZES(PTC_CASE)
 #undef PTC_CASE
     default: return PTC_E_UNSUPPORTED;

@@ -290,8 +290,18 @@ bool fft2_has_z(int nz, int req, int *tile);
 // acc: the x-inverse adds its result to dst
 // f64: FP64 spectra whatever the plan's storage precision (the statistics spectra);
 // div: form the rows from the velocities instead of reading src
+// The FP64 residual of the FP32-storage refinement formed in the x forward
+// pass: the pass transforms f - L phi (the operator of k_residual: 4th-order
+// staggered x and z, am2ph/ac2ph/ap2ph in y with the Neumann ghosts) and
+// writes each workgroup's sum of phi over its lines of row j = 1 to
+// psum[workgroup].
+struct ResSrc {
+  const double *f, *phi;
+  double rdx2, rdz2;
+  double *psum;
+};
 int fft2_x_fwd(ptc_ctx *c, const double *src, void *spec, int j0, int nj, bool f64 = false,
-               const DivSrc *div = nullptr);
+               const DivSrc *div = nullptr, const ResSrc *res = nullptr);
 // workgroups of one specialised x forward launch over nj planes (0: no such kernel)
 long long fft2_x_fwd_blocks(const ptc_ctx *c, int nj);
 // persistent prefetching forms built for this x length (nx/2) / z length and tile

@@ -72,6 +72,10 @@ int poisson_free_plan(ptc_ctx *c) {
   return PTC_OK;
 }
 
+// the refinement's residual formed in the x forward pass (fft2_x_fwd with a
+// ResSrc): the specialised x kernels of unslabbed plans
+static bool residual_in_x_pass(const PoissonPlan &P) { return P.x_fast && P.slab == 0; }
+
 int poisson_build_plan(ptc_ctx *c) {
   PoissonPlan &P = c->pp;
   if (P.ready) return PTC_OK;
@@ -190,13 +194,18 @@ int poisson_build_plan(ptc_ctx *c) {
   }
   PTC_HIP(hipMemsetAsync(P.d_spec, 0, spec_bytes, c->stream));
   if (P.prec == PTC_PREC_FP32_REFINE && P.sweeps > 0) {
-    if (hipMalloc(&P.d_res, sizeof(double) * c->n3) != hipSuccess) {
-      P.d_res = nullptr;
-      set_error("residual field allocation failed");
-      return PTC_E_ALLOC;
+    // the specialised x pass forms the residual as it loads its lines; only
+    // the other plans store it as a field
+    if (!residual_in_x_pass(P)) {
+      if (hipMalloc(&P.d_res, sizeof(double) * c->n3) != hipSuccess) {
+        P.d_res = nullptr;
+        set_error("residual field allocation failed");
+        return PTC_E_ALLOC;
+      }
+      PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
     }
-    PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
-    if (hipMalloc(&P.d_psum, sizeof(double) * (size_t)residual_blocks(c)) != hipSuccess) {
+    const long long npart = std::max<long long>(residual_blocks(c), fft2_x_fwd_blocks(c, c->ny));
+    if (hipMalloc(&P.d_psum, sizeof(double) * (size_t)npart) != hipSuccess) {
       P.d_psum = nullptr;
       set_error("residual partial-sum allocation failed");
       return PTC_E_ALLOC;
@@ -225,12 +234,19 @@ int poisson_prepare(ptc_ctx *c) {
 
 // One pass sequence src -> spectrum -> dst (acc: dst += result); ev (may be
 // NULL) receives six events around the five passes.
+int launch_pin_shift(ptc_ctx *c, int nblk);
+
 static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, bool ghosts, hipEvent_t *ev,
-                          const DivSrc *div = nullptr) {
+                          const DivSrc *div = nullptr, const ResSrc *res = nullptr) {
   PoissonPlan &P = c->pp;
   if (ev) PTC_HIP(hipEventRecord(ev[0], c->stream));
   if (div) {
     PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, 0, c->ny, false, div));
+    if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
+    PTC_TRY(launch_fft_z(c, P.d_spec, false));
+  } else if (res) {
+    PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, 0, c->ny, false, nullptr, res));
+    PTC_TRY(launch_pin_shift(c, (int)fft2_x_fwd_blocks(c, c->ny)));
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
     PTC_TRY(launch_fft_z(c, P.d_spec, false));
   } else if (P.slab == 0) {
@@ -288,9 +304,14 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts) {
     // the correction through the same FP32-storage passes; the correction's
     // x-inverse also removes the mean of phi's first row, so the (0,0) mode
     // keeps the reference's pin phi(j=1) = 0 (src/m_Poisson.f90:399-404)
+    const ResSrc res{src, dst, c->geo.rdx * c->geo.rdx, c->geo.rdz * c->geo.rdz, P.d_psum};
     for (int s = 0; s < P.sweeps; ++s) {
-      PTC_TRY(launch_residual(c, src, dst, P.d_res));
-      PTC_TRY(solve_sequence(c, P.d_res, dst, true, ghosts, nullptr));
+      if (residual_in_x_pass(P)) {
+        PTC_TRY(solve_sequence(c, nullptr, dst, true, ghosts, nullptr, nullptr, &res));
+      } else {
+        PTC_TRY(launch_residual(c, src, dst, P.d_res));
+        PTC_TRY(solve_sequence(c, P.d_res, dst, true, ghosts, nullptr));
+      }
     }
   }
   return PTC_OK;
@@ -492,6 +513,15 @@ __global__ void __launch_bounds__(kReduceThreads) k_sum_partials(const double *_
   if (threadIdx.x == 0) out[0] = s * scale;
 }
 
+// the mean of phi's first row from nblk per-workgroup sums in d_psum, into the
+// device slot the correction's x-inverse subtracts
+int launch_pin_shift(ptc_ctx *c, int nblk) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kReduceThreads), 0, c->stream, c->pp.d_psum, nblk,
+                     1.0 / ((double)c->nx * (double)c->nz), c->d_scal + kSlotPinShift);
+  PTC_HIP(hipGetLastError());
+  return PTC_OK;
+}
+
 int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
   int kc = 0;
   const dim3 grid = residual_grid(c, &kc);
@@ -502,10 +532,7 @@ int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
   PTC_HIP(hipGetLastError());
   // the x-z mean of phi's first row: the (0,0) mode's pinned value, which no
   // residual sees (a constant is in the Neumann operator's null space)
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kReduceThreads), 0, c->stream, c->pp.d_psum, nblk,
-                     1.0 / ((double)c->nx * (double)c->nz), c->d_scal + kSlotPinShift);
-  PTC_HIP(hipGetLastError());
-  return PTC_OK;
+  return launch_pin_shift(c, nblk);
 }
 
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax) {
