@@ -206,9 +206,8 @@ def bench_fp32_refine(r, ev, steps=20, max_sweeps=4):
     ms = ev.elapsed_ms(e0, e1) / steps
     res = s.poisson_residual(capi.F_ARR1, capi.F_HALO)
     plan32 = s.poisson_get_config()
-    # the specialised x pass of an unslabbed plan forms the residual as it
-    # loads its lines
-    fused = plan32["x_fast"] and not plan32["slab_planes"]
+    # the specialised x pass forms the residual as it loads its lines
+    fused = plan32["x_fast"]
     # back to the FP64 plan
     s.poisson_config(capi.PREC_FP64, capi.TRI_PARTITIONED if plan64["tri"] == "partitioned" else capi.TRI_THOMAS,
                      plan64["z_tile"] if not (plan64["x_fast"] and plan64["z_fast"]) else 0, 0)

@@ -56,7 +56,6 @@ static size_t ctx_bytes(const ptc_ctx *c) {
   b += sizeof(double) * (size_t)c->red_cap;
   if (c->pp.d_spec) b += c->pp.spec_bytes;
   if (c->pp.d_tri_scratch) b += sizeof(double) * c->pp.spec_elems;
-  if (c->pp.d_res) b += sizeof(double) * c->n3;
   return b;
 }
 

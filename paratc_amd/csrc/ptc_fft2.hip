@@ -461,6 +461,13 @@ template <int N>
 constexpr int f2_x_min_waves() {
   return N == 2304 ? 4 : (N == 4608 ? 5 : 1);
 }
+// The residual-forming form holds twenty more loads per point pair: held to
+// those bounds it spills 96-188 bytes per lane, so it takes the registers it
+// needs (three waves per SIMD at most).
+template <int N, int SRC>
+constexpr int f2_xfwd_min_waves() {
+  return SRC == 2 ? ((N == 2304 || N == 4608) ? 3 : 1) : f2_x_min_waves<N>();
+}
 
 // ---- x forward: nlpb real rows of length 2N -> half spectra ---------------
 // SRC = kXDiv: the rows are not read from src but formed from the velocities,
@@ -473,7 +480,7 @@ constexpr int f2_x_min_waves() {
 // lines of row j = 1 to rs.psum[blockIdx.x].
 constexpr int kXPlain = 0, kXDiv = 1, kXRes = 2;
 template <int N, typename S, bool STREAM, int SRC>
-__global__ void __launch_bounds__(f2_xblock<N>(), f2_x_min_waves<N>()) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
+__global__ void __launch_bounds__(f2_xblock<N>(), (f2_xfwd_min_waves<N, SRC>())) k2_fft_x_fwd(const double *__restrict__ src, S *__restrict__ spec,
                                                     const double2 *__restrict__ tw, const double2 *__restrict__ wx,
                                                     int ny, int nz, int j0, int nj, int pitch, long long rowlen,
                                                     DivSrc dv, ResSrc rs, Geo geo, int tj, int tk) {

@@ -113,7 +113,6 @@ struct PoissonPlan {
   double2 *d_tw = nullptr;    // all twiddle tables
   const double2 *d_wx = nullptr;  // real-FFT post twiddles exp(-2 pi i k / nx), k = 0..h
   void *d_spec = nullptr;     // spectral buffer pitch * ny * nz complex (double2, or float2 for FP32 storage)
-  double *d_res = nullptr;    // FP32-storage form: FP64 residual field of the refinement sweeps
   double *d_psum = nullptr;   // FP32-storage form: per-block sums of phi over the j = 1 plane
   double *d_tri_scratch = nullptr;  // Thomas variant: c' per point (real)
   size_t spec_elems = 0;

@@ -32,7 +32,6 @@ bool fft_config_ok(const FftLen &L, int ncol, int nthr);
 int tri_plan(int ny, size_t lds_limit, bool poisson, int mm_req, int lpb_req, int *P_out, int *mm_out, int *lpb_out);
 int tri_plan_lds(int ny, size_t lds_limit, int ll, int mm_req, int *P_out, int *mm_out);
 int tri_plan_cplx(int ny, size_t lds_limit, int cl, int mm_req, int *P_out, int *mm_out);
-int residual_blocks(const ptc_ctx *c);
 
 static void make_twiddles(int n, std::vector<double2> &out) {
   const double twopi = 6.283185307179586476925286766559005768;
@@ -49,7 +48,6 @@ int poisson_free_plan(ptc_ctx *c) {
   ++c->buf_gen;        // and so do the ensemble's captured steps
   if (P.d_tw) (void)hipFree(P.d_tw);
   if (P.d_spec) (void)hipFree(P.d_spec);
-  if (P.d_res) (void)hipFree(P.d_res);
   if (P.d_psum) (void)hipFree(P.d_psum);
   if (P.d_tri_scratch) (void)hipFree(P.d_tri_scratch);
   for (auto &e : P.prof_ev)
@@ -71,10 +69,6 @@ int poisson_free_plan(ptc_ctx *c) {
   P = keep;
   return PTC_OK;
 }
-
-// the refinement's residual formed in the x forward pass (fft2_x_fwd with a
-// ResSrc): the specialised x kernels of unslabbed plans
-static bool residual_in_x_pass(const PoissonPlan &P) { return P.x_fast && P.slab == 0; }
 
 int poisson_build_plan(ptc_ctx *c) {
   PoissonPlan &P = c->pp;
@@ -194,17 +188,10 @@ int poisson_build_plan(ptc_ctx *c) {
   }
   PTC_HIP(hipMemsetAsync(P.d_spec, 0, spec_bytes, c->stream));
   if (P.prec == PTC_PREC_FP32_REFINE && P.sweeps > 0) {
-    // the specialised x pass forms the residual as it loads its lines; only
-    // the other plans store it as a field
-    if (!residual_in_x_pass(P)) {
-      if (hipMalloc(&P.d_res, sizeof(double) * c->n3) != hipSuccess) {
-        P.d_res = nullptr;
-        set_error("residual field allocation failed");
-        return PTC_E_ALLOC;
-      }
-      PTC_HIP(hipMemsetAsync(P.d_res, 0, sizeof(double) * c->n3, c->stream));
-    }
-    const long long npart = std::max<long long>(residual_blocks(c), fft2_x_fwd_blocks(c, c->ny));
+    // the correction's x pass forms the residual as it loads its lines and
+    // writes per-workgroup sums of phi's first row (a slabbed x pass rounds
+    // up once per slab)
+    const long long npart = fft2_x_fwd_blocks(c, c->ny) + c->ny;
     if (hipMalloc(&P.d_psum, sizeof(double) * (size_t)npart) != hipSuccess) {
       P.d_psum = nullptr;
       set_error("residual partial-sum allocation failed");
@@ -244,7 +231,7 @@ static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, 
     PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, 0, c->ny, false, div));
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
     PTC_TRY(launch_fft_z(c, P.d_spec, false));
-  } else if (res) {
+  } else if (res && P.slab == 0) {
     PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, 0, c->ny, false, nullptr, res));
     PTC_TRY(launch_pin_shift(c, (int)fft2_x_fwd_blocks(c, c->ny)));
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
@@ -254,12 +241,23 @@ static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, 
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
     PTC_TRY(launch_fft_z(c, P.d_spec, false));
   } else {
-    // slabbed: the x and z times are reported together under x_fwd / x_inv
+    // slabbed: the x and z times are reported together under x_fwd / x_inv;
+    // a residual-forming x pass writes each slab's partial sums after the
+    // previous slab's
+    long long npart = 0;
     for (int j0 = 0; j0 < c->ny; j0 += P.slab) {
       const int nj = std::min(P.slab, c->ny - j0);
-      PTC_TRY(fft2_x_fwd(c, src, P.d_spec, j0, nj));
+      if (res) {
+        ResSrc rs = *res;
+        rs.psum += npart;
+        PTC_TRY(fft2_x_fwd(c, nullptr, P.d_spec, j0, nj, false, nullptr, &rs));
+        npart += fft2_x_fwd_blocks(c, nj);
+      } else {
+        PTC_TRY(fft2_x_fwd(c, src, P.d_spec, j0, nj));
+      }
       PTC_TRY(fft2_z(c, P.d_spec, false, j0, nj));
     }
+    if (res) PTC_TRY(launch_pin_shift(c, (int)npart));
     if (ev) PTC_HIP(hipEventRecord(ev[1], c->stream));
   }
   if (ev) PTC_HIP(hipEventRecord(ev[2], c->stream));
@@ -281,8 +279,6 @@ static int solve_sequence(ptc_ctx *c, const double *src, double *dst, bool acc, 
   return PTC_OK;
 }
 
-int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r);
-
 int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts) {
   PTC_TRY(poisson_build_plan(c));
   PoissonPlan &P = c->pp;
@@ -300,19 +296,14 @@ int poisson_solve(ptc_ctx *c, const double *src, double *dst, bool ghosts) {
   }
   PTC_TRY(solve_sequence(c, src, dst, false, ghosts, ev));
   if (P.prec == PTC_PREC_FP32_REFINE) {
-    // defect correction: phi += L^-1 (f - L phi), the residual in FP64 and
-    // the correction through the same FP32-storage passes; the correction's
-    // x-inverse also removes the mean of phi's first row, so the (0,0) mode
-    // keeps the reference's pin phi(j=1) = 0 (src/m_Poisson.f90:399-404)
+    // defect correction: phi += L^-1 (f - L phi), the residual in FP64,
+    // formed by the correction's x forward pass as it loads its lines (the
+    // FP32 form always has the specialised x kernels), and the correction
+    // through the same FP32-storage passes; the correction's x-inverse also
+    // removes the mean of phi's first row, so the (0,0) mode keeps the
+    // reference's pin phi(j=1) = 0 (src/m_Poisson.f90:399-404)
     const ResSrc res{src, dst, c->geo.rdx * c->geo.rdx, c->geo.rdz * c->geo.rdz, P.d_psum};
-    for (int s = 0; s < P.sweeps; ++s) {
-      if (residual_in_x_pass(P)) {
-        PTC_TRY(solve_sequence(c, nullptr, dst, true, ghosts, nullptr, nullptr, &res));
-      } else {
-        PTC_TRY(launch_residual(c, src, dst, P.d_res));
-        PTC_TRY(solve_sequence(c, P.d_res, dst, true, ghosts, nullptr));
-      }
-    }
+    for (int s = 0; s < P.sweeps; ++s) PTC_TRY(solve_sequence(c, nullptr, dst, true, ghosts, nullptr, nullptr, &res));
   }
   return PTC_OK;
 }
@@ -385,15 +376,13 @@ __device__ __forceinline__ void wave_sync() {
 // past the stencil's reach, and f, the two y neighbours and the six x-halo
 // columns (lanes 0..5) come through register queues of that depth.  The x
 // neighbours are the adjacent lanes' centre values, exchanged through a
-// per-wavefront LDS row.  WRITE: store r (field layout).  part (may be NULL):
-// per-block max |r|; psum (may be NULL): per-block sum of phi over j = 1.
+// per-wavefront LDS row.  part: per-block max |r| (the residual check of
+// ptc_poisson_residual; the refinement forms its residual in the x pass).
 constexpr int kResAhead = 4;
 
-template <bool WRITE>
 __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, const double *__restrict__ phi,
                                                  Geo g, double rdx2, double rdz2, int kc,
-                                                 double *__restrict__ rout, double *__restrict__ part,
-                                                 double *__restrict__ psum) {
+                                                 double *__restrict__ part) {
   constexpr int D = kResAhead, NW = 7 + D - 1;
   __shared__ double xrow[4][2][72];  // per wavefront: one plane row, 3 halo columns each side
   __shared__ double sh[256];
@@ -430,7 +419,7 @@ __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, 
     qp[s] = cc[kp + nx];
     hq[s] = halo ? hc[kp] : 0.0;
   }
-  double mx = 0.0, ps = 0.0;
+  double mx = 0.0;
   for (int k = k0; k < k1; ++k) {
     double *R = xrow[wv][k & 1];
     const double c0 = w[3];
@@ -444,11 +433,7 @@ __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, 
     const double yp = jc < ny ? qp[0] : c0;
     const double lap = rdx2 * lx + rdz2 * lz + (am * ym + ac * c0 + ap * yp);
     const double res = fq[0] - lap;
-    if (act) {
-      if (WRITE) rout[col + (long long)nx * jc + (long long)k * pl] = res;
-      mx = fmax(mx, fabs(res));
-      if (jc == 1) ps += c0;
-    }
+    if (act) mx = fmax(mx, fabs(res));
     // advance the window and the queues by one plane; the new entries are
     // the planes k + 1 + 3 + (D - 1) (window) and k + D (queues)
 #pragma unroll
@@ -469,25 +454,13 @@ __global__ void __launch_bounds__(256) k_residual(const double *__restrict__ f, 
   }
   const int t = threadIdx.x + 64 * threadIdx.y;
   const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-  if (part) {
-    sh[t] = mx;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-      if (t < h) sh[t] = fmax(sh[t], sh[t + h]);
-      __syncthreads();
-    }
-    if (t == 0) part[blk] = sh[0];
+  sh[t] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) sh[t] = fmax(sh[t], sh[t + h]);
     __syncthreads();
   }
-  if (psum) {
-    sh[t] = ps;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-      if (t < h) sh[t] += sh[t + h];
-      __syncthreads();
-    }
-    if (t == 0) psum[blk] = sh[0];
-  }
+  if (t == 0) part[blk] = sh[0];
 }
 
 // launch shape of k_residual: (i, j) tiles of 64 x 4, k split into chunks
@@ -498,12 +471,6 @@ static dim3 residual_grid(const ptc_ctx *c, int *kc) {
   const int nch = std::min(c->nz, want);
   *kc = (c->nz + nch - 1) / nch;
   return dim3((unsigned)gx, (unsigned)gy, (unsigned)((c->nz + *kc - 1) / *kc));
-}
-
-int residual_blocks(const ptc_ctx *c) {
-  int kc = 0;
-  const dim3 gr = residual_grid(c, &kc);
-  return (int)(gr.x * gr.y * gr.z);
 }
 
 // out = (sum of the nblk partial sums, in a fixed order) * scale
@@ -522,27 +489,13 @@ int launch_pin_shift(ptc_ctx *c, int nblk) {
   return PTC_OK;
 }
 
-int launch_residual(ptc_ctx *c, const double *f, const double *phi, double *r) {
-  int kc = 0;
-  const dim3 grid = residual_grid(c, &kc);
-  const int nblk = (int)(grid.x * grid.y * grid.z);
-  const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual<true>, grid, dim3(64, 4), 0, c->stream, f, phi, c->geo, rdx2, rdz2, kc, r, nullptr,
-                     c->pp.d_psum);
-  PTC_HIP(hipGetLastError());
-  // the x-z mean of phi's first row: the (0,0) mode's pinned value, which no
-  // residual sees (a constant is in the Neumann operator's null space)
-  return launch_pin_shift(c, nblk);
-}
-
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax) {
   int kc = 0;
   const dim3 grid = residual_grid(c, &kc);
   const int nblk = (int)(grid.x * grid.y * grid.z);
   PTC_TRY(ensure_red(c, nblk));
   const double rdx2 = c->geo.rdx * c->geo.rdx, rdz2 = c->geo.rdz * c->geo.rdz;
-  hipLaunchKernelGGL(k_residual<false>, grid, dim3(64, 4), 0, c->stream, src, dst, c->geo, rdx2, rdz2, kc, nullptr,
-                     c->d_red, nullptr);
+  hipLaunchKernelGGL(k_residual, grid, dim3(64, 4), 0, c->stream, src, dst, c->geo, rdx2, rdz2, kc, c->d_red);
   PTC_HIP(hipGetLastError());
   std::vector<double> h((size_t)nblk);
   PTC_HIP(hipMemcpyAsync(h.data(), c->d_red, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
