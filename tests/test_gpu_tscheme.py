@@ -484,6 +484,36 @@ def test_step_graph_equals_launches():
     assert ca[7] == pytest.approx(cb[7], abs=1e-14)
 
 
+def test_step_graph_recaptured_after_rhs_shape_change():
+    """A change of the RHS kernel's workgroup shape (ptc_rhs_set) changes its
+    launch and its count of flow-rate partials; the captured step is dropped
+    and re-captured, so graph replays around the changes give the fields and
+    monitors of ordinary launches bit for bit."""
+    case = small_channel()
+    outs = []
+    for graphs in (True, False):
+        orc = Oracle(case)
+        st = State(orc)
+        st.ux, st.uy, st.uz = ic.channel_fields(case, seed=7, amp=0.1)
+        orc.SetBC_and_UpdateHalo(st.ux, st.uy, st.uz, st.uy_ym)
+        with Solver(case) as s:
+            upload_state(s, st.ux, st.uy, st.uz, st.p, None)
+            s.poisson_config(F.PREC_FP64, F.TRI_AUTO, 0, 0)
+            s.step_graphs(graphs)
+            mons = [s.ChannelIterate(2)]
+            s.rhs_shape(0, 4, 1)
+            assert s.rhs_get_shape()["rows"] == 4
+            mons.append(s.ChannelIterate(2))
+            s.rhs_shape(0, 1, 0)
+            mons.append(s.ChannelIterate(1))
+            outs.append(([s.download(f) for f in (F.F_UX, F.F_UY, F.F_UZ, F.F_PR)], mons))
+    (fa, ma), (fb, mb) = outs
+    for a, b in zip(fa, fb):
+        assert np.array_equal(a, b)
+    for a, b in zip(ma, mb):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kind", ["channel", "gravity"])
 def test_fused_source_equals_separate_pass(kind):
     """The substep forms clcPrSrc's divergence inside the Poisson x forward
