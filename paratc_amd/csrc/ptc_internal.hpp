@@ -328,8 +328,9 @@ struct CnJob {
 };
 constexpr int kCnMaxJobs = 4;
 // doubles of one job's Crank-Nicolson factor table (chunk rows P * MM <= 256,
-// 2P <= 32 reduced rows, at most 5 cyclic-reduction steps)
-constexpr int kCnFacStride = 2048;
+// 2P <= 64 reduced rows, at most 6 cyclic-reduction steps: 4 * 256 + 64 +
+// 4 * 6 * 64 = 2624)
+constexpr int kCnFacStride = 2688;
 struct CnJobs {
   CnJob j[kCnMaxJobs];
   int n;

@@ -530,7 +530,7 @@ struct CnFacIdx {
 template <int MM>
 __global__ void __launch_bounds__(64) k_cn_factor(CnJobs jobs, const double *__restrict__ met, int nyh, int ny,
                                                   int P, double *__restrict__ fac) {
-  __shared__ double sa[2][32], sc[2][32];
+  __shared__ double sa[2][64], sc[2][64];
   const CnJob &J = jobs.j[blockIdx.x];
   double *T = fac + (long long)blockIdx.x * kCnFacStride;
   const CnFacIdx X{P * MM, P};
@@ -915,7 +915,7 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
   PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
   int nP = 1, mmax = 16, lpb = 64;
   PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
-  PTC_CHECK(nP * mmax <= 256 && 2 * nP <= 32, PTC_E_STATE, "Crank-Nicolson chunk plan outside the factor table");
+  PTC_CHECK(nP * mmax <= 256 && 2 * nP <= 64, PTC_E_STATE, "Crank-Nicolson chunk plan outside the factor table");
   dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz, (unsigned)jobs.n);
   const size_t lds = sizeof(double) * 2 * (size_t)(2 * nP) * lpb;
 #define PTC_CN(MM, LPB)                                                                                      \

@@ -151,6 +151,38 @@ def rhs_against_oracle(kind, ns, grid=None, shape=None):
     assert abs(coe[8] - orc.coe()[8]) <= 1e-12 * max(1.0, abs(orc.coe()[8]))
 
 
+# tall lines: 12-row chunks x 64 lanes (ny 140), 8-row chunks x 32 lanes with
+# 25 and 32 chunks (ny 200, 256: 50 and 64 reduced rows, six reduction steps)
+@pytest.mark.parametrize("ny", [140, 200, 256])
+def test_cn_solves_tall_lines(ny):
+    case = small_channel(16, ny, 8)
+    orc = Oracle(case)
+    ux, uy, uz, p, sc = rand_state(case, orc, seed=22)
+    rng = np.random.default_rng(4)
+    rhs = [np.asfortranarray(rng.standard_normal((case.nx, case.ny, case.nz))) for _ in range(2)]
+    rzh = np.asfortranarray(rng.standard_normal((case.nx + 4, case.ny + 2, case.nz + 4)))
+    dym = np.asfortranarray(0.01 * rng.standard_normal((case.nx, case.nz)))
+    orc.PMcoeUpdate(3, 5e-3)
+    o_ux, o_uy, o_uz = ux.copy(order="F"), uy.copy(order="F"), uz.copy(order="F")
+    orc.clcU1Hat(o_ux, rhs[0])
+    orc.clcU2Hat(o_uy, rhs[1], dym)
+    orc.clcU3Hat(o_uz, rzh)
+    with Solver(case) as s:
+        upload_state(s, ux, uy, uz, p, None)
+        s.upload(F.F_ARR1, rhs[0], F.L_INTERIOR)
+        s.upload(F.F_ARR2, rhs[1], F.L_INTERIOR)
+        s.upload(F.F_HALO, rzh)
+        s.upload(F.F_DUY_YM, dym, F.L_PLANE)
+        s.PMcoeUpdate(3, 5e-3)
+        s.clcU1Hat(F.F_UX, F.F_ARR1)
+        s.clcU2Hat(F.F_UY, F.F_ARR2, F.F_DUY_YM)
+        s.clcU3Hat(F.F_UZ, F.F_HALO)
+        for fid, ref in [(F.F_UX, o_ux), (F.F_UY, o_uy), (F.F_UZ, o_uz)]:
+            e = rel_err(s.download(fid, F.L_INTERIOR), inner(ref))
+            print(f"ny={ny} field {fid}: {e:.3e}", flush=True)
+            assert e <= TOL, (fid, e)
+
+
 @pytest.mark.parametrize("kind", ["channel", "halfchannel", "gravity-2-3", "gravity-1-1", "gravity-3-2"])
 def test_cn_solves(kind):
     if kind == "channel":
