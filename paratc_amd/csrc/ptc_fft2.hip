@@ -33,7 +33,7 @@ __device__ __forceinline__ double2 mul_mi(double2 a) {
   return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
-// ---- compile-time radix plan: 8s first, then 4 or 2, then 3, 5, 7 ----
+// ---- compile-time radix plan: 8s first, then 4 or 2, then 9, 3, 5, 7 ----
 struct PlanC {
   int ns;
   int r[12];
@@ -45,6 +45,7 @@ constexpr PlanC make_plan(int n) {
   while (m % 8 == 0) { p.r[p.ns++] = 8; m /= 8; }
   if (m % 4 == 0) { p.r[p.ns++] = 4; m /= 4; }
   if (m % 2 == 0) { p.r[p.ns++] = 2; m /= 2; }
+  while (m % 9 == 0) { p.r[p.ns++] = 9; m /= 9; }
   while (m % 3 == 0) { p.r[p.ns++] = 3; m /= 3; }
   while (m % 5 == 0) { p.r[p.ns++] = 5; m /= 5; }
   while (m % 7 == 0) { p.r[p.ns++] = 7; m /= 7; }
@@ -221,6 +222,42 @@ template <bool INV>
 struct Dft<5, INV> : DftOdd<5, INV> {};
 template <bool INV>
 struct Dft<7, INV> : DftOdd<7, INV> {};
+// 9 points as 3 x 3: three 3-point DFTs of the stride-3 subsequences, the
+// twiddles w9^(n2 k1), three 3-point DFTs across them; X[k1 + 3 k2]
+template <bool INV>
+struct Dft<9, INV> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    // cos and sin of 2 pi k / 9, k = 1, 2, 4
+    constexpr double c1 = 0.7660444431189780352024, s1 = 0.6427876096865393263226;
+    constexpr double c2 = 0.1736481776669303488517, s2 = 0.9848077530122080593667;
+    constexpr double c4 = -0.9396926207859083840541, s4 = 0.3420201433256687330441;
+    double2 t[3][3];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      t[n2][0] = v[n2];
+      t[n2][1] = v[n2 + 3];
+      t[n2][2] = v[n2 + 6];
+      Dft<3, INV>::run(t[n2]);
+    }
+    // w9^k = (cos, -sin) forward, (cos, sin) inverse
+    auto tw = [](double2 a, double c, double sn) {
+      const double sg = INV ? sn : -sn;
+      return make_double2(a.x * c - a.y * sg, a.x * sg + a.y * c);
+    };
+    t[1][1] = tw(t[1][1], c1, s1);
+    t[1][2] = tw(t[1][2], c2, s2);
+    t[2][1] = tw(t[2][1], c2, s2);
+    t[2][2] = tw(t[2][2], c4, s4);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      double2 u[3] = {t[0][k1], t[1][k1], t[2][k1]};
+      Dft<3, INV>::run(u);
+      v[k1] = u[0];
+      v[k1 + 3] = u[1];
+      v[k1 + 6] = u[2];
+    }
+  }
+};
 
 // ---- one stage for the G threads of a line; Load(m) / Store(m, v) address the
 //      line's elements.  SYNC: a barrier separates this stage's loads from its
@@ -382,10 +419,12 @@ __device__ __forceinline__ void run_fft(int g, const double2 *__restrict__ tw, c
 
 }  // namespace f2
 
-// threads per line: the largest-radix stage has one butterfly per thread
+// threads per line: the largest-radix stage has one butterfly per thread (a
+// radix-9 stage counts as 8: its N/9 butterflies fit the N/8 threads of the
+// radix-8 stages)
 template <int N>
 constexpr int f2_threads() {
-  return N / f2::plan_rmax(N);
+  return N / (f2::plan_rmax(N) > 8 ? 8 : f2::plan_rmax(N));
 }
 
 // x-pass workgroup: 256 threads (several lines) or one line of G > 256 threads
