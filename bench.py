@@ -560,6 +560,19 @@ def bench_ensemble(copy_gbs, steps=500, alone_steps=20, budget_gib=64):
     progress(f"ensemble: one case at a time would take {seq_s:.1f} s for {steps} steps each")
     r = E.serve(wl, steps, budget_gib << 30, solvers, log=progress)
     progress(f"ensemble: served in {r['wall_s']:.1f} s, p50 {r['p50_ms']:.3f} ms, p99 {r['p99_ms']:.3f} ms")
+    # HBM bytes one step of every case moves, from the committed FETCH_SIZE /
+    # WRITE_SIZE totals of this workload (null when absent)
+    from paratc_amd import profsum
+    counted = profsum.read_value(os.path.join(ROOT, "profiles", "ensemble_c3_hbm_traffic.md"),
+                                 "step_bytes_counted")
+    alg_step = sum(E.step_bytes(c) for c in wl.cases)
+    cnt = None
+    if counted:
+        gbs = counted * steps / r["wall_s"] / 1e9
+        cnt = {"step_bytes_counted": counted, "step_bytes_alg": alg_step,
+               "counted_over_alg": round(counted / alg_step, 3), "served_hbm_gbs_counted": round(gbs, 1),
+               "served_frac_of_copy": round(gbs / copy_gbs, 3),
+               "one_at_a_time_hbm_gbs_counted": round(counted * steps / seq_s / 1e9, 1)}
     return {
         "workload": "64 closed-channel cases (seed 3), nx 64-256, ny 48-128, nz 64-192, half with a settling "
                     "scalar, Poisson arrivals 20/s, 500 steps each, dt 2e-3, FP64",
@@ -574,6 +587,7 @@ def bench_ensemble(copy_gbs, steps=500, alone_steps=20, budget_gib=64):
         "one_at_a_time_hbm_gbs_alg": round(seq_bytes / seq_s / 1e9, 1),
         "speedup_ceiling_at_copy_rate": round(copy_gbs / (seq_bytes / seq_s / 1e9), 3),
         "admission_wait_p99_s": round(r["admit_wait_p99_s"], 4),
+        "hbm_counted": cnt,
     }
 
 
@@ -592,6 +606,9 @@ def main():
     ap.add_argument("--large-legs", default="c4a,c4b,c2",
                     help="large legs to run: c4a (9216x140x1400 Poisson), c4b (4608x140x700 substep), "
                          "c2 (1536x192x256 substep)")
+    ap.add_argument("--ensemble-steps", type=int, default=500, help="steps per case of the served ensemble")
+    ap.add_argument("--ensemble-alone-steps", type=int, default=20,
+                    help="timed steps per case when each case runs alone")
     ap.add_argument("--calib-copy", action="store_true",
                     help="launch one plain field copy before the solves (counter calibration)")
     args = ap.parse_args()
@@ -608,7 +625,8 @@ def main():
     rk = None if args.no_rk else bench_rk_substep(ev)
     copy = copy_ceiling(r, ev)
     progress(f"copy kernel: {copy['gbs']} GB/s")
-    ens = None if args.no_ensemble else bench_ensemble(copy["gbs"])
+    ens = None if args.no_ensemble else bench_ensemble(copy["gbs"], steps=args.ensemble_steps,
+                                                       alone_steps=args.ensemble_alone_steps)
     large = None
     if not args.no_large:
         r["solver"].close()  # free the 1024x256x512 fields before the large case

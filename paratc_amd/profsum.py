@@ -9,6 +9,9 @@
         per-kernel shares of wave time from one --pmc run of SQ_WAVE_CYCLES,
         SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_WAIT_INST_LDS, SQ_ACTIVE_INST_VALU and
         SQ_ACTIVE_INST_LDS (with SQ_WAVES and SQ_BUSY_CYCLES)
+    python -m paratc_amd.profsum totals <fetch.csv> <write.csv> <copy_bytes>
+        HBM bytes summed over every dispatch of the run (k_copy excluded),
+        scaled as in the traffic table
     python -m paratc_amd.profsum traffic <fetch.csv> <write.csv> <copy_bytes>
         per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
         --pmc runs, scaled to HBM bytes by what the same counters read for
@@ -130,6 +133,46 @@ def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
     return "\n".join(out)
 
 
+def _counter_sums(path: str) -> Tuple[Dict[str, float], Dict[str, int]]:
+    """Summed counter value and dispatch count per kernel name."""
+    tot: Dict[str, float] = collections.defaultdict(float)
+    cnt: Dict[str, int] = collections.defaultdict(int)
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            k = short_name(r["Kernel_Name"])
+            tot[k] += float(r["Counter_Value"])
+            cnt[k] += 1
+    return tot, cnt
+
+
+def totals(fetch_csv: str, write_csv: str, nbytes: float) -> Tuple[int, float, float]:
+    """(dispatches, read bytes, write bytes) over the whole run, k_copy left
+    out; the copy's own counters give the scale factors."""
+    fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
+    copy = [k for k in fetch if k.startswith("k_copy @") or k.startswith("k_copy_nt @")]
+    if not copy or copy[0] not in write:
+        raise SystemExit("k_copy is not in the counter files: profile bench.py --calib-copy")
+    rf = nbytes / (fetch[copy[0]][1] * 1024.0)
+    wf = nbytes / (write[copy[0]][1] * 1024.0)
+    (fs, fc), (ws, _) = _counter_sums(fetch_csv), _counter_sums(write_csv)
+    keep = [k for k in fs if not k.startswith("k_copy")]
+    n = sum(fc[k] for k in keep)
+    return n, sum(fs[k] for k in keep) * 1024.0 * rf, sum(ws.get(k, 0.0) for k in keep) * 1024.0 * wf
+
+
+def read_value(md_path: str, key: str) -> Optional[float]:
+    """The number after `key` on a line of its own (``key value``) in a
+    summary, or None."""
+    try:
+        for ln in open(md_path):
+            cells = ln.split()
+            if len(cells) == 2 and cells[0] == key:
+                return float(cells[1])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def read_traffic(md_path: str, kernel: str) -> Optional[float]:
     """Traffic bytes per launch of `kernel` (a short-name prefix) from a traffic
     table, or None; with several grid sizes, the largest launch."""
@@ -154,6 +197,12 @@ def main(argv: List[str]) -> None:
         print(lds_table(argv[1]))
     elif len(argv) >= 2 and argv[0] == "sq":
         print(sq_table(argv[1]))
+    elif len(argv) >= 4 and argv[0] == "totals":
+        n, rb, wb = totals(argv[1], argv[2], float(argv[3]))
+        print(f"dispatches {n}")
+        print(f"read_bytes {rb:.0f}")
+        print(f"write_bytes {wb:.0f}")
+        print(f"traffic_bytes {rb + wb:.0f}")
     elif len(argv) >= 4 and argv[0] == "traffic":
         print(traffic_table(argv[1], argv[2], float(argv[3])))
     else:
