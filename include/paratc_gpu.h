@@ -227,12 +227,19 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
  *                            segment is at most 128 threads, else 1)
  *   PTC_RHS_XCD_RUN          consecutive tiles run on one XCD (1 = plain
  *                            order; automatic 32)
+ *   PTC_RHS_MIN_PLANES       z planes per chunk at least (each chunk starts
+ *                            its z window afresh; automatic 16, halved down
+ *                            to 2 on grids that would otherwise give fewer
+ *                            than 1.5 workgroups per CU); the chunks are
+ *                            otherwise many enough for about 48 wavefronts
+ *                            per CU
  * A change drops the captured step graph.  Results do not depend on the
  * shape except through the order of the IsUxConst flow-rate sum. */
 enum {
   PTC_RHS_SEGMENT_THREADS = 1,
   PTC_RHS_ROWS = 2,
-  PTC_RHS_XCD_RUN = 3
+  PTC_RHS_XCD_RUN = 3,
+  PTC_RHS_MIN_PLANES = 4
 };
 int ptc_rhs_set(ptc_ctx *ctx, int knob, int value);
 /* the shape in use for this grid: threads per row segment (the padded row

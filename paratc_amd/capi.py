@@ -32,7 +32,7 @@ TRI_AUTO, TRI_THOMAS, TRI_PARTITIONED = 0, 1, 2
  VISU_LAMBDA2) = range(11)
 (POISSON_CHUNK_ROWS, POISSON_SLAB_PLANES, POISSON_TRI_LINES, POISSON_FAST_ZTILE, POISSON_TRI_LANES,
  POISSON_TRI_COMPLEX, POISSON_STREAMING, POISSON_PERSIST) = 1, 2, 3, 4, 5, 6, 7, 8
-RHS_SEGMENT_THREADS, RHS_ROWS, RHS_XCD_RUN = 1, 2, 3
+RHS_SEGMENT_THREADS, RHS_ROWS, RHS_XCD_RUN, RHS_MIN_PLANES = 1, 2, 3, 4
 S_DIVMAX_PRSRC, S_PRGRADAVER, S_SIMTIME, S_DT = 0, 1, 2, 3
 
 # every symbol declared in include/paratc_gpu.h

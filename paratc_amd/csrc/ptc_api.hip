@@ -544,13 +544,16 @@ int ptc_rhs_set(ptc_ctx *ctx, int knob, int value) {
     PTC_CHECK(value >= 0 && value <= 8, PTC_E_ARG, "RHS rows per workgroup must be in 0..8");
   } else if (knob == PTC_RHS_XCD_RUN) {
     PTC_CHECK(value >= 0 && value <= 256, PTC_E_ARG, "RHS tiles per XCD run must be in 0..256");
+  } else if (knob == PTC_RHS_MIN_PLANES) {
+    PTC_CHECK(value >= 0 && value <= 4096, PTC_E_ARG, "RHS planes per z chunk must be in 0..4096");
   } else {
     PTC_CHECK(false, PTC_E_ARG, "unknown RHS knob");
   }
   PTC_HIP(hipStreamSynchronize(ctx->stream));
   if (knob == PTC_RHS_SEGMENT_THREADS) ctx->rhs_seg_req = value;
   else if (knob == PTC_RHS_ROWS) ctx->rhs_rows_req = value;
-  else ctx->rhs_run_req = value;
+  else if (knob == PTC_RHS_XCD_RUN) ctx->rhs_run_req = value;
+  else ctx->rhs_minpl_req = value;
   // captured steps hold the old launch shape and reduction-partial count
   drop_step_graph(ctx);
   ++ctx->buf_gen;
@@ -560,11 +563,12 @@ int ptc_rhs_set(ptc_ctx *ctx, int knob, int value) {
 int ptc_rhs_get(ptc_ctx *ctx, int knob, int *value) {
   PTC_CTX(ctx);
   PTC_CHECK(value != nullptr, PTC_E_ARG, "null output");
-  int tx = 0, ty = 0, run = 0;
-  rhs_shape_in_use(ctx, &tx, &ty, &run);
+  int tx = 0, ty = 0, run = 0, minpl = 0;
+  rhs_shape_in_use(ctx, &tx, &ty, &run, &minpl);
   if (knob == PTC_RHS_SEGMENT_THREADS) *value = tx;
   else if (knob == PTC_RHS_ROWS) *value = ty;
   else if (knob == PTC_RHS_XCD_RUN) *value = run;
+  else if (knob == PTC_RHS_MIN_PLANES) *value = minpl;
   else PTC_CHECK(false, PTC_E_ARG, "unknown RHS knob");
   return PTC_OK;
 }

@@ -186,7 +186,7 @@ struct ptc_ctx {
   unsigned long long buf_gen = 1;
   unsigned long long step_gen = 0;       // generation of step_graph
   // clcRhsXYZ workgroup shape requests (0 = automatic; ptc_rhs_set)
-  int rhs_seg_req = 0, rhs_rows_req = 0, rhs_run_req = 0;
+  int rhs_seg_req = 0, rhs_rows_req = 0, rhs_run_req = 0, rhs_minpl_req = 0;
 };
 
 namespace ptc {
@@ -273,7 +273,7 @@ void drop_step_graph(ptc_ctx *c);
 int ensure_red(ptc_ctx *c, int n);
 // the clcRhsXYZ workgroup shape in use: threads per row segment (the padded
 // row where a row is one segment), rows per workgroup, tiles per XCD run
-void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run);
+void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run, int *min_planes);
 int poisson_residual(ptc_ctx *c, const double *src, const double *dst, double *rmax);
 int poisson_stage_times(ptc_ctx *c, float *ms5, int *nsolves);
 

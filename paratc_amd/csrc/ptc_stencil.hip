@@ -953,14 +953,34 @@ static int pt_nblocks(const ptc_ctx *c) {
 // ms against 17.2 ms for one row of 256 threads, and four rows of 64 threads
 // 16.3 ms (the halo columns cost what the edge rows save); 512-thread
 // workgroups (two rows of 256) fit one per CU and ran 17.3 ms.  The k chunks
-// are at least 16 planes long and many enough for about 48 wavefronts per CU
-// over the launch.
+// are many enough for about 48 wavefronts per CU over the launch and at least
+// 16 planes long, or shorter where that leaves the chip short of workgroups
+// (z_chunks).
 constexpr int kRhsRowsMaxNx = 512;
 constexpr int kRhsSegThreads = 128;
 constexpr int kRhsXcdRun = 32;
+constexpr int kRhsMinPlanes = 16;
 struct RhsShape {
-  int tx, ty, seg, run;
+  int tx, ty, seg, run, minpl, nseg, nyb, nch;
 };
+
+// Number of z chunks over `tiles` workgroup tiles per plane range: at most
+// `want` (the launch's wavefront target), each at least `minpl` planes long.
+// Each chunk starts its z window afresh (a prologue of several planes), so
+// chunks are long (16 planes) where the grid offers enough tiles; where that
+// leaves fewer than 1.5 workgroups per CU (small grids) the automatic length
+// (shrink) halves, down to 2 planes, until it does not.  RK substep, 64 x 48
+// x 64 channel: 98.5 us with 16-plane chunks, 81.3 with 4; 96 x 64 x 96 with a
+// scalar: 133 against 119 us with 8; 128 x 96 x 128 and larger unchanged.
+static int z_chunks(int nz, long long tiles, int want, int minpl, bool shrink, int n_cu, int *minpl_used) {
+  int nch = std::max(1, std::min(want, nz / minpl));
+  while (shrink && minpl > 2 && 2 * tiles * nch < 3LL * n_cu) {
+    minpl /= 2;
+    nch = std::max(1, std::min(want, nz / minpl));
+  }
+  *minpl_used = minpl;
+  return nch;
+}
 static RhsShape rhs_shape(const ptc_ctx *c) {
   RhsShape s{};
   if (c->nx <= kRhsRowsMaxNx) {
@@ -973,23 +993,26 @@ static RhsShape rhs_shape(const ptc_ctx *c) {
   const int ty = c->rhs_rows_req ? c->rhs_rows_req : (s.tx <= 128 ? 2 : 1);
   s.ty = std::max(1, std::min({ty, 512 / s.tx, c->ny}));
   s.run = c->rhs_run_req ? c->rhs_run_req : kRhsXcdRun;
+  s.nseg = s.seg ? (c->nx + s.seg - 1) / s.seg : 1;
+  s.nyb = (c->ny + s.ty - 1) / s.ty;
+  const int wpb = s.tx * s.ty / 64;
+  const long long tiles = (long long)s.nyb * s.nseg;
+  const int want = (int)((48LL * c->n_cu + tiles * wpb - 1) / (tiles * wpb));
+  s.nch = z_chunks(c->nz, tiles, want, c->rhs_minpl_req ? c->rhs_minpl_req : kRhsMinPlanes, c->rhs_minpl_req == 0,
+                   c->n_cu, &s.minpl);
   return s;
 }
-void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run) {
+void rhs_shape_in_use(const ptc_ctx *c, int *seg_threads, int *rows, int *run, int *min_planes) {
   const RhsShape s = rhs_shape(c);
   *seg_threads = s.tx;
   *rows = s.ty;
   *run = s.run;
+  *min_planes = s.minpl;
 }
 static dim3 rhs_rows_grid(const ptc_ctx *c, int *kc) {
   const RhsShape sh = rhs_shape(c);
-  const int nseg = sh.seg ? (c->nx + sh.seg - 1) / sh.seg : 1;
-  const int nyb = (c->ny + sh.ty - 1) / sh.ty;
-  const int wpb = sh.tx * sh.ty / 64;
-  const int want = (48 * c->n_cu + nyb * nseg * wpb - 1) / (nyb * nseg * wpb);
-  const int nch = std::max(1, std::min(want, c->nz / 16));
-  *kc = (c->nz + nch - 1) / nch;
-  return dim3((unsigned)nseg, (unsigned)nyb, (unsigned)((c->nz + *kc - 1) / *kc));
+  *kc = (c->nz + sh.nch - 1) / sh.nch;
+  return dim3((unsigned)sh.nseg, (unsigned)sh.nyb, (unsigned)((c->nz + *kc - 1) / *kc));
 }
 static int rhs_nblocks(const ptc_ctx *c) {
   int kc = 0;
@@ -1124,12 +1147,13 @@ static BcArgs vel_bc_args(const ptc_ctx *c, int mode) {
 
 // z chunks of the velocity/pressure update: enough for about 48 wavefronts per
 // CU over the launch, at least 16 planes each (each chunk re-reads a 3-plane
-// window prologue)
+// window prologue), shorter on grids with few tiles (z_chunks)
 constexpr int kVelXcdRun = 32;
 static dim3 vel_grid(const ptc_ctx *c, int *kc) {
   const int gx = (c->nx + 63) / 64, gy = (c->ny + 3) / 4;
   const int want = (48 * c->n_cu + gx * gy * 4 - 1) / (gx * gy * 4);
-  const int nch = std::max(1, std::min(want, c->nz / 16));
+  int minpl = 0;
+  const int nch = z_chunks(c->nz, (long long)gx * gy, want, 16, true, c->n_cu, &minpl);
   *kc = (c->nz + nch - 1) / nch;
   return dim3((unsigned)gx, (unsigned)gy, (unsigned)((c->nz + *kc - 1) / *kc));
 }

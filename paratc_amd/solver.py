@@ -157,16 +157,16 @@ class Solver:
     def poisson_persist(self, mode: int = 1):
         check(self.L.ptc_poisson_set(self.h, capi.POISSON_PERSIST, mode), "ptc_poisson_set")
 
-    def rhs_shape(self, segment_threads: int = 0, rows: int = 0, xcd_run: int = 0):
-        """Workgroup shape of the clcRhsXYZ kernel (0 = automatic)."""
+    def rhs_shape(self, segment_threads: int = 0, rows: int = 0, xcd_run: int = 0, min_planes: int = 0):
+        """Workgroup shape and z chunking of the clcRhsXYZ kernel (0 = automatic)."""
         for knob, v in ((capi.RHS_SEGMENT_THREADS, segment_threads), (capi.RHS_ROWS, rows),
-                        (capi.RHS_XCD_RUN, xcd_run)):
+                        (capi.RHS_XCD_RUN, xcd_run), (capi.RHS_MIN_PLANES, min_planes)):
             check(self.L.ptc_rhs_set(self.h, knob, v), "ptc_rhs_set")
 
     def rhs_get_shape(self) -> dict:
         out = {}
         for name, knob in (("segment_threads", capi.RHS_SEGMENT_THREADS), ("rows", capi.RHS_ROWS),
-                           ("xcd_run", capi.RHS_XCD_RUN)):
+                           ("xcd_run", capi.RHS_XCD_RUN), ("min_planes", capi.RHS_MIN_PLANES)):
             v = C.c_int()
             check(self.L.ptc_rhs_get(self.h, knob, C.byref(v)), "ptc_rhs_get")
             out[name] = v.value

@@ -79,12 +79,15 @@ def test_clc_rhs_xyz_row_lengths(kind, grid):
 
 
 # workgroup shapes of the row kernel: (segment threads, rows, tiles per XCD
-# run); rows that do not divide ny leave part-filled last workgroups, and the
-# 1024-point rows go in segments of 58, 122 or 250 columns
+# run, z planes per chunk at least); rows that do not divide ny leave
+# part-filled last workgroups, the 1024-point rows go in segments of 58, 122
+# or 250 columns, and chunks of 1-4 planes start their z window at every
+# plane or wrap it across the periodic end
 @pytest.mark.parametrize("kind", ["channel", "gravity"])
 @pytest.mark.parametrize("grid", [(1024, 10, 20), (96, 10, 12), (200, 14, 16)], ids=lambda g: "x".join(map(str, g)))
-@pytest.mark.parametrize("shape", [(64, 4, 8), (128, 3, 1), (256, 2, 64), (128, 1, 32), (64, 8, 16)],
-                         ids=lambda t: "s%d-r%d-g%d" % t)
+@pytest.mark.parametrize("shape", [(64, 4, 8, 0), (128, 3, 1, 0), (256, 2, 64, 0), (128, 1, 32, 0), (64, 8, 16, 0),
+                                   (128, 2, 32, 1), (64, 4, 8, 4), (0, 1, 0, 3)],
+                         ids=lambda t: "s%d-r%d-g%d-p%d" % t)
 def test_clc_rhs_xyz_workgroup_shapes(kind, grid, shape):
     rhs_against_oracle(kind, 2, grid, shape)
 
@@ -92,14 +95,19 @@ def test_clc_rhs_xyz_workgroup_shapes(kind, grid, shape):
 def test_rhs_shape_knobs():
     case = small_channel(1024, 10, 20)
     with Solver(case) as s:
-        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32}
-        s.rhs_shape(256, 3, 1)
+        # 9 segments x 5 row pairs x 20 planes: the automatic chunks shrink to 2 planes
+        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32, "min_planes": 2}
+        s.rhs_shape(256, 3, 1, 4)
         # at most 512 threads per workgroup
-        assert s.rhs_get_shape() == {"segment_threads": 256, "rows": 2, "xcd_run": 1}
-        for knob, bad in ((F.RHS_SEGMENT_THREADS, 96), (F.RHS_ROWS, 9), (F.RHS_XCD_RUN, -1), (7, 0)):
+        assert s.rhs_get_shape() == {"segment_threads": 256, "rows": 2, "xcd_run": 1, "min_planes": 4}
+        for knob, bad in ((F.RHS_SEGMENT_THREADS, 96), (F.RHS_ROWS, 9), (F.RHS_XCD_RUN, -1),
+                          (F.RHS_MIN_PLANES, -1), (F.RHS_MIN_PLANES, 4097), (7, 0)):
             assert s.L.ptc_rhs_set(s.h, knob, bad) == F.PTC_E_ARG
     with Solver(small_channel(96, 10, 12)) as s:
-        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32}
+        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32, "min_planes": 2}
+    with Solver(small_channel(128, 96, 128)) as s:
+        # 48 row pairs x 8 chunks of 16 planes: 384 workgroups, 1.5 per CU
+        assert s.rhs_get_shape()["min_planes"] == 16
     with Solver(small_channel(400, 8, 8)) as s:
         assert s.rhs_get_shape()["rows"] == 1
 
