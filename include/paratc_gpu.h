@@ -195,8 +195,8 @@ int ptc_poisson_config(ptc_ctx *ctx, int prec, int tri, int tile, int sweeps);
  *                           -1 off, 0 (automatic) on where the spectrum is
  *                           larger than the last-level cache.
  *   PTC_POISSON_PERSIST     1: the x and z passes of the lengths that have
- *                           one (x: nx = 1024, 4608, 9216; z: nz = 700,
- *                           1400) run as persistent workgroups that load
+ *                           one (x: nx = 1024, 4608, 9216; z: nz = 1400)
+ *                           run as persistent workgroups that load
  *                           their next line or column tile while they
  *                           transform the current one; 2: the z passes
  *                           only; 3: the x passes only; 0 or -1: one

@@ -949,12 +949,14 @@ __global__ void __launch_bounds__(f2_threads<N>() * T) k2p_fft_z(S *__restrict__
 
 // ---- dispatch ----------------------------------------------------------------
 // lengths with persistent forms: the x lengths of the 1024-, 4608- and
-// 9216-point rows, the z lengths 700 and 1400 with their tiles.  No z form of
-// 512: at 1024x256x512 the one-shot 8-column pass ran 0.416 ms against 0.438
-// (16 columns, 44 bytes of scratch per lane at the 128 VGPRs of a 1024-thread
-// workgroup; 1.34x the read bytes) and 0.456 (8 columns) persistent.
+// 9216-point rows, the z length 1400 with its 2- and 4-column tiles.  The
+// shorter z lengths run faster one-shot: at 1024x256x512 the 8-column pass
+// took 0.416 ms against 0.438 (16 columns, 44 bytes of scratch per lane at the
+// 128 VGPRs of a 1024-thread workgroup, 1.34x the read bytes) and 0.456 (8
+// columns) persistent; at 4608x140x700 1.77 ms against 1.80-1.82 (8 columns,
+// 28 bytes of scratch per lane) and 2.18 (4 columns).
 constexpr bool has_px(int n) { return n == 512 || n == 2304 || n == 4608; }
-constexpr bool has_pz(int n, int t) { return (n == 700 && (t == 4 || t == 8)) || (n == 1400 && (t == 2 || t == 4)); }
+constexpr bool has_pz(int n, int t) { return n == 1400 && (t == 2 || t == 4); }
 
 // workgroups of a persistent kernel resident per CU (at least one)
 template <class K>

@@ -376,10 +376,10 @@ def test_persistent_passes_match_one_shot(grid, ztile, slab, prec):
             s.poisson_persist(mode)
             got = s.poisson_get_knob(capi.POISSON_PERSIST)
             if mode > 0:
-                # x: the 512-, 2304- and 4608-point lengths; z: 700, 1400;
+                # x: the 512-, 2304- and 4608-point lengths; z: 1400;
                 # mode 2 the z passes only, 3 the x passes only
                 want = ((1 if nx // 2 in (512, 2304, 4608) and mode != 2 else 0) |
-                        (2 if nz in (700, 1400) and mode != 3 else 0))
+                        (2 if nz == 1400 and mode != 3 else 0))
                 assert got == want, (got, want)
             else:
                 assert got == 0
