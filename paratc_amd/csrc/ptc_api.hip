@@ -119,22 +119,54 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
   };
   double best = 1e300;
   Cand pick{PTC_TRI_AUTO, 0, 0, 0, 0, 0, 0, 0, 0};
-  // A slabbed plan must beat the best unslabbed one by 5 %: only an
-  // unslabbed FP64 plan lets a time step form the Poisson source in its x
-  // pass (ChannelIterate, no separate clcPrSrc pass of 32 B per point; at
-  // 4608x140x700 that is 1.2 ms of a 12 ms solve), which a solve-only timing
-  // does not see.
+  // A slabbed plan must beat the unslabbed ones by 5 % (its time counts 1/0.95
+  // of what it measures): only an unslabbed FP64 plan lets a time step form
+  // the Poisson source in its x pass (ChannelIterate, no separate clcPrSrc
+  // pass of 32 B per point; at 4608x140x700 that is 1.2 ms of a 12 ms solve),
+  // which a solve-only timing does not see.
   constexpr double kSlabMargin = 0.05;
+  auto rank = [&](const Cand &cd, double ms) { return cd.slab > 0 ? ms / (1.0 - kSlabMargin) : ms; };
+  struct Timed {
+    double ms;
+    Cand cd;
+  };
+  std::vector<Timed> timed;
   auto consider = [&](const Cand &cd) {
     double ms = 0;
-    const double bar = cd.slab > 0 && pick.slab == 0 ? best * (1.0 - kSlabMargin) : best;
-    if (run(cd, &ms) == PTC_OK && ms < bar) {
-      best = ms;
+    if (run(cd, &ms) != PTC_OK) return;
+    const double r = rank(cd, ms);
+    timed.push_back(Timed{r, cd});
+    if (r < best) {
+      best = r;
       pick = cd;
     }
   };
+  // The three fastest so far are timed again, in turn, twice, and the fastest
+  // of those timings becomes the pick.  One timing can be off by a few per
+  // cent (the clocks change over a long search), as much as separates close
+  // plans: at 4608x140x700 the 8- and 4-column z passes differ by 3.5 % per
+  // solve, and single timings picked either.  Done once the FFT kernels are
+  // chosen (the later candidates build on them) and at the end.
+  auto settle = [&]() {
+    std::vector<Timed> t = timed;
+    std::stable_sort(t.begin(), t.end(), [](const Timed &a, const Timed &b) { return a.ms < b.ms; });
+    const int n = std::min<int>(3, (int)t.size());
+    if (n < 2) return;
+    double fin[3] = {1e300, 1e300, 1e300};
+    for (int round = 0; round < 2; ++round)
+      for (int q = 0; q < n; ++q) {
+        double ms = 0;
+        if (run(t[q].cd, &ms) == PTC_OK) fin[q] = std::min(fin[q], rank(t[q].cd, ms));
+      }
+    int w = 0;
+    for (int q = 1; q < n; ++q)
+      if (fin[q] < fin[w]) w = q;
+    pick = t[w].cd;
+    best = fin[w];
+  };
   for (int tile : {0, 16, 8, 4}) consider(Cand{PTC_TRI_AUTO, 0, tile, 0, 0, 0, 0, 0, 0});
   for (int zf : {2, 4, 8, 16, 32}) consider(Cand{PTC_TRI_AUTO, 0, 0, 0, 0, zf, 0, 0, 0});
+  settle();
   const int tile = pick.tile, zf = pick.zf;
   for (int mm : {4, 8, 12, 16}) {
     for (int lanes : {64, 32, 16}) consider(Cand{PTC_TRI_PARTITIONED, mm, tile, 0, 0, zf, lanes, 0, 0});
@@ -155,6 +187,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
         if (zf != b2.zf && ps != 3) consider(Cand{b2.tri, b2.mm, 0, b2.slab, b2.ll, zf, b2.lanes, b2.cl, ps});
     }
   }
+  settle();
   poisson_free_plan(c);
   apply(pick);
   PTC_TRY(poisson_build_plan(c));
