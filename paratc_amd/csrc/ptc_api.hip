@@ -199,6 +199,38 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
 }
 
 // ---- one time step (ChannelIterate) ---------------------------------------
+// The substep's clcScalar, clcU1Hat, clcU2Hat, clcU3Hat
+// (src/m_ChannelSystem.f90:111,116-118): independent solves, one launch, with
+// the current substep's beta.  clcU2Hat carries Update_uy_ym (:105: nothing in
+// between reads uy_ym or duy_ym) and SetBC_and_UpdateHaloForPrSrc's v wall
+// rows (:122).
+static int step_cn_jobs(ptc_ctx *c, CnJobs *cn) {
+  double *ux, *uy, *uz, *a1, *a2, *hl, *uym, *duym;
+  PTC_TRY(field(c, PTC_F_UX, &ux));
+  PTC_TRY(field(c, PTC_F_UY, &uy));
+  PTC_TRY(field(c, PTC_F_UZ, &uz));
+  PTC_TRY(field(c, PTC_F_ARR1, &a1));
+  PTC_TRY(field(c, PTC_F_ARR2, &a2));
+  PTC_TRY(field(c, PTC_F_HALO, &hl));
+  PTC_TRY(field(c, PTC_F_UY_YM, &uym));
+  PTC_TRY(field(c, PTC_F_DUY_YM, &duym));
+  *cn = CnJobs{};
+  if (c->prm.scalar != 0) {
+    double *sc, *ac;
+    PTC_TRY(field(c, PTC_F_SCALAR, &sc));
+    PTC_TRY(field(c, PTC_F_ARRC, &ac));
+    cn->j[cn->n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
+  }
+  cn->j[cn->n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
+  CnJob v = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
+  v.uym_upd = uym;
+  v.duym_upd = duym;
+  v.top = c->prm.flow_type == 1 ? c->prm.uybc[3] : 0.0;
+  cn->j[cn->n++] = v;
+  cn->j[cn->n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
+  return PTC_OK;
+}
+
 int channel_step(ptc_ctx *c, double *mon) {
   double *ux, *uy, *uz, *p, *hx, *hy, *hz, *a1, *a2, *hl, *uym, *duym;
   double *sc = nullptr, *hc = nullptr, *ac = nullptr;
@@ -233,18 +265,10 @@ int channel_step(ptc_ctx *c, double *mon) {
     PTC_TRY(pm_coe_update(c, ns, dt));
     // Update_uy_ym (:105) runs inside the v solve below: nothing in between reads uy_ym or duy_ym
     PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p));
-    {  // clcScalar, clcU1Hat, clcU2Hat, clcU3Hat (:111,116-118): independent solves, one launch
+    {
       CnJobs cn{};
-      if (scal) cn.j[cn.n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
-      cn.j[cn.n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
-      // clcU2Hat with Update_uy_ym and SetBC_and_UpdateHaloForPrSrc's v wall rows (:122)
-      CnJob v = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
-      v.uym_upd = uym;
-      v.duym_upd = duym;
-      v.top = c->prm.flow_type == 1 ? c->prm.uybc[3] : 0.0;
-      cn.j[cn.n++] = v;
-      cn.j[cn.n++] = cn_job(0, uz, hl, nullptr, c->pmBeta, nullptr);
-      PTC_TRY(launch_tri_cn_jobs(c, cn));
+      PTC_TRY(step_cn_jobs(c, &cn));
+      PTC_TRY(launch_tri_cn_step(c, cn, ns));
     }
     // clcPrSrc and clcPPE (:124-126), with SetBC_and_UpdateHalo_pr(prphiHalo):
     // the divergence is formed in the Poisson x pass where the plan allows,
@@ -286,10 +310,23 @@ int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec) {
                           c->pmBetaSc, c->dt, c->simtime, c->prgradaver};
   const bool prof = c->pp.profile;
   c->pp.profile = false;
-  PTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = channel_step(c, nullptr);
+  // the Crank-Nicolson factor tables of every substep formed before the
+  // capture, so the captured step only reads them (the coefficients the
+  // substeps set are put back below)
+  int rc = PTC_OK;
+  for (int ns = 1; ns <= c->iadvance && rc == PTC_OK; ++ns) {
+    CnJobs cn{};
+    rc = pm_coe_update(c, ns, c->prm.dtmax);
+    if (rc == PTC_OK) rc = step_cn_jobs(c, &cn);
+    if (rc == PTC_OK) rc = launch_tri_cn_step(c, cn, ns, false);
+  }
   hipGraph_t g = nullptr;
-  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  hipError_t ec = hipSuccess;
+  if (rc == PTC_OK) {
+    PTC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = channel_step(c, nullptr);
+    ec = hipStreamEndCapture(c->stream, &g);
+  }
   c->pp.profile = prof;
   c->pmGamma = save[0];
   c->pmTheta = save[1];

@@ -354,7 +354,7 @@ int create(const ptc_grid *grid, const ptc_params *prm, const double *yp, ptc_ct
       hipMalloc(&c->d_wnx, sizeof(double) * c->nx) != hipSuccess ||
       hipMalloc(&c->d_wnz, sizeof(double) * c->nz) != hipSuccess ||
       hipMalloc(&c->d_scal, sizeof(double) * 64) != hipSuccess ||
-      hipMalloc(&c->d_cnfac, sizeof(double) * kCnMaxJobs * kCnFacStride) != hipSuccess ||
+      hipMalloc(&c->d_cnfac, sizeof(double) * (kCnMaxJobs + kCnStepTables) * kCnFacStride) != hipSuccess ||
       hipHostMalloc(&c->h_pinned, sizeof(double) * 64) != hipSuccess) {
     set_error("device allocation failed in ptc_create");
     return fail(PTC_E_ALLOC);

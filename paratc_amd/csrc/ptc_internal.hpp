@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -158,7 +159,11 @@ struct ptc_ctx {
   double *d_red = nullptr;     // reduction partials
   int red_cap = 0;
   double *d_scal = nullptr;    // device scalars
-  double *d_cnfac = nullptr;   // Crank-Nicolson factor tables, kCnMaxJobs x kCnFacStride
+  double *d_cnfac = nullptr;   // Crank-Nicolson factor tables (kCnMaxJobs + kCnStepTables) x kCnFacStride
+  // beta each step table (substep ns, kind) holds, NaN when none; the tables
+  // also depend on the chunk plan (cn_tab_mm, cn_tab_p) and the metrics
+  double cn_tab_beta[9] = {NAN, NAN, NAN, NAN, NAN, NAN, NAN, NAN, NAN};
+  int cn_tab_mm = 0, cn_tab_p = 0;
   double *h_pinned = nullptr;  // pinned host scalars
   hipStream_t stream = nullptr;
   int device = 0;
@@ -325,8 +330,14 @@ struct CnJob {
   // the old uy_ym, uses -old as the wall du, writes uy_ym = 0 and
   // duy_ym = -old, and fills the wall row with the new uy_ym
   double *uym_upd, *duym_upd;
+  // factor table of the job (index into d_cnfac) and whether the launch
+  // forms it (k_cn_factor) or finds it formed by an earlier launch
+  int tab, need;
 };
 constexpr int kCnMaxJobs = 4;
+// tables kept across steps: one per substep (1..3) and kind (0 u/w, 1 v, 2
+// scalar), after the kCnMaxJobs per-launch ones
+constexpr int kCnStepTables = 9;
 // doubles of one job's Crank-Nicolson factor table (chunk rows P * MM <= 256,
 // 2P <= 64 reduced rows, at most 6 cyclic-reduction steps: 4 * 256 + 64 +
 // 4 * 6 * 64 = 2624)
@@ -337,6 +348,10 @@ struct CnJobs {
 };
 CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, double beta, const double *shift);
 int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs);
+// the jobs of substep ns of a step: each job's factors in its kept step
+// table, formed only where the table holds another beta (or none); with
+// solve false, only the factors
+int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve = true);
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym,
                   double beta, const double *shift);
 

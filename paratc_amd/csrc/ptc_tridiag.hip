@@ -532,7 +532,8 @@ __global__ void __launch_bounds__(64) k_cn_factor(CnJobs jobs, const double *__r
                                                   int P, double *__restrict__ fac) {
   __shared__ double sa[2][64], sc[2][64];
   const CnJob &J = jobs.j[blockIdx.x];
-  double *T = fac + (long long)blockIdx.x * kCnFacStride;
+  if (!J.need) return;  // formed by an earlier launch, or by another job of this one
+  double *T = fac + (long long)J.tab * kCnFacStride;
   const CnFacIdx X{P * MM, P};
   const int t = threadIdx.x, n2 = 2 * P;
   if (t < P) {
@@ -611,7 +612,7 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_fac(CnJobs jo
   extern __shared__ double smem[];  // f of the 2P reduced rows x LPB lines, double-buffered
   double *RF = smem;
   const CnJob &J = jobs.j[blockIdx.z];
-  const double *__restrict__ T = fac + (long long)blockIdx.z * kCnFacStride;
+  const double *__restrict__ T = fac + (long long)J.tab * kCnFacStride;
   const CnFacIdx X{P * MM, P};
   double *__restrict__ u = J.u;
   const double *__restrict__ rhs = J.rhs;
@@ -904,29 +905,34 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
   J.shift = shift;
   J.beta = beta;
   J.kind = kind;
+  J.tab = 0;
+  J.need = 1;
   if (kind == 0) { J.rA = M_AM2CCN; J.rB = M_AC2CCN; J.rC = M_AP2CCN; }
   else if (kind == 1) { J.rA = M_AM2P; J.rB = M_AC2P; J.rC = M_AP2P; }
   else { J.rA = M_AM2CSCCN; J.rB = M_AC2CSCCN; J.rC = M_AP2CSCCN; }
   return J;
 }
 
-// one launch for all the jobs (grid z = job), after the factor tables
-int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
-  PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
+// the factor launch (where a job needs one) and the solve launch (grid z = job)
+static int launch_cn(ptc_ctx *c, const CnJobs &jobs, bool factor, bool solve) {
   int nP = 1, mmax = 16, lpb = 64;
   PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
   PTC_CHECK(nP * mmax <= 256 && 2 * nP <= 64, PTC_E_STATE, "Crank-Nicolson chunk plan outside the factor table");
   dim3 grid((unsigned)((c->nx + lpb - 1) / lpb), (unsigned)c->nz, (unsigned)jobs.n);
   const size_t lds = sizeof(double) * 2 * (size_t)(2 * nP) * lpb;
-#define PTC_CN(MM, LPB)                                                                                      \
-  if (mmax == MM && lpb == LPB) {                                                                            \
-    hipLaunchKernelGGL((k_cn_factor<MM>), dim3(jobs.n), dim3(64), 0, c->stream, jobs, c->d_met, c->ny + 2,  \
-                       c->ny, nP, c->d_cnfac);                                                               \
-    PTC_HIP(hipGetLastError());                                                                              \
-    hipLaunchKernelGGL((k_tri_cn_fac<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_cnfac,      \
-                       c->nx, c->ny, nP);                                                                    \
-    PTC_HIP(hipGetLastError());                                                                              \
-    return PTC_OK;                                                                                           \
+#define PTC_CN(MM, LPB)                                                                                        \
+  if (mmax == MM && lpb == LPB) {                                                                              \
+    if (factor) {                                                                                              \
+      hipLaunchKernelGGL((k_cn_factor<MM>), dim3(jobs.n), dim3(64), 0, c->stream, jobs, c->d_met, c->ny + 2,  \
+                         c->ny, nP, c->d_cnfac);                                                               \
+      PTC_HIP(hipGetLastError());                                                                              \
+    }                                                                                                          \
+    if (solve) {                                                                                               \
+      hipLaunchKernelGGL((k_tri_cn_fac<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_cnfac,      \
+                         c->nx, c->ny, nP);                                                                    \
+      PTC_HIP(hipGetLastError());                                                                              \
+    }                                                                                                          \
+    return PTC_OK;                                                                                             \
   }
   PTC_CN(4, 64)
   PTC_CN(8, 64)
@@ -936,6 +942,57 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
 #undef PTC_CN
   set_error("bad tridiagonal launch shape");
   return PTC_E_STATE;
+}
+
+// one launch for all the jobs, after the factor tables of this launch (the
+// per-launch tables 0 .. n-1)
+int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
+  PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
+  CnJobs J = jobs;
+  for (int q = 0; q < J.n; ++q) {
+    J.j[q].tab = q;
+    J.j[q].need = 1;
+  }
+  return launch_cn(c, J, true, true);
+}
+
+// The matrices of a fixed-dt step repeat every step: per substep, one for u
+// and w, one for v, one for the scalar.  Their tables are kept (tables
+// kCnMaxJobs + 3 (ns - 1) + kind) with the beta each was formed for, and the
+// factor launch runs only for a table whose beta differs (the first step,
+// or every substep of a CFL-adaptive one).  At 64 x 48 x 64 that launch is
+// about 4.5 us of a 75 us substep.
+int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve) {
+  PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
+  PTC_CHECK(ns >= 1 && ns <= 3, PTC_E_ARG, "substep index out of range");
+  int nP = 1, mmax = 16, lpb = 64;
+  PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
+  if (mmax != c->cn_tab_mm || nP != c->cn_tab_p) {
+    for (double &b : c->cn_tab_beta) b = NAN;
+    c->cn_tab_mm = mmax;
+    c->cn_tab_p = nP;
+  }
+  bool any = false, formed[kCnStepTables] = {};
+  for (int q = 0; q < jobs.n; ++q) {
+    CnJob &J = jobs.j[q];
+    PTC_CHECK(J.kind >= 0 && J.kind <= 2, PTC_E_ARG, "bad Crank-Nicolson job kind");
+    const int t = 3 * (ns - 1) + J.kind;
+    J.tab = kCnMaxJobs + t;
+    for (int r = 0; r < q; ++r)
+      PTC_CHECK(jobs.j[r].tab != J.tab || jobs.j[r].beta == J.beta, PTC_E_ARG,
+                "two betas for one Crank-Nicolson table");
+    // formed by an earlier launch (the cached beta is NaN for none), or by an
+    // earlier job of this one (u and w share a table)
+    J.need = !(c->cn_tab_beta[t] == J.beta) && !formed[t];
+    if (J.need) {
+      formed[t] = true;
+      any = true;
+    }
+  }
+  PTC_TRY(launch_cn(c, jobs, any, solve));
+  for (int q = 0; q < jobs.n; ++q)
+    if (jobs.j[q].need) c->cn_tab_beta[jobs.j[q].tab - kCnMaxJobs] = jobs.j[q].beta;
+  return PTC_OK;
 }
 
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
