@@ -20,7 +20,6 @@ int download(ptc_ctx *c, int id, double *host, int layout);
 int copy_device(ptc_ctx *c, double *dst, const double *src, long long n);
 size_t field_count(const ptc_ctx *c, int id);
 int launch_add_shift(ptc_ctx *c, double *rhsx);
-const double *flowrate_shift_ptr(ptc_ctx *c);
 int set_prgrad(ptc_ctx *c, double v);
 int get_device_scalar(ptc_ctx *c, int slot, double *out);
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym, double beta,
@@ -41,7 +40,9 @@ static int opt_field(ptc_ctx *c, int id, double **out) {
 
 // PMcoeUpdate (src/m_Parameters.f90:215-230); the PrGradAver reset of ns == 1
 // is a device write ordered on the stream.
-static int pm_coe_update(ptc_ctx *c, int ns, double dt) {
+// zero_prgrad: the first substep zeroes PrGradAver's device sum (a step whose
+// Crank-Nicolson launch forms the flow-rate correction sets it there instead)
+static int pm_coe_update(ptc_ctx *c, int ns, double dt, bool zero_prgrad = true) {
   PTC_CHECK(ns >= 1 && ns <= c->iadvance, PTC_E_ARG, "substep index out of range");
   const double a = c->aconst[ns - 1];
   c->dt = dt;
@@ -52,7 +53,7 @@ static int pm_coe_update(ptc_ctx *c, int ns, double dt) {
   c->pmBetaSc = 0.5 * a * c->prm.sdiff * dt;
   c->simtime = c->simtime + dt * a;
   c->pmAlphaC = a;
-  if (ns == 1) {
+  if (ns == 1 && zero_prgrad) {
     hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, c->stream, c->d_scal + S_PRGRAD, 0.0);
     PTC_HIP(hipGetLastError());
   }
@@ -203,7 +204,8 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
 // (src/m_ChannelSystem.f90:111,116-118): independent solves, one launch, with
 // the current substep's beta.  clcU2Hat carries Update_uy_ym (:105: nothing in
 // between reads uy_ym or duy_ym) and SetBC_and_UpdateHaloForPrSrc's v wall
-// rows (:122).
+// rows (:122); with a constant flow rate clcU1Hat forms the flow-rate
+// correction of its right-hand side from the RHS kernel's partial sums.
 static int step_cn_jobs(ptc_ctx *c, CnJobs *cn) {
   double *ux, *uy, *uz, *a1, *a2, *hl, *uym, *duym;
   PTC_TRY(field(c, PTC_F_UX, &ux));
@@ -221,7 +223,9 @@ static int step_cn_jobs(ptc_ctx *c, CnJobs *cn) {
     PTC_TRY(field(c, PTC_F_ARRC, &ac));
     cn->j[cn->n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
   }
-  cn->j[cn->n++] = cn_job(0, ux, a1, nullptr, c->pmBeta, c->prm.is_ux_const ? flowrate_shift_ptr(c) : nullptr);
+  CnJob u = cn_job(0, ux, a1, nullptr, c->pmBeta, nullptr);
+  u.fold = c->prm.is_ux_const ? 1 : 0;
+  cn->j[cn->n++] = u;
   CnJob v = cn_job(1, uy, a2, duym, c->pmBeta, nullptr);
   v.uym_upd = uym;
   v.duym_upd = duym;
@@ -261,14 +265,19 @@ int channel_step(ptc_ctx *c, double *mon) {
     cflmp = uddxmax * c->dt;
     if (c->prm.icfl == 1) dt = std::min(c->prm.cflc / uddxmax, c->prm.dtmax);
   }
+  // constant flow rate: the Crank-Nicolson launch sums the RHS kernel's
+  // flow-rate partials itself (no k_flowrate launch, no PrGradAver zeroing)
+  const bool fold = c->prm.is_ux_const != 0;
   for (int ns = 1; ns <= c->iadvance; ++ns) {
-    PTC_TRY(pm_coe_update(c, ns, dt));
+    PTC_TRY(pm_coe_update(c, ns, dt, !fold));
     // Update_uy_ym (:105) runs inside the v solve below: nothing in between reads uy_ym or duy_ym
-    PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p));
+    PTC_TRY(launch_rhs(c, ux, uy, uz, sc, a1, a2, hl, ac, hx, hy, hz, hc, p, fold));
     {
       CnJobs cn{};
+      FlowFold ff{};
       PTC_TRY(step_cn_jobs(c, &cn));
-      PTC_TRY(launch_tri_cn_step(c, cn, ns));
+      if (fold) PTC_TRY(flow_fold(c, ns == 1, &ff));
+      PTC_TRY(launch_tri_cn_step(c, cn, ns, true, fold ? &ff : nullptr));
     }
     // clcPrSrc and clcPPE (:124-126), with SetBC_and_UpdateHalo_pr(prphiHalo):
     // the divergence is formed in the Poisson x pass where the plan allows,

@@ -333,6 +333,20 @@ struct CnJob {
   // factor table of the job (index into d_cnfac) and whether the launch
   // forms it (k_cn_factor) or finds it formed by an earlier launch
   int tab, need;
+  // u job of a constant-flow-rate step: the shift added to the right-hand
+  // side comes from the flow-rate partials (FlowFold), not from shift
+  int fold;
+};
+// The constant-flow-rate correction (src/m_TScheme.f90:269-300) formed inside
+// the Crank-Nicolson launch: the RHS kernel's n partial sums of the bulk flow
+// rate and the coefficients k_flowrate would take; dp1ns and prgrad are the
+// device scalars it would write (prgrad set rather than added on the first
+// substep).
+struct FlowFold {
+  const double *part;
+  int n, scalar_build, first;
+  double xnu, ubulk, ucrf, nxnz, yly, alpha, alphaC, dt;
+  double *dp1ns, *prgrad;
 };
 constexpr int kCnMaxJobs = 4;
 // tables kept across steps: one per substep (1..3) and kind (0 u/w, 1 v, 2
@@ -351,14 +365,17 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs);
 // the jobs of substep ns of a step: each job's factors in its kept step
 // table, formed only where the table holds another beta (or none); with
 // solve false, only the factors
-int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve = true);
+int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve = true, const FlowFold *ff = nullptr);
 int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const double *duy_ym,
                   double beta, const double *shift);
 
 // stencils (ptc_stencil.hip)
+// defer_flowrate: a constant-flow-rate case leaves the flow-rate sum to the
+// Crank-Nicolson launch (flow_fold) instead of launching k_flowrate
+int flow_fold(ptc_ctx *c, bool first, FlowFold *ff);
 int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
                const double *sc, double *rhsx, double *rhsy, double *rhsz, double *rhsc,
-               double *hx, double *hy, double *hz, double *hc, const double *p);
+               double *hx, double *hy, double *hz, double *hc, const double *p, bool defer_flowrate = false);
 int launch_prsrc(ptc_ctx *c, const double *ux, const double *uy, const double *uz, double *prsrc,
                  double *divmax_host, bool reduce = true);
 // the same with the workgroup partials in part (room for prsrc_partials(c)

@@ -1036,7 +1036,7 @@ int ensure_red(ptc_ctx *c, int n) {
 
 int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz, const double *sc, double *rhsx,
                double *rhsy, double *rhsz, double *rhsc, double *hx, double *hy, double *hz, double *hc,
-               const double *p) {
+               const double *p, bool defer_flowrate) {
   RhsCoef cf{};
   cf.gamma = c->pmGamma;
   cf.theta = c->pmTheta;
@@ -1075,7 +1075,7 @@ int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
   }
 #undef PTC_ROWS
   PTC_HIP(hipGetLastError());
-  if (cf.s3_mode) {
+  if (cf.s3_mode && !defer_flowrate) {
     hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(kReduceThreads), 0, c->stream, c->d_red, nb, c->d_scal, cf.scalar,
                        c->prm.xnu, c->prm.ubulk, c->prm.ucrf, (double)(c->nx * c->nz), c->yly, c->pmAlpha,
                        c->pmAlphaC, c->dt);
@@ -1090,7 +1090,27 @@ int launch_add_shift(ptc_ctx *c, double *rhsx) {
   return PTC_OK;
 }
 
-const double *flowrate_shift_ptr(ptc_ctx *c) { return c->d_scal + S_DP1NS; }
+// what k_flowrate would take, for the Crank-Nicolson launch that forms the
+// correction itself (after launch_rhs has written the partials)
+int flow_fold(ptc_ctx *c, bool first, FlowFold *ff) {
+  PTC_CHECK(c->prm.is_ux_const, PTC_E_STATE, "flow-rate correction without a constant flow rate");
+  ff->part = c->d_red;
+  ff->n = rhs_nblocks(c);
+  PTC_CHECK(ff->n <= c->red_cap, PTC_E_STATE, "flow-rate partials larger than the reduction buffer");
+  ff->scalar_build = c->prm.scalar ? 1 : 0;
+  ff->first = first ? 1 : 0;
+  ff->xnu = c->prm.xnu;
+  ff->ubulk = c->prm.ubulk;
+  ff->ucrf = c->prm.ucrf;
+  ff->nxnz = (double)(c->nx * c->nz);
+  ff->yly = c->yly;
+  ff->alpha = c->pmAlpha;
+  ff->alphaC = c->pmAlphaC;
+  ff->dt = c->dt;
+  ff->dp1ns = c->d_scal + S_DP1NS;
+  ff->prgrad = c->d_scal + S_PRGRAD;
+  return PTC_OK;
+}
 
 // reduction partials of the RHS and PrSrc kernels, allocated ahead of a
 // captured step

@@ -606,9 +606,46 @@ __global__ void __launch_bounds__(64) k_cn_factor(CnJobs jobs, const double *__r
   }
 }
 
+// The constant-flow-rate shift of RhsX (src/m_TScheme.f90:269-300), in the
+// arithmetic of k_flowrate: every workgroup of the u job sums the RHS
+// kernel's partials in the same fixed order (strided per thread, then a tree
+// in red), so all hold the same shift; workgroup (0, 0) stores it and the
+// PrGradAver increment, as k_flowrate would.  Called by all threads.
+__device__ double flow_fold_shift(const FlowFold &ff, double *red) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  double acc = 0.0;
+  for (int q = t; q < ff.n; q += nt) acc += ff.part[q];
+  red[t] = acc;
+  __syncthreads();
+  for (int n = nt; n > 1;) {
+    const int h = (n + 1) / 2;
+    if (t < n - h) red[t] += red[t + h];
+    __syncthreads();
+    n = h;
+  }
+  const double tot = red[0];
+  __syncthreads();  // red is the solve's LDS
+  double shift, inc;
+  if (ff.scalar_build) {
+    const double s3 = tot / ff.nxnz / ff.yly;
+    const double dpm = ff.ubulk - ff.ucrf - s3;
+    shift = ff.alphaC * dpm;
+    inc = dpm * ff.alphaC / ff.dt;
+  } else {
+    const double dpm = ff.xnu * tot / ff.nxnz / ff.yly;
+    shift = -(ff.alpha * dpm);
+    inc = dpm * ff.alphaC;
+  }
+  if (t == 0 && blockIdx.x == 0 && blockIdx.y == 0) {
+    *ff.dp1ns = shift;
+    *ff.prgrad = (ff.first ? 0.0 : *ff.prgrad) + inc;
+  }
+  return shift;
+}
+
 template <int MM, int LPB>
 __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_fac(CnJobs jobs, const double *__restrict__ fac,
-                                                                     int nx, int ny, int P) {
+                                                                     int nx, int ny, int P, FlowFold ff) {
   extern __shared__ double smem[];  // f of the 2P reduced rows x LPB lines, double-buffered
   double *RF = smem;
   const CnJob &J = jobs.j[blockIdx.z];
@@ -623,17 +660,19 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_fac(CnJobs jo
   const bool valid = i < nx;
   const int s = p * MM;
   const long long col = (long long)(valid ? i : 0) + (long long)nx * ((long long)(ny + 2) * k);
-  const double sh = J.shift ? J.shift[0] : 0.0;
   // the right-hand side and the old u, all rows in flight at once
   // (unconditional loads: padding rows read row ny, masked by a multiply)
   double f[MM], uo[MM];
 #pragma unroll
   for (int r = 0; r < MM; ++r) {
     const long long o = col + (long long)nx * (min(s + r, ny - 1) + 1);
-    const double v = rhs[o];
+    f[r] = rhs[o];
     uo[r] = u[o];
-    f[r] = (v + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   }
+  double sh = J.shift ? J.shift[0] : 0.0;
+  if (J.fold) sh = flow_fold_shift(ff, RF);  // uniform over the workgroup (one job)
+#pragma unroll
+  for (int r = 0; r < MM; ++r) f[r] = (f[r] + sh) * ((valid && s + r < ny) ? 1.0 : 0.0);
   // du at the wall row: duy_ym, or with Update_uy_ym folded in (src/m_FlowCase.f90:112-122)
   // the new uy_ym (zero) minus the old one, both planes written after the solve
   const double wall_new = 0.0;
@@ -907,6 +946,7 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
   J.kind = kind;
   J.tab = 0;
   J.need = 1;
+  J.fold = 0;
   if (kind == 0) { J.rA = M_AM2CCN; J.rB = M_AC2CCN; J.rC = M_AP2CCN; }
   else if (kind == 1) { J.rA = M_AM2P; J.rB = M_AC2P; J.rC = M_AP2P; }
   else { J.rA = M_AM2CSCCN; J.rB = M_AC2CSCCN; J.rC = M_AP2CSCCN; }
@@ -914,7 +954,7 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
 }
 
 // the factor launch (where a job needs one) and the solve launch (grid z = job)
-static int launch_cn(ptc_ctx *c, const CnJobs &jobs, bool factor, bool solve) {
+static int launch_cn(ptc_ctx *c, const CnJobs &jobs, bool factor, bool solve, const FlowFold &ff) {
   int nP = 1, mmax = 16, lpb = 64;
   PTC_TRY(tri_plan(c->ny, c->lds_limit, false, 0, 0, &nP, &mmax, &lpb));
   PTC_CHECK(nP * mmax <= 256 && 2 * nP <= 64, PTC_E_STATE, "Crank-Nicolson chunk plan outside the factor table");
@@ -929,7 +969,7 @@ static int launch_cn(ptc_ctx *c, const CnJobs &jobs, bool factor, bool solve) {
     }                                                                                                          \
     if (solve) {                                                                                               \
       hipLaunchKernelGGL((k_tri_cn_fac<MM, LPB>), grid, dim3(LPB * nP), lds, c->stream, jobs, c->d_cnfac,      \
-                         c->nx, c->ny, nP);                                                                    \
+                         c->nx, c->ny, nP, ff);                                                                \
       PTC_HIP(hipGetLastError());                                                                              \
     }                                                                                                          \
     return PTC_OK;                                                                                             \
@@ -952,8 +992,9 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
   for (int q = 0; q < J.n; ++q) {
     J.j[q].tab = q;
     J.j[q].need = 1;
+    PTC_CHECK(!J.j[q].fold, PTC_E_ARG, "flow-rate folding is for the step's launch");
   }
-  return launch_cn(c, J, true, true);
+  return launch_cn(c, J, true, true, FlowFold{});
 }
 
 // The matrices of a fixed-dt step repeat every step: per substep, one for u
@@ -962,7 +1003,7 @@ int launch_tri_cn_jobs(ptc_ctx *c, const CnJobs &jobs) {
 // factor launch runs only for a table whose beta differs (the first step,
 // or every substep of a CFL-adaptive one).  At 64 x 48 x 64 that launch is
 // about 4.5 us of a 75 us substep.
-int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve) {
+int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve, const FlowFold *ff) {
   PTC_CHECK(jobs.n >= 1 && jobs.n <= kCnMaxJobs, PTC_E_ARG, "bad Crank-Nicolson job count");
   PTC_CHECK(ns >= 1 && ns <= 3, PTC_E_ARG, "substep index out of range");
   int nP = 1, mmax = 16, lpb = 64;
@@ -976,6 +1017,8 @@ int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve) {
   for (int q = 0; q < jobs.n; ++q) {
     CnJob &J = jobs.j[q];
     PTC_CHECK(J.kind >= 0 && J.kind <= 2, PTC_E_ARG, "bad Crank-Nicolson job kind");
+    PTC_CHECK(!solve || !J.fold || (ff != nullptr && J.shift == nullptr), PTC_E_ARG,
+              "flow-rate folding without its sums");
     const int t = 3 * (ns - 1) + J.kind;
     J.tab = kCnMaxJobs + t;
     for (int r = 0; r < q; ++r)
@@ -989,7 +1032,7 @@ int launch_tri_cn_step(ptc_ctx *c, CnJobs jobs, int ns, bool solve) {
       any = true;
     }
   }
-  PTC_TRY(launch_cn(c, jobs, any, solve));
+  PTC_TRY(launch_cn(c, jobs, any, solve, ff ? *ff : FlowFold{}));
   for (int q = 0; q < jobs.n; ++q)
     if (jobs.j[q].need) c->cn_tab_beta[jobs.j[q].tab - kCnMaxJobs] = jobs.j[q].beta;
   return PTC_OK;
