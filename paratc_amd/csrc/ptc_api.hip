@@ -221,7 +221,17 @@ static int step_cn_jobs(ptc_ctx *c, CnJobs *cn) {
     double *sc, *ac;
     PTC_TRY(field(c, PTC_F_SCALAR, &sc));
     PTC_TRY(field(c, PTC_F_ARRC, &ac));
-    cn->j[cn->n++] = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
+    // clcScalar with SetBC_and_UpdateHalo_scalar (:138) folded in: nothing
+    // between the solve and the end of the substep reads or writes c
+    CnJob sj = cn_job(2, sc, ac, nullptr, c->pmBetaSc, nullptr);
+    double r[4];
+    PTC_TRY(scalar_bc_coefs(c, r));
+    sj.sbc = 1;
+    sj.r1m = r[0];
+    sj.r2m = r[1];
+    sj.r1p = r[2];
+    sj.r2p = r[3];
+    cn->j[cn->n++] = sj;
   }
   CnJob u = cn_job(0, ux, a1, nullptr, c->pmBeta, nullptr);
   u.fold = c->prm.is_ux_const ? 1 : 0;
@@ -295,7 +305,6 @@ int channel_step(ptc_ctx *c, double *mon) {
     // FluidVelUpdate, PressureUpdate, then SetBC_and_UpdateHalo and
     // SetBC_and_UpdateHalo_pr (:129-136) in one launch
     PTC_TRY(launch_vel_p_update_bc(c, hl, ux, uy, uz, p, uym));
-    if (scal) PTC_TRY(launch_bc_scalar(c, sc));
   }
   if (mon) {
     PTC_TRY(get_device_scalar(c, S_DIVMAX, &mon[0]));

@@ -336,6 +336,11 @@ struct CnJob {
   // u job of a constant-flow-rate step: the shift added to the right-hand
   // side comes from the flow-rate partials (FlowFold), not from shift
   int fold;
+  // scalar job of a step: the solve also writes the wall ghost rows,
+  // c(0) = r1m c(1) + r2m and c(ny+1) = r1p c(ny) + r2p
+  // (SetBC_and_UpdateHalo_scalar, src/m_BC_and_Halo.f90:137-176)
+  int sbc;
+  double r1m, r2m, r1p, r2p;
 };
 // The constant-flow-rate correction (src/m_TScheme.f90:269-300) formed inside
 // the Crank-Nicolson launch: the RHS kernel's n partial sums of the bulk flow
@@ -373,6 +378,8 @@ int launch_tri_cn(ptc_ctx *c, int kind, double *u, const double *rhs, const doub
 // defer_flowrate: a constant-flow-rate case leaves the flow-rate sum to the
 // Crank-Nicolson launch (flow_fold) instead of launching k_flowrate
 int flow_fold(ptc_ctx *c, bool first, FlowFold *ff);
+// the scalar's wall-row coefficients (r1m, r2m, r1p, r2p)
+int scalar_bc_coefs(const ptc_ctx *c, double *r);
 int launch_rhs(ptc_ctx *c, const double *ux, const double *uy, const double *uz,
                const double *sc, double *rhsx, double *rhsy, double *rhsz, double *rhsc,
                double *hx, double *hy, double *hz, double *hc, const double *p, bool defer_flowrate = false);

@@ -1241,9 +1241,8 @@ int launch_bc_pr(ptc_ctx *c, double *p) {
   return launch_bc(c, b, p, nullptr, nullptr, nullptr, nullptr);
 }
 
-int launch_bc_scalar(ptc_ctx *c, double *s) {
+int scalar_bc_coefs(const ptc_ctx *c, double *r) {
   BcArgs b{};
-  b.mode = 3;
   const double *v = c->prm.sbc_val;
   const double dyc1 = c->h_met[(size_t)M_DYC * (c->ny + 2) + 1];
   const double dycp = c->h_met[(size_t)M_DYC * (c->ny + 2) + c->ny + 1];
@@ -1266,6 +1265,22 @@ int launch_bc_scalar(ptc_ctx *c, double *s) {
       break;
     default: set_error("ScalarBcOption(2) wrong !!!"); return PTC_E_ARG;
   }
+  r[0] = b.r1m;
+  r[1] = b.r2m;
+  r[2] = b.r1p;
+  r[3] = b.r2p;
+  return PTC_OK;
+}
+
+int launch_bc_scalar(ptc_ctx *c, double *s) {
+  BcArgs b{};
+  b.mode = 3;
+  double r[4];
+  PTC_TRY(scalar_bc_coefs(c, r));
+  b.r1m = r[0];
+  b.r2m = r[1];
+  b.r1p = r[2];
+  b.r2p = r[3];
   return launch_bc(c, b, s, nullptr, nullptr, nullptr, nullptr);
 }
 

@@ -734,7 +734,11 @@ __global__ void __launch_bounds__(tri_cn_block_cap<MM>()) k_tri_cn_fac(CnJobs jo
     for (int r = 0; r < MM; ++r)
       if (s + r < ny) {
         const long long o = col + (long long)nx * (s + r + 1);
-        u[o] = (wall && s + r == 0) ? (J.uym_upd ? wall_new : J.wall[i + (long long)nx * k]) : uo[r] + f[r];
+        const double v = (wall && s + r == 0) ? (J.uym_upd ? wall_new : J.wall[i + (long long)nx * k]) : uo[r] + f[r];
+        u[o] = v;
+        // the scalar's wall ghost rows from rows 1 and ny
+        if (J.sbc && s + r == 0) u[col] = J.r1m * v + J.r2m;
+        if (J.sbc && s + r == ny - 1) u[col + (long long)nx * (ny + 1)] = J.r1p * v + J.r2p;
       }
     if (wall && s <= ny - 1 && ny - 1 < s + MM) u[col + (long long)nx * (ny + 1)] = J.top;
   }
@@ -947,6 +951,8 @@ CnJob cn_job(int kind, double *u, const double *rhs, const double *duy_ym, doubl
   J.tab = 0;
   J.need = 1;
   J.fold = 0;
+  J.sbc = 0;
+  J.r1m = J.r2m = J.r1p = J.r2p = 0.0;
   if (kind == 0) { J.rA = M_AM2CCN; J.rB = M_AC2CCN; J.rC = M_AP2CCN; }
   else if (kind == 1) { J.rA = M_AM2P; J.rB = M_AC2P; J.rC = M_AP2P; }
   else { J.rA = M_AM2CSCCN; J.rB = M_AC2CSCCN; J.rC = M_AP2CSCCN; }
