@@ -250,6 +250,13 @@ int ptc_rhs_get(ptc_ctx *ctx, int knob, int *value);
  * workgroup, slab planes, LDS-staged lines (0 = register kernel)] (after
  * autotuning; chunk fields are 0 for Thomas) */
 int ptc_poisson_get_config(ptc_ctx *ctx, int *out10);
+/* The last ptc_init_poisson_solver search as text, one line per timing: each
+ * candidate plan's knobs and ms per solve, the re-timings of the closest
+ * ones, the pick (the reference prints its four choices' times,
+ * src/m_Poisson.f90:50-171).  Copies at most cap - 1 bytes and a NUL into
+ * buf (buf may be NULL with cap 0); *len (may be NULL) receives the full
+ * length. */
+int ptc_poisson_search_log(ptc_ctx *ctx, char *buf, int cap, int *len);
 /* FP64 residual max|src - L dst| over the interior (test and refinement aid) */
 int ptc_poisson_residual(ptc_ctx *ctx, int src, int dst, double *rmax);
 /* Per-stage kernel times: with enable = 1 every later solve records HIP events

@@ -42,7 +42,7 @@ EXPORTS = [
     "ptc_create", "ptc_destroy", "ptc_last_error", "ptc_abi_version", "ptc_upload", "ptc_download",
     "ptc_fill", "ptc_copy", "ptc_field_device_ptr", "ptc_synchronize", "ptc_get_stream", "ptc_get_metric",
     "ptc_pm_coe_update", "ptc_get_coe", "ptc_set_coe", "ptc_init_poisson_solver", "ptc_clc_ppe",
-    "ptc_destroy_poisson_plan", "ptc_poisson_config", "ptc_poisson_set", "ptc_poisson_get_knob", "ptc_rhs_set", "ptc_rhs_get", "ptc_poisson_get_config", "ptc_poisson_residual",
+    "ptc_destroy_poisson_plan", "ptc_poisson_config", "ptc_poisson_set", "ptc_poisson_get_knob", "ptc_rhs_set", "ptc_rhs_get", "ptc_poisson_get_config", "ptc_poisson_search_log", "ptc_poisson_residual",
     "ptc_poisson_stage_times", "ptc_init_time_scheme", "ptc_clc_rhs_xyz", "ptc_clc_u1hat",
     "ptc_clc_u2hat", "ptc_clc_u3hat", "ptc_clc_scalar", "ptc_clc_prsrc", "ptc_clc_prsrc_noalloc", "ptc_fluid_vel_update",
     "ptc_pressure_update", "ptc_set_bc_and_update_halo", "ptc_set_bc_and_update_halo_for_prsrc",
@@ -112,6 +112,7 @@ def load(path: str = LIB_PATH):
     L.ptc_poisson_config.argtypes = [vp, ip, ip, ip, ip]
     L.ptc_poisson_residual.argtypes = [vp, ip, ip, dp]
     L.ptc_poisson_get_config.argtypes = [vp, C.POINTER(ip)]
+    L.ptc_poisson_search_log.argtypes = [vp, C.c_char_p, ip, C.POINTER(ip)]
     L.ptc_poisson_set.argtypes = [vp, ip, ip]
     L.ptc_poisson_get_knob.argtypes = [vp, ip, C.POINTER(ip)]
     L.ptc_rhs_set.argtypes = [vp, ip, ip]

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "ptc_internal.hpp"
@@ -132,9 +134,17 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     Cand cd;
   };
   std::vector<Timed> timed;
+  c->search_log.clear();
+  auto log = [&](const char *what, const Cand &cd, double ms) {
+    char line[200];
+    std::snprintf(line, sizeof line, "%s tri %d mm %d tile %d slab %d ll %d zf %d lanes %d cl %d ps %d: %.4f ms\n",
+                  what, cd.tri, cd.mm, cd.tile, cd.slab, cd.ll, cd.zf, cd.lanes, cd.cl, cd.ps, ms);
+    c->search_log += line;
+  };
   auto consider = [&](const Cand &cd) {
     double ms = 0;
     if (run(cd, &ms) != PTC_OK) return;
+    log("candidate", cd, ms);
     const double r = rank(cd, ms);
     timed.push_back(Timed{r, cd});
     if (r < best) {
@@ -157,7 +167,10 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     for (int round = 0; round < 2; ++round)
       for (int q = 0; q < n; ++q) {
         double ms = 0;
-        if (run(t[q].cd, &ms) == PTC_OK) fin[q] = std::min(fin[q], rank(t[q].cd, ms));
+        if (run(t[q].cd, &ms) == PTC_OK) {
+          log("re-timed", t[q].cd, ms);
+          fin[q] = std::min(fin[q], rank(t[q].cd, ms));
+        }
       }
     int w = 0;
     for (int q = 1; q < n; ++q)
@@ -189,6 +202,7 @@ static int init_poisson(ptc_ctx *c, int src, int dst) {
     }
   }
   settle();
+  log("picked", pick, best);
   poisson_free_plan(c);
   apply(pick);
   PTC_TRY(poisson_build_plan(c));
@@ -675,6 +689,19 @@ int ptc_poisson_get_config(ptc_ctx *ctx, int *o) {
   o[7] = part ? ctx->pp.tri_lpb : 0;
   o[8] = ctx->pp.slab;
   o[9] = part ? ctx->pp.tri_ll : 0;
+  return PTC_OK;
+}
+
+int ptc_poisson_search_log(ptc_ctx *ctx, char *buf, int cap, int *len) {
+  PTC_CTX(ctx);
+  PTC_CHECK(cap >= 0 && (buf != nullptr || cap == 0), PTC_E_ARG, "bad log buffer");
+  const std::string &s = ctx->search_log;
+  if (len) *len = (int)s.size();
+  if (cap > 0) {
+    const size_t n = std::min(s.size(), (size_t)cap - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  }
   return PTC_OK;
 }
 

@@ -190,6 +190,7 @@ struct ptc_ctx {
   // before a replay when the two differ.
   unsigned long long buf_gen = 1;
   unsigned long long step_gen = 0;       // generation of step_graph
+  std::string search_log;  // the last Poisson plan search (ptc_poisson_search_log)
   // clcRhsXYZ workgroup shape requests (0 = automatic; ptc_rhs_set)
   int rhs_seg_req = 0, rhs_rows_req = 0, rhs_run_req = 0, rhs_minpl_req = 0;
 };

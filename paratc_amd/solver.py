@@ -188,6 +188,14 @@ class Solver:
                 "streaming": self.poisson_get_knob(capi.POISSON_STREAMING),
                 "persist": self.poisson_get_knob(capi.POISSON_PERSIST)}
 
+    def poisson_search_log(self) -> str:
+        """The last InitPoissonSolver search: each candidate's knobs and ms per solve."""
+        n = C.c_int()
+        check(self.L.ptc_poisson_search_log(self.h, None, 0, C.byref(n)), "ptc_poisson_search_log")
+        buf = C.create_string_buffer(n.value + 1)
+        check(self.L.ptc_poisson_search_log(self.h, buf, n.value + 1, None), "ptc_poisson_search_log")
+        return buf.value.decode()
+
     def poisson_residual(self, src: int, dst: int) -> float:
         r = C.c_double()
         check(self.L.ptc_poisson_residual(self.h, src, dst, C.byref(r)), "ptc_poisson_residual")

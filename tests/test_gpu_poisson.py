@@ -260,6 +260,10 @@ def test_init_poisson_solver_autotunes_and_zeroes():
     with Solver(case) as s:
         s.upload(capi.F_ARR1, f, capi.L_INTERIOR)
         s.InitPoissonSolver(capi.F_ARR1, capi.F_HALO)
+        # the search's timings, as the reference prints its choices' times
+        log = s.poisson_search_log().splitlines()
+        assert sum(ln.startswith("candidate ") for ln in log) >= 10, log
+        assert log[-1].startswith("picked ") and log[-1].endswith(" ms"), log[-1]
         # src/m_Poisson.f90:174 -- both arrays are zero afterwards
         assert not s.download(capi.F_ARR1, capi.L_INTERIOR).any()
         assert not s.download(capi.F_HALO, capi.L_INTERIOR).any()
