@@ -502,9 +502,10 @@ __global__ void __launch_bounds__(256) k_tri_poisson_thomas(double2 *__restrict_
 //
 // The matrix I - beta L_y of a job is the same for every line (i, k).  Its
 // chunk elimination factors, the reduced rows and the coefficients of every
-// cyclic-reduction step are formed once per launch by k_cn_factor (one
-// workgroup per job, the same arithmetic chunk_reduce and pcr_row apply to
-// the coefficients), and the solve reads them as wave-uniform scalars: it
+// cyclic-reduction step are formed by k_cn_factor (one workgroup per job,
+// the same arithmetic chunk_reduce and pcr_row apply to the coefficients;
+// once per launch, or once per beta for the tables a step keeps), and the
+// solve reads them as wave-uniform scalars: it
 // keeps only the right-hand side and the old u in registers and does no
 // division.  Table of one job (doubles):
 //   [0, PM)        rr_q  reciprocal pivot of chunk row q = s + r (r >= 1)
