@@ -231,7 +231,7 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
  *                            its z window afresh; automatic 16, halved down
  *                            to 2 on grids that would otherwise give fewer
  *                            than 1.5 workgroups per CU); the chunks are
- *                            otherwise many enough for about 48 wavefronts
+ *                            otherwise many enough for about 96 wavefronts
  *                            per CU
  * A change drops the captured step graph.  Results do not depend on the
  * shape except through the order of the IsUxConst flow-rate sum. */

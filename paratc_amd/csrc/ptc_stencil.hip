@@ -953,9 +953,11 @@ static int pt_nblocks(const ptc_ctx *c) {
 // ms against 17.2 ms for one row of 256 threads, and four rows of 64 threads
 // 16.3 ms (the halo columns cost what the edge rows save); 512-thread
 // workgroups (two rows of 256) fit one per CU and ran 17.3 ms.  The k chunks
-// are many enough for about 48 wavefronts per CU over the launch and at least
-// 16 planes long, or shorter where that leaves the chip short of workgroups
-// (z_chunks).
+// are many enough for about 96 wavefronts per CU over the launch (more rounds
+// of resident workgroups, so the partial last one costs less: the RHS alone
+// 15.3 against 15.4 ms at 4608x140x700 and 3.01 against 3.07 ms at 1536 x 192
+// x 256 with 48) and at least 16 planes long, or shorter where that leaves the
+// chip short of workgroups (z_chunks).
 constexpr int kRhsRowsMaxNx = 512;
 constexpr int kRhsSegThreads = 128;
 constexpr int kRhsXcdRun = 32;
@@ -997,7 +999,7 @@ static RhsShape rhs_shape(const ptc_ctx *c) {
   s.nyb = (c->ny + s.ty - 1) / s.ty;
   const int wpb = s.tx * s.ty / 64;
   const long long tiles = (long long)s.nyb * s.nseg;
-  const int want = (int)((48LL * c->n_cu + tiles * wpb - 1) / (tiles * wpb));
+  const int want = (int)((96LL * c->n_cu + tiles * wpb - 1) / (tiles * wpb));
   s.nch = z_chunks(c->nz, tiles, want, c->rhs_minpl_req ? c->rhs_minpl_req : kRhsMinPlanes, c->rhs_minpl_req == 0,
                    c->n_cu, &s.minpl);
   return s;
