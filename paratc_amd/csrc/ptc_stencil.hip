@@ -1167,13 +1167,15 @@ static BcArgs vel_bc_args(const ptc_ctx *c, int mode) {
   return b;
 }
 
-// z chunks of the velocity/pressure update: enough for about 48 wavefronts per
-// CU over the launch, at least 16 planes each (each chunk re-reads a 3-plane
-// window prologue), shorter on grids with few tiles (z_chunks)
+// z chunks of the velocity/pressure update: enough for about 96 wavefronts per
+// CU over the launch (the velocity update alone 5.34 against 5.39 ms at
+// 4608x140x700 and 0.863 against 0.892 ms at 1536x192x256 with 48), at least
+// 16 planes each (each chunk re-reads a 3-plane window prologue), shorter on
+// grids with few tiles (z_chunks)
 constexpr int kVelXcdRun = 32;
 static dim3 vel_grid(const ptc_ctx *c, int *kc) {
   const int gx = (c->nx + 63) / 64, gy = (c->ny + 3) / 4;
-  const int want = (48 * c->n_cu + gx * gy * 4 - 1) / (gx * gy * 4);
+  const int want = (96 * c->n_cu + gx * gy * 4 - 1) / (gx * gy * 4);
   int minpl = 0;
   const int nch = z_chunks(c->nz, (long long)gx * gy, want, 16, true, c->n_cu, &minpl);
   *kc = (c->nz + nch - 1) / nch;
