@@ -12,13 +12,29 @@ module m_Poisson
 contains
 
   ! InitPoissonSolver (src/m_Poisson.f90:28-175): FFT plans and the timed
-  ! choice of variant; leaves both arrays zero, as the reference does (:174)
+  ! choice of variant; leaves both arrays zero, as the reference does (:174),
+  ! and prints each candidate's time and the pick, as the reference prints
+  ! its choices' (:50-171)
   subroutine InitPoissonSolver(prsrc, prphiHalo)
     real(c_double), dimension(:, :, :), intent(inout), target, contiguous :: prsrc, prphiHalo
-    integer(c_int) :: fs, fd
+    integer(c_int) :: fs, fd, n, i, i0
+    character(kind=c_char) :: none(1)
+    character(kind=c_char), allocatable :: buf(:)
     fs = paratc_out(prsrc, PTC_F_ARR1)
     fd = paratc_out(prphiHalo, PTC_F_HALO)
     call gpu_check(ptc_init_poisson_solver(paratc_ctx(), fs, fd), 'InitPoissonSolver')
+    write(*, *) 'Auto-tuning mode for Poisson Solver......'
+    call gpu_check(ptc_poisson_search_log(paratc_ctx(), none, 0_c_int, n), 'InitPoissonSolver')
+    allocate(buf(n + 1))
+    call gpu_check(ptc_poisson_search_log(paratc_ctx(), buf, n + 1_c_int, n), 'InitPoissonSolver')
+    i0 = 1
+    do i = 1, n
+      if (buf(i) == c_new_line) then
+        write(*, '(A,999A)') '  ', buf(i0:i - 1)
+        i0 = i + 1
+      end if
+    end do
+    deallocate(buf)
   end subroutine InitPoissonSolver
 
   ! clcPPE (src/m_Poisson.f90:16, the solve of :352-452): the interior of

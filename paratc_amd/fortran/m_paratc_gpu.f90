@@ -49,6 +49,7 @@ module m_paratc_gpu
   public :: ptc_clc_u3hat, ptc_clc_scalar, ptc_clc_prsrc, ptc_fluid_vel_update, ptc_pressure_update
   public :: ptc_set_bc_and_update_halo_for_prsrc, ptc_update_uy_ym, ptc_calc_max_cfl, ptc_calc_vmax
   public :: ptc_clc_mean_value, ptc_poisson_config, ptc_poisson_set, ptc_poisson_get_config
+  public :: ptc_poisson_search_log
   public :: ptc_poisson_get_knob, ptc_clc_stat, ptc_get_stat, ptc_visu_field
   public :: ptc_init_spectra, ptc_clc_spectra, ptc_get_spectra
   integer(c_int), parameter, public :: PTC_VISU_UX = 0, PTC_VISU_UY = 1, PTC_VISU_UZ = 2, PTC_VISU_PR = 3
@@ -247,6 +248,14 @@ module m_paratc_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: knob, val
+    end function
+    ! the last plan search, one line per timing (ptc_poisson_search_log)
+    integer(c_int) function ptc_poisson_search_log(ctx, buf, cap, len) bind(C, name="ptc_poisson_search_log")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(out) :: buf
+      integer(c_int), value :: cap
+      integer(c_int), intent(out) :: len
     end function
     integer(c_int) function ptc_poisson_get_knob(ctx, knob, val) bind(C, name="ptc_poisson_get_knob")
       import :: c_int, c_ptr

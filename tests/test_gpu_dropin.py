@@ -48,6 +48,8 @@ def _run_host(tmp_path, case, nsteps, fields):
     r = subprocess.run([exe, str(tmp_path / "run.nml")], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
+    # InitPoissonSolver prints its search, as the reference prints its choices' times
+    assert "Auto-tuning mode for Poisson Solver" in r.stdout and "  picked " in r.stdout, r.stdout
     nf = 5 if case.scalar else 4
     shape = (case.nx, case.ny + 2, case.nz)
     raw = np.fromfile(tmp_path / "fields.out")
