@@ -223,8 +223,9 @@ int ptc_poisson_get_knob(ptc_ctx *ctx, int knob, int *value);
  *                            than 512 points: 64, 128 or 256 (automatic 128)
  *   PTC_RHS_ROWS             rows j per workgroup, at most 512 threads in
  *                            all: the inner rows' j-1 / j+1 values come
- *                            through LDS (automatic: 2 where a row or
- *                            segment is at most 128 threads, else 1)
+ *                            through LDS (automatic: 4 for rows in
+ *                            segments, 2 where a whole row is at most 128
+ *                            threads, else 1)
  *   PTC_RHS_XCD_RUN          consecutive tiles run on one XCD (1 = plain
  *                            order; automatic 32)
  *   PTC_RHS_MIN_PLANES       z planes per chunk at least (each chunk starts

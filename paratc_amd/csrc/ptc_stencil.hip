@@ -952,7 +952,8 @@ static int pt_nblocks(const ptc_ctx *c) {
 // from HBM: at 4608x140x700 (SC) 128-thread segments of two rows ran 16.4-16.7
 // ms against 17.2 ms for one row of 256 threads, and four rows of 64 threads
 // 16.3 ms (the halo columns cost what the edge rows save); 512-thread
-// workgroups (two rows of 256) fit one per CU and ran 17.3 ms.  The k chunks
+// workgroups of two rows of 256 ran 17.3 ms, of four rows of 128 (one per
+// CU, half the edge rows of two) 15.7 against 16.1 ms for two rows of 128.  The k chunks
 // are many enough for about 96 wavefronts per CU over the launch (more rounds
 // of resident workgroups, so the partial last one costs less: the RHS alone
 // 15.3 against 15.4 ms at 4608x140x700 and 3.01 against 3.07 ms at 1536 x 192
@@ -992,7 +993,10 @@ static RhsShape rhs_shape(const ptc_ctx *c) {
     s.tx = c->rhs_seg_req ? c->rhs_seg_req : kRhsSegThreads;
     s.seg = s.tx - 6;
   }
-  const int ty = c->rhs_rows_req ? c->rhs_rows_req : (s.tx <= 128 ? 2 : 1);
+  // rows per workgroup: four for segmented rows (the RHS alone at
+  // 4608x140x700 15.7 against 16.1 ms with two, at 1536x192x256 2.71 against
+  // 2.74), two for whole rows of at most 128 points, else one
+  const int ty = c->rhs_rows_req ? c->rhs_rows_req : (s.seg ? 4 : (s.tx <= 128 ? 2 : 1));
   s.ty = std::max(1, std::min({ty, 512 / s.tx, c->ny}));
   s.run = c->rhs_run_req ? c->rhs_run_req : kRhsXcdRun;
   s.nseg = s.seg ? (c->nx + s.seg - 1) / s.seg : 1;

@@ -95,8 +95,8 @@ def test_clc_rhs_xyz_workgroup_shapes(kind, grid, shape):
 def test_rhs_shape_knobs():
     case = small_channel(1024, 10, 20)
     with Solver(case) as s:
-        # 9 segments x 5 row pairs x 20 planes: the automatic chunks shrink to 2 planes
-        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 2, "xcd_run": 32, "min_planes": 2}
+        # 9 segments x 3 row groups of four x 20 planes: the automatic chunks shrink to 2 planes
+        assert s.rhs_get_shape() == {"segment_threads": 128, "rows": 4, "xcd_run": 32, "min_planes": 2}
         s.rhs_shape(256, 3, 1, 4)
         # at most 512 threads per workgroup
         assert s.rhs_get_shape() == {"segment_threads": 256, "rows": 2, "xcd_run": 1, "min_planes": 4}
