@@ -369,6 +369,11 @@ int capture_step(ptc_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec) {
   c->dt = save[6];
   c->simtime = save[7];
   c->prgradaver = save[8];
+  if (rc != PTC_OK || ec != hipSuccess) {
+    // factor launches a failed capture recorded never ran: forget what the
+    // kept Crank-Nicolson tables were marked as holding
+    for (double &b : c->cn_tab_beta) b = NAN;
+  }
   if (rc != PTC_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
