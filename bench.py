@@ -52,7 +52,8 @@ def stage_kernels(plan: dict, nx: int, nz: int) -> list:
     if plan["x_fast"] and persist & 1:
         xf, xi = f"k2p_fft_x_fwd<{h}, {d}, 2u>, {st}>", f"k2p_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
     elif plan["x_fast"]:
-        xf, xi = f"k2_fft_x_fwd<{h}, {d}, 2u>, {st}>", f"k2_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
+        # the one-shot forward pass carries its source form (0: the plain field)
+        xf, xi = f"k2_fft_x_fwd<{h}, {d}, 2u>, {st}, 0>", f"k2_fft_x_inv<{h}, {d}, 2u>, false, {st}>"
     else:
         xf, xi = "k_fft_x_fwd<", "k_fft_x_inv<"
     if plan["z_fast"] and persist & 2:

@@ -41,3 +41,22 @@ def test_read_value_and_read_traffic(tmp_path):
     assert profsum.read_value(str(md), "missing") is None
     assert profsum.read_value(str(tmp_path / "none.md"), "step_bytes_counted") is None
     assert profsum.read_traffic(str(md), "k_a<1>") == 50.0
+
+
+def test_bench_finds_the_traffic_of_every_pass_the_search_can_pick():
+    """bench.py's roofline.traffic looks the dominant pass's kernel up in the
+    committed 1024x256x512 traffic summary by name: every pass of the plans
+    the search picks there (8- or 16-column z tiles, one-shot or persistent x
+    passes, the complex-line y-solve) has its row."""
+    import os
+    import bench
+
+    path = os.path.join(bench.ROOT, "profiles", "poisson_1024x256x512_hbm_traffic.md")
+    for z_tile in (8, 16):
+        for persist in (0, 1):
+            plan = {"x_fast": True, "z_fast": True, "z_tile": z_tile, "persist": persist, "streaming": 1,
+                    "tri": "partitioned", "chunk_rows": 8, "tri_complex_lines": 16, "tri_lds_lines": 0,
+                    "lanes_per_block": 16, "slab_planes": 0}
+            for kernel in bench.stage_kernels(plan, 1024, 512):
+                t = profsum.read_traffic(path, kernel)
+                assert t is not None and 2.1e9 < t < 2.7e9, (plan, kernel, t)
