@@ -9,9 +9,10 @@
         per-kernel shares of wave time from one --pmc run of SQ_WAVE_CYCLES,
         SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_WAIT_INST_LDS, SQ_ACTIVE_INST_VALU and
         SQ_ACTIVE_INST_LDS (with SQ_WAVES and SQ_BUSY_CYCLES)
-    python -m paratc_amd.profsum totals <fetch.csv> <write.csv> <copy_bytes>
+    python -m paratc_amd.profsum totals <fetch.csv> <write.csv> <copy_bytes> [--max-grid N] [--skip S ...]
         HBM bytes summed over every dispatch of the run (k_copy excluded),
-        scaled as in the traffic table
+        scaled as in the traffic table; --max-grid leaves out launches of more
+        than N threads and --skip kernels whose name contains S
     python -m paratc_amd.profsum traffic <fetch.csv> <write.csv> <copy_bytes>
         per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
         --pmc runs, scaled to HBM bytes by what the same counters read for
@@ -133,28 +134,35 @@ def traffic_table(fetch_csv: str, write_csv: str, nbytes: float) -> str:
     return "\n".join(out)
 
 
-def _counter_sums(path: str) -> Tuple[Dict[str, float], Dict[str, int]]:
-    """Summed counter value and dispatch count per kernel name."""
+def _counter_sums(path: str, max_grid: Optional[int] = None,
+                  skip: Tuple[str, ...] = ()) -> Tuple[Dict[str, float], Dict[str, int]]:
+    """Summed counter value and dispatch count per kernel name, leaving out
+    launches of more than max_grid threads and names that contain a skip
+    string."""
     tot: Dict[str, float] = collections.defaultdict(float)
     cnt: Dict[str, int] = collections.defaultdict(int)
     with open(path) as fh:
         for r in csv.DictReader(fh):
             k = short_name(r["Kernel_Name"])
+            if (max_grid is not None and int(r["Grid_Size"]) > max_grid) or any(s in k for s in skip):
+                continue
             tot[k] += float(r["Counter_Value"])
             cnt[k] += 1
     return tot, cnt
 
 
-def totals(fetch_csv: str, write_csv: str, nbytes: float) -> Tuple[int, float, float]:
-    """(dispatches, read bytes, write bytes) over the whole run, k_copy left
-    out; the copy's own counters give the scale factors."""
+def totals(fetch_csv: str, write_csv: str, nbytes: float, max_grid: Optional[int] = None,
+           skip: Tuple[str, ...] = ()) -> Tuple[int, float, float]:
+    """(dispatches, read bytes, write bytes) over the run, k_copy left out
+    (and launches past max_grid threads or with a skip string in the name);
+    the copy's own counters give the scale factors."""
     fetch, write = _counter_means(fetch_csv), _counter_means(write_csv)
     copy = [k for k in fetch if k.startswith("k_copy @") or k.startswith("k_copy_nt @")]
     if not copy or copy[0] not in write:
         raise SystemExit("k_copy is not in the counter files: profile bench.py --calib-copy")
     rf = nbytes / (fetch[copy[0]][1] * 1024.0)
     wf = nbytes / (write[copy[0]][1] * 1024.0)
-    (fs, fc), (ws, _) = _counter_sums(fetch_csv), _counter_sums(write_csv)
+    (fs, fc), (ws, _) = _counter_sums(fetch_csv, max_grid, skip), _counter_sums(write_csv, max_grid, skip)
     keep = [k for k in fs if not k.startswith("k_copy")]
     n = sum(fc[k] for k in keep)
     return n, sum(fs[k] for k in keep) * 1024.0 * rf, sum(ws.get(k, 0.0) for k in keep) * 1024.0 * wf
@@ -198,7 +206,15 @@ def main(argv: List[str]) -> None:
     elif len(argv) >= 2 and argv[0] == "sq":
         print(sq_table(argv[1]))
     elif len(argv) >= 4 and argv[0] == "totals":
-        n, rb, wb = totals(argv[1], argv[2], float(argv[3]))
+        max_grid, skip, rest = None, [], argv[4:]
+        while rest:
+            if rest[0] == "--max-grid" and len(rest) > 1:
+                max_grid, rest = int(rest[1]), rest[2:]
+            elif rest[0] == "--skip" and len(rest) > 1:
+                skip, rest = skip + [rest[1]], rest[2:]
+            else:
+                raise SystemExit(__doc__)
+        n, rb, wb = totals(argv[1], argv[2], float(argv[3]), max_grid, tuple(skip))
         print(f"dispatches {n}")
         print(f"read_bytes {rb:.0f}")
         print(f"write_bytes {wb:.0f}")

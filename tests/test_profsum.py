@@ -32,6 +32,18 @@ def test_totals_scale_by_the_copy_and_leave_it_out(tmp_path):
     assert wb == (4.0 + 4.0 + 1.0) * 1024.0 * 1.0
 
 
+def test_totals_leave_out_large_launches_and_named_kernels(tmp_path):
+    rows = [("k_copy_nt(double*)", 64, 512.0), ("k_a(double*)", 64, 10.0), ("k_a(double*)", 5000, 30.0),
+            ("k_b<512>(double*)", 128, 5.0), ("k_c<64>(double*)", 128, 7.0)]
+    f, w = tmp_path / "f.csv", tmp_path / "w.csv"
+    _pmc_csv(f, rows, "FETCH_SIZE")
+    _pmc_csv(w, [(n, g, 2.0 * v) for n, g, v in rows], "WRITE_SIZE")
+    n, rb, wb = profsum.totals(str(f), str(w), float(1 << 20), max_grid=1000, skip=("<512",))
+    assert n == 2
+    assert rb == (10.0 + 7.0) * 1024.0 * 2.0
+    assert wb == (20.0 + 14.0) * 1024.0 * 1.0
+
+
 def test_read_value_and_read_traffic(tmp_path):
     md = tmp_path / "s.md"
     md.write_text("# title\n\nstep_bytes_counted 1234.5\nprose step_bytes_counted 9 more\n\n"
