@@ -377,9 +377,6 @@ def c0_channel():
     return CS.case_channel(128, 96, 128, 1.8)
 
 
-NORTH_STAR_BAR = 1e-12  # relative field error after one step
-
-
 def check_fields(label, nsteps, out, ref, tol, spread):
     """Every field within tol of the oracle.  p may carry the spread of the
     reference's own Poisson variants: its bar is max(tol, 2 spread) capped at
@@ -483,8 +480,8 @@ def test_channel_iterate_fp32_refine(nsteps):
     """The precision form inside the time step: every Poisson solve of
     ChannelIterate with FP32 spectra and FP64 defect correction (two sweeps),
     on the gravity current's physics (Re 5000, Ri 1, ws 0.02), against the
-    FP64 oracle.  The per-field error is printed; the bar is the north-star
-    form's 1e-9 against FP64."""
+    FP64 oracle.  The per-field error is printed; the bar is the precision
+    form's accuracy target, 1e-9 against FP64."""
     case = variant_case("fp32_refine")
 
     def plan(s):
